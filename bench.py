@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPE time-stepping hot path on one MI355X.
+
+Workload (the headline metric's configuration): 512^3 periodic grid,
+dx = 0.3, 4th-order centred FD, eqn 1 (i/2(lap + (1-|psi|^2)psi)), classic
+RK4 with dt = 0.005, random-phase turbulence IC (Gaussian shell spectrum,
+k0 = 1, sigma = 0.2, seed 0, mean |psi|^2 = 1), complex128 fields.  A step
+is one RK4 step of the whole grid (4 fused stencil+stage kernels); inputs
+are resident in HBM when the timed region starts.
+
+Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
+dominant kernel (RK4 stages 2/3, timed live with HIP events around each
+launch on the context's stream), the complex64-storage run, and a CPU
+baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
+workload: --cpu-steps RK4 steps of the same grid and IC).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as ct
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+class Hip:
+    """HIP events through the runtime's C API (the kernels run on the
+    context's own stream, so torch's current-stream events would not see them)."""
+
+    def __init__(self):
+        self.h = ct.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+
+    def event(self):
+        e = ct.c_void_p()
+        assert self.h.hipEventCreate(ct.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ct.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        assert self.h.hipEventSynchronize(b) == 0
+        f = ct.c_float()
+        assert self.h.hipEventElapsedTime(ct.byref(f), a, b) == 0
+        return f.value
+
+    def device_sync(self):
+        assert self.h.hipDeviceSynchronize() == 0
+
+
+def timed_steps(ctx, hip, scheme, steps, info):
+    """K steps between two events on the context's stream, bracketed by device syncs."""
+    hip.device_sync()
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, ctx.stream)
+    ctx.step(scheme, steps, info)
+    hip.record(e1, ctx.stream)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.device_sync()
+    return ms
+
+
+def kernel_profile(ctx, scheme, steps, info):
+    """Per-kind launch count, average duration (ms) and algorithmic bytes per launch."""
+    lib = ctx.lib
+    assert lib.gpe_prof_enable(ctx.h, 1) == 0
+    ctx.step(scheme, steps, info)
+    out = {}
+    for k in range(14):
+        ms, n, b = ct.c_double(), ct.c_int64(), ct.c_double()
+        assert lib.gpe_prof_read(ctx.h, k, ct.byref(ms), ct.byref(n), ct.byref(b)) == 0
+        if n.value:
+            out[lib.gpe_kernel_kind_name(k).decode()] = {"launches": n.value, "avg_ms": ms.value / n.value,
+                                                         "bytes": b.value}
+    assert lib.gpe_prof_enable(ctx.h, 0) == 0
+    return out
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+    committed under profiles/ as JSON; None if not measured."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def run_storage(p, psi, args, hip):
+    from gpe_amd import Context, StepInfo
+
+    with Context(p) as ctx:
+        if args.tiles:
+            ctx.set_tiles(*[int(v) for v in args.tiles.split(",")])
+        ctx.upload(psi)
+        info = StepInfo(p.dt0())
+        ctx.step("rk4", args.warmup, info)
+        ms = timed_steps(ctx, hip, "rk4", args.steps, info)
+        prof = kernel_profile(ctx, "rk4", args.steps, info)
+        tiles = ctx.tiles()
+    return ms, prof, tiles
+
+
+def roofline(prof, kind, traffic=None):
+    k = prof[kind]
+    achieved = k["bytes"] / (k["avg_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": kind, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_launch_ms": round(k["avg_ms"], 5),
+            "algorithmic_bytes_per_launch": k["bytes"]}
+
+
+def cpu_baseline(p, psi, steps):
+    """The FP64 oracle (plain C restatement of the reference, OpenMP) on
+    `steps` RK4 steps of the same workload."""
+    from oracle import oracle as orc
+
+    orc.build()
+    t0 = time.perf_counter()
+    orc.rk4_n(p, psi, p.dt0(), steps)
+    s = time.perf_counter() - t0
+    return {"value": round(p.cells * steps / s / 1e9, 6), "unit": "Gcell-steps/s", "cores": orc.threads(),
+            "kind": "port", "sample": f"{steps} RK4 step(s) of the same {p.nx}^3 workload ({s:.1f} s)"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="grid points per axis (512 = the benchmark workload)")
+    ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path would shard as z-slabs with 2-plane halos, not built)", file=sys.stderr)
+        return 2
+
+    from gpe_amd import ics
+
+    t0 = time.perf_counter()
+    p, psi = ics.config_turb(n=args.n, storage="c128", seed=0, scheme="rk4", tau=0.005)
+    print(f"[bench] IC {args.n}^3 built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    hip = Hip()
+
+    ms, prof, tiles = run_storage(p, psi, args, hip)
+    print(f"[bench] c128: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    value = p.cells * args.steps / (ms * 1e-3) / 1e9
+    step_bytes = sum(v["bytes"] for k, v in prof.items() if k.startswith("rk4")) + prof["rk4_stage2_3"]["bytes"]
+    rf = roofline(prof, "rk4_stage2_3", load_traffic(f"traffic_rk4_stage2_3_c128_{args.n}.json"))
+    result = {
+        "metric": "RK4 Gcell-steps/s + %HBM peak at 512³; time-to-T, learned vs plain RKF45",
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"random-phase turbulence {args.n}^3 periodic, dx=0.3, 4th-order FD, eqn 1, "
+                               "RK4 dt=0.005, complex128 fields",
+                   "grid": [args.n] * 3, "order": 4, "bcs": "periodic", "scheme": "rk4", "dt": 0.005,
+                   "storage": "c128", "tiles": list(tiles)},
+        "roofline": rf,
+        "step_hbm_frac": round(step_bytes * args.steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
+                        "GB_s": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k, v in prof.items()},
+    }
+    if not args.no_c64:
+        p64 = p.replace(storage="c64")
+        ms64, prof64, _ = run_storage(p64, psi, args, hip)
+        print(f"[bench] c64: {ms64 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        result["c64"] = {"value": round(p.cells * args.steps / (ms64 * 1e-3) / 1e9, 4),
+                         "ms_per_step": round(ms64 / args.steps, 4), "dtype": "f32 stencil, f64 reductions",
+                         "roofline": roofline(prof64, "rk4_stage2_3",
+                                              load_traffic(f"traffic_rk4_stage2_3_c64_{args.n}.json"))}
+    if args.cpu_steps > 0:
+        result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,103 @@
+"""ctypes binding of libgpehip.so (the C ABI declared in include/gpe_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+this module raises, it never falls back to another implementation.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpehip.so")
+
+GPE_OK = 0
+GPE_C128, GPE_C64 = 0, 1
+GPE_HOST_C128, GPE_HOST_C64 = 0, 1
+SCHEMES = {"euler": 0, "rk2": 1, "rk4": 2, "rk45": 3}
+DERIV_OPS = {"x": 0, "y": 1, "z": 2, "xx": 3, "yy": 4, "zz": 5, "lap": 6}
+
+
+class GpeError(RuntimeError):
+    """A failed libgpehip call (the reference stops with emergency_stop)."""
+
+
+class gpe_params(ct.Structure):
+    _fields_ = [
+        ("nx", ct.c_int32), ("ny", ct.c_int32), ("nz", ct.c_int32),
+        ("order", ct.c_int32), ("bcs", ct.c_int32), ("eqn_to_solve", ct.c_int32),
+        ("real_time", ct.c_int32), ("storage", ct.c_int32),
+        ("xr", ct.c_double), ("yr", ct.c_double), ("zr", ct.c_double),
+        ("Urhs", ct.c_double), ("diss_amp", ct.c_double), ("g", ct.c_double), ("mu", ct.c_double),
+        ("nn", ct.c_double), ("omx", ct.c_double), ("omy", ct.c_double), ("omz", ct.c_double),
+        ("vscale", ct.c_double),
+        ("eps", ct.c_double), ("safety", ct.c_double), ("dt_decrease", ct.c_double), ("dt_increase", ct.c_double),
+    ]
+
+
+class gpe_step_info(ct.Structure):
+    _fields_ = [
+        ("t", ct.c_double), ("im_t", ct.c_double), ("dt_re", ct.c_double), ("dt_im", ct.c_double),
+        ("errmax", ct.c_double), ("psi2max", ct.c_double),
+        ("n_accept", ct.c_int64), ("n_reject", ct.c_int64), ("n_rhs", ct.c_int64),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point of include/gpe_hip.h
+_VP, _I64, _D, _I = ct.c_void_p, ct.c_int64, ct.c_double, ct.c_int
+PROTOTYPES = [
+    ("gpe_abi_version", _I, []),
+    ("gpe_last_error", ct.c_char_p, [_VP]),
+    ("gpe_ctx_create", _I, [ct.POINTER(gpe_params), ct.POINTER(_VP)]),
+    ("gpe_ctx_destroy", None, [_VP]),
+    ("gpe_set_tiles", _I, [_VP, _I, _I, _I]),
+    ("gpe_get_tiles", _I, [_VP, ct.POINTER(_I), ct.POINTER(_I), ct.POINTER(_I)]),
+    ("gpe_upload", _I, [_VP, _VP, _I, _I64, _I64, _I64]),
+    ("gpe_download", _I, [_VP, _VP, _I]),
+    ("gpe_step", _I, [_VP, _I, _I, ct.POINTER(gpe_step_info)]),
+    ("gpe_solver", _I, [_VP, _I, _VP, _I64, _I64, _I64, _VP, _I, ct.POINTER(gpe_step_info)]),
+    ("gpe_rk45_attempt", _I, [_VP, _D, _D, _D, _D, ct.POINTER(_D)]),
+    ("gpe_rk45_accept", _I, [_VP]),
+    ("gpe_deriv", _I, [_VP, _I, _VP, _I64, _I64, _I64, _VP, _I]),
+    ("gpe_rhs", _I, [_VP, _VP, _I64, _I64, _I64, _VP, _I]),
+    ("gpe_norm", _I, [_VP, ct.POINTER(_D)]),
+    ("gpe_energy", _I, [_VP, ct.POINTER(_D)]),
+    ("gpe_mass", _I, [_VP, ct.POINTER(_D)]),
+    ("gpe_renormalise", _I, [_VP, _D]),
+    ("gpe_prof_enable", _I, [_VP, _I]),
+    ("gpe_prof_read", _I, [_VP, _I, ct.POINTER(_D), ct.POINTER(_I64), ct.POINTER(_D)]),
+    ("gpe_kernel_kind_name", ct.c_char_p, [_I]),
+    ("gpe_bench_copy", _I, [_VP, _VP, _I64, _VP]),
+    ("gpe_state_ptr", _VP, [_VP]),
+    ("gpe_stream", _VP, [_VP]),
+    ("gpe_sync", _I, [_VP]),
+]
+
+_lib = None
+
+
+def load(path: str | None = None) -> ct.CDLL:
+    """Load libgpehip.so once and attach prototypes.  Raises GpeError if the
+    library has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise GpeError(f"{p} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                       "or `make -C gpe_amd/csrc`")
+    lib = ct.CDLL(p)
+    for name, res, args in PROTOTYPES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    if rc != GPE_OK:
+        lib = load()
+        msg = lib.gpe_last_error(ctx)
+        raise GpeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

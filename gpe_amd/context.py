@@ -1,0 +1,182 @@
+"""Device-resident solver context over libgpehip.so.
+
+`Context` owns one whole-domain field on the GPU and advances it with the
+HIP kernels; host arrays cross the boundary only in upload/download and the
+host-array drop-ins.  Fields are numpy arrays of shape (nz, ny, nx) in C
+order (x fastest), complex128 or complex64.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import _lib
+from .params import RunParams
+
+
+def _host_dtype(a: np.ndarray) -> int:
+    if a.dtype == np.complex128:
+        return _lib.GPE_HOST_C128
+    if a.dtype == np.complex64:
+        return _lib.GPE_HOST_C64
+    raise TypeError(f"field must be complex128 or complex64, got {a.dtype}")
+
+
+class StepInfo:
+    """Python view of gpe_step_info (t, im_t, dt and counters)."""
+
+    def __init__(self, dt: complex = 0j, t: float = 0.0, im_t: float = 0.0):
+        self.c = _lib.gpe_step_info(t, im_t, dt.real, dt.imag, 0.0, 0.0, 0, 0, 0)
+
+    @property
+    def t(self) -> float:
+        return self.c.t
+
+    @property
+    def im_t(self) -> float:
+        return self.c.im_t
+
+    @property
+    def dt(self) -> complex:
+        return complex(self.c.dt_re, self.c.dt_im)
+
+    @dt.setter
+    def dt(self, v: complex) -> None:
+        self.c.dt_re, self.c.dt_im = v.real, v.imag
+
+    def __getattr__(self, k):
+        if k in ("errmax", "psi2max", "n_accept", "n_reject", "n_rhs"):
+            return getattr(self.c, k)
+        raise AttributeError(k)
+
+
+class Context:
+    def __init__(self, params: RunParams):
+        self.lib = _lib.load()
+        self.params = params
+        self._cp = params.to_c()
+        h = ct.c_void_p()
+        _lib.check(self.lib.gpe_ctx_create(ct.byref(self._cp), ct.byref(h)), None, "gpe_ctx_create")
+        self.h = h
+        self.storage_dtype = np.complex128 if params.storage == "c128" else np.complex64
+
+    # ---- lifetime
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.gpe_ctx_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc: int, what: str) -> None:
+        _lib.check(rc, self.h, what)
+
+    # ---- configuration
+    def set_tiles(self, tx: int, ty: int, tz: int = 0) -> None:
+        self._chk(self.lib.gpe_set_tiles(self.h, tx, ty, tz), "gpe_set_tiles")
+
+    def tiles(self):
+        a, b, c = ct.c_int(), ct.c_int(), ct.c_int()
+        self._chk(self.lib.gpe_get_tiles(self.h, ct.byref(a), ct.byref(b), ct.byref(c)), "gpe_get_tiles")
+        return a.value, b.value, c.value
+
+    # ---- transfers
+    def _layout(self, a: np.ndarray):
+        p = self.params
+        if not a.flags.c_contiguous:
+            raise ValueError("field must be C-contiguous (x fastest)")
+        if a.shape == p.shape:
+            return p.ny, 0, 0
+        if a.shape == (p.nz + 4, p.ny + 4, p.nx):
+            return p.ny + 4, 2, 2  # the reference's halo array (0:nx1, js-2:je+2, ks-2:ke+2)
+        raise ValueError(f"field shape {a.shape} is neither {p.shape} nor the halo shape")
+
+    def upload(self, a: np.ndarray) -> None:
+        nya, yo, zo = self._layout(a)
+        self._chk(self.lib.gpe_upload(self.h, a.ctypes.data, _host_dtype(a), nya, yo, zo), "gpe_upload")
+
+    def download(self, dtype=np.complex128) -> np.ndarray:
+        out = np.empty(self.params.shape, dtype=dtype)
+        self._chk(self.lib.gpe_download(self.h, out.ctypes.data, _host_dtype(out)), "gpe_download")
+        return out
+
+    # ---- stepping
+    def step(self, scheme: str, nsteps: int, info: StepInfo) -> StepInfo:
+        if scheme not in _lib.SCHEMES:
+            raise _lib.GpeError("ERROR: Unrecognised time stepping scheme.")
+        self._chk(self.lib.gpe_step(self.h, _lib.SCHEMES[scheme], nsteps, ct.byref(info.c)), "gpe_step")
+        return info
+
+    def rk45_attempt(self, dt: complex, dt0: complex):
+        out = (ct.c_double * 2)()
+        self._chk(self.lib.gpe_rk45_attempt(self.h, dt.real, dt.imag, dt0.real, dt0.imag, out), "gpe_rk45_attempt")
+        return out[0], out[1]
+
+    def rk45_accept(self) -> None:
+        self._chk(self.lib.gpe_rk45_accept(self.h), "gpe_rk45_accept")
+
+    # ---- operators on host arrays (or on the state when a is None)
+    def deriv(self, op: str, a: np.ndarray | None = None) -> np.ndarray:
+        dt = a.dtype if a is not None else np.complex128
+        out = np.empty(self.params.shape, dtype=dt)
+        if a is None:
+            rc = self.lib.gpe_deriv(self.h, _lib.DERIV_OPS[op], None, 0, 0, 0, out.ctypes.data, _host_dtype(out))
+        else:
+            nya, yo, zo = self._layout(a)
+            rc = self.lib.gpe_deriv(self.h, _lib.DERIV_OPS[op], a.ctypes.data, nya, yo, zo, out.ctypes.data,
+                                    _host_dtype(a))
+        self._chk(rc, "gpe_deriv")
+        return out
+
+    def rhs(self, a: np.ndarray | None = None) -> np.ndarray:
+        dt = a.dtype if a is not None else np.complex128
+        out = np.empty(self.params.shape, dtype=dt)
+        if a is None:
+            rc = self.lib.gpe_rhs(self.h, None, 0, 0, 0, out.ctypes.data, _host_dtype(out))
+        else:
+            nya, yo, zo = self._layout(a)
+            rc = self.lib.gpe_rhs(self.h, a.ctypes.data, nya, yo, zo, out.ctypes.data, _host_dtype(a))
+        self._chk(rc, "gpe_rhs")
+        return out
+
+    # ---- reductions of the state
+    def norm(self) -> float:
+        v = ct.c_double()
+        self._chk(self.lib.gpe_norm(self.h, ct.byref(v)), "gpe_norm")
+        return v.value
+
+    def energy(self) -> float:
+        v = ct.c_double()
+        self._chk(self.lib.gpe_energy(self.h, ct.byref(v)), "gpe_energy")
+        return v.value
+
+    def mass(self) -> float:
+        v = ct.c_double()
+        self._chk(self.lib.gpe_mass(self.h, ct.byref(v)), "gpe_mass")
+        return v.value
+
+    def renormalise(self, norm: float) -> None:
+        self._chk(self.lib.gpe_renormalise(self.h, norm), "gpe_renormalise")
+
+    # ---- raw handles
+    @property
+    def stream(self) -> int:
+        return self.lib.gpe_stream(self.h)
+
+    @property
+    def state_ptr(self) -> int:
+        return self.lib.gpe_state_ptr(self.h)
+
+    def sync(self) -> None:
+        self._chk(self.lib.gpe_sync(self.h), "gpe_sync")

@@ -1,0 +1,820 @@
+// C ABI of libgpehip.so (declared in include/gpe_hip.h): context, device
+// memory, the stage sequences of euler/rk2/rk4/rkck, the rkqs controller,
+// reductions and the host-array drop-ins of the reference's module routines.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/gpe_hip.h"
+#include "gpe_launch.h"
+
+using namespace gpe;
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+}  // namespace
+
+struct gpe_ctx {
+    gpe_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    long long n = 0;   // cells
+    size_t esz = 0;    // bytes per complex value
+    void* psi = nullptr;
+    // RK temporaries
+    void* uA = nullptr;
+    void* uB = nullptr;
+    void* acc = nullptr;
+    // Cash-Karp
+    void* f0 = nullptr;
+    void* k2 = nullptr;
+    void* k3 = nullptr;
+    void* k4 = nullptr;
+    void* k5 = nullptr;
+    void* pnew = nullptr;
+    bool f0_valid = false;
+    bool cand_valid = false;
+    // drop-in scratch
+    void* sin = nullptr;
+    void* sout = nullptr;
+    void* stage = nullptr;  // host-dtype staging for conversions
+    size_t stage_bytes = 0;
+    // reductions
+    double* partial = nullptr;
+    size_t partial_cap = 0;  // doubles
+    double* dres = nullptr;
+    double* hres = nullptr;  // pinned, 4 doubles
+    void* dtab = nullptr;    // dissipation tables (storage precision)
+    Tiles tiles{64, 8, 0};
+    std::string err;
+    // per-launch HIP event timing (gpe_prof_enable)
+    struct ProfRec {
+        int kind;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    bool prof = false;
+    std::vector<ProfRec> recs;
+};
+
+static const char* const kKindNames[GPE_NKINDS] = {
+    "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2_3", "rk4_stage4", "rkck_stage1",
+    "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy"};
+
+static int fail(gpe_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    g_thread_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                                          \
+    do {                                                                                                           \
+        hipError_t e_ = (expr);                                                                                    \
+        if (e_ != hipSuccess)                                                                                      \
+            return fail(ctx, GPE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+static int dalloc(gpe_ctx* c, void** p, size_t bytes, const char* what) {
+    if (*p) return 0;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, GPE_ENOMEM, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) + " B): " +
+                                       hipGetErrorString(e));
+    }
+    return 0;
+}
+
+static size_t field_bytes(const gpe_ctx* c) { return (size_t)c->n * c->esz; }
+
+static int validate(const gpe_params* p, std::string& why) {
+    if (!p) { why = "params is NULL"; return GPE_EINVAL; }
+    const int mn = p->order == 4 ? 6 : 6;
+    if (p->nx < mn || p->ny < mn || p->nz < mn) { why = "nx, ny, nz must be >= 6"; return GPE_EINVAL; }
+    if ((long long)p->nx * p->ny > (1LL << 31) - 1) { why = "nx*ny too large"; return GPE_EINVAL; }
+    if (p->order != 2 && p->order != 4) { why = "order must be 2 or 4"; return GPE_EINVAL; }
+    if (p->bcs != 1 && p->bcs != 2) { why = "bcs must be 1 (periodic) or 2 (reflective)"; return GPE_EINVAL; }
+    if (p->eqn_to_solve < 1 || p->eqn_to_solve > 4) { why = "eqn_to_solve must be 1..4"; return GPE_EINVAL; }
+    if (p->storage != GPE_C128 && p->storage != GPE_C64) { why = "storage must be GPE_C128 or GPE_C64"; return GPE_EINVAL; }
+    if (!(p->xr > 0) || !(p->yr > 0) || !(p->zr > 0)) { why = "xr, yr, zr must be > 0"; return GPE_EINVAL; }
+    if (!(p->eps > 0) || !(p->safety > 0)) { why = "eps and safety must be > 0"; return GPE_EINVAL; }
+    return 0;
+}
+
+static Tiles default_tiles(const gpe_params& p) {
+    Tiles t{64, 8, 0};
+    if (p.nx <= 32) t.tx = 32;
+    const int ntxy = ((p.nx + t.tx - 1) / t.tx) * ((p.ny + t.ty - 1) / t.ty);
+    // aim for ~2048 blocks while keeping the z-halo re-read small
+    long long tz = ((long long)p.nz * ntxy) / 2048;
+    if (tz < 8) tz = 8;
+    if (tz > p.nz) tz = p.nz;
+    t.tz = (int)tz;
+    return t;
+}
+
+template <typename R> static Geom make_geom(const gpe_ctx* c) {
+    Geom g{};
+    g.nx = c->p.nx;
+    g.ny = c->p.ny;
+    g.nz = c->p.nz;
+    g.bcs = c->p.bcs;
+    g.plane = (long long)c->p.nx * c->p.ny;
+    return g;
+}
+
+static void spacing(const gpe_params& p, double d[3]) {
+    // dx = (xr - xl)/nx1 with xl = -xr (src/ic.f90:43-46)
+    d[0] = (p.xr + p.xr) / (p.nx - 1);
+    d[1] = (p.yr + p.yr) / (p.ny - 1);
+    d[2] = (p.zr + p.zr) / (p.nz - 1);
+}
+
+template <typename R> static StW<R> make_w(const gpe_params& p) {
+    double d[3];
+    spacing(p, d);
+    StW<R> w;
+    if (p.order == 2) {
+        w.w1x = (R)(1.0 / (2.0 * d[0])); w.w1y = (R)(1.0 / (2.0 * d[1])); w.w1z = (R)(1.0 / (2.0 * d[2]));
+        w.w2x = (R)(1.0 / (d[0] * d[0])); w.w2y = (R)(1.0 / (d[1] * d[1])); w.w2z = (R)(1.0 / (d[2] * d[2]));
+        w.w2c = (R)(2.0 * (1.0 / (d[0] * d[0]) + 1.0 / (d[1] * d[1]) + 1.0 / (d[2] * d[2])));
+    } else {
+        w.w1x = (R)(1.0 / (12.0 * d[0])); w.w1y = (R)(1.0 / (12.0 * d[1])); w.w1z = (R)(1.0 / (12.0 * d[2]));
+        w.w2x = (R)(1.0 / (12.0 * d[0] * d[0])); w.w2y = (R)(1.0 / (12.0 * d[1] * d[1]));
+        w.w2z = (R)(1.0 / (12.0 * d[2] * d[2]));
+        w.w2c = (R)(30.0 * (1.0 / (12.0 * d[0] * d[0]) + 1.0 / (12.0 * d[1] * d[1]) + 1.0 / (12.0 * d[2] * d[2])));
+    }
+    return w;
+}
+
+template <typename R> static RhsC<R> make_rhs(const gpe_ctx* c) {
+    const gpe_params& p = c->p;
+    double d[3];
+    spacing(p, d);
+    RhsC<R> k{};
+    int flags = 0;
+    switch (p.eqn_to_solve) {
+        case 1:
+        case 3:
+            // 0.5*(i + diss)*(lap + (1 - |psi|^2 [- vscale V]) psi) + U dpsi/dx  (src/solve.f90:415-418, 425-428)
+            k.a = 1; k.b = 1; k.c = -1; k.d = (R)(-p.vscale); k.pim = (R)0.5;
+            if (p.vscale != 0.0) flags |= RHS_V;
+            if (p.diss_amp != 0.0) flags |= RHS_DISS;
+            if (p.Urhs != 0.0) flags |= RHS_U;
+            if (p.eqn_to_solve == 3) flags |= RHS_MASK;
+            break;
+        case 2:
+            // i*(lap - |psi|^2 psi) + U dpsi/dx  (src/solve.f90:420-423)
+            k.a = 1; k.b = 0; k.c = -1; k.d = 0; k.pim = 1;
+            if (p.Urhs != 0.0) flags |= RHS_U;
+            break;
+        default:
+            // -i*(-0.5 lap + V psi + g|psi|^2 psi - mu psi)  (src/solve.f90:430-436)
+            k.a = (R)-0.5; k.b = (R)(-p.mu); k.c = (R)p.g; k.d = 1; k.pim = -1;
+            flags |= RHS_V;
+            break;
+    }
+    k.U = (R)p.Urhs;
+    k.vx = (R)(0.5 * p.omx * p.omx);
+    k.vy = (R)(0.5 * p.omy * p.omy);
+    k.vz = (R)(0.5 * p.omz * p.omz);
+    k.xl = (R)(-p.xr); k.yl = (R)(-p.yr); k.zl = (R)(-p.zr);
+    k.dx = (R)d[0]; k.dy = (R)d[1]; k.dz = (R)d[2];
+    k.diss_amp = (R)p.diss_amp;
+    k.dtab = (const R*)c->dtab;
+    k.nx = p.nx;
+    k.ny = p.ny;
+    k.flags = flags;
+    return k;
+}
+
+template <typename R> static C<R> cdt(double re, double im, double s) { return mkc<R>((R)(re * s), (R)(im * s)); }
+
+template <typename R> static CkTab<R> make_tab() {
+    // Cash-Karp coefficients (src/solve.f90:255-281)
+    const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0, c6 = 512.0 / 1771.0;
+    CkTab<R> t;
+    t.b21 = (R)0.2; t.b31 = (R)0.075; t.b32 = (R)0.225; t.b41 = (R)0.3; t.b42 = (R)-0.9; t.b43 = (R)1.2;
+    t.b51 = (R)(-11.0 / 54.0); t.b52 = (R)2.5; t.b53 = (R)(-70.0 / 27.0); t.b54 = (R)(35.0 / 27.0);
+    t.b61 = (R)(1631.0 / 55296.0); t.b62 = (R)(175.0 / 512.0); t.b63 = (R)(575.0 / 13824.0);
+    t.b64 = (R)(44275.0 / 110592.0); t.b65 = (R)(253.0 / 4096.0);
+    t.c1 = (R)c1; t.c3 = (R)c3; t.c4 = (R)c4; t.c6 = (R)c6;
+    t.dc1 = c1 - 2825.0 / 27648.0;
+    t.dc3 = c3 - 18575.0 / 48384.0;
+    t.dc4 = c4 - 13525.0 / 55296.0;
+    t.dc5 = c5 - 277.0 / 14336.0;
+    t.dc6 = c6 - 0.25;
+    (void)c2;
+    return t;
+}
+
+static int ensure_partial(gpe_ctx* c, size_t ndoubles) {
+    if (c->partial_cap >= ndoubles) return 0;
+    if (c->partial) (void)hipFree(c->partial);
+    c->partial = nullptr;
+    c->partial_cap = 0;
+    int rc = dalloc(c, (void**)&c->partial, ndoubles * sizeof(double), "partials");
+    if (rc) return rc;
+    c->partial_cap = ndoubles;
+    return 0;
+}
+
+template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const void* u, const Op& op) {
+    Geom g = make_geom<R>(c);
+    const int nb = tile_geom(c->tiles, g);
+    if (Op::kNRed > 0) {
+        int rc = ensure_partial(c, (size_t)nb * Op::kNRed);
+        if (rc) return rc;
+    }
+    gpe_ctx::ProfRec rec{Op::kKind, (double)Op::kAccess * (double)c->esz * (double)c->n, nullptr, nullptr};
+    if (c->prof) {
+        HIPCHK(c, hipEventCreate(&rec.a));
+        HIPCHK(c, hipEventCreate(&rec.b));
+        HIPCHK(c, hipEventRecord(rec.a, c->stream));
+    }
+    HIPCHK(c, (launch_stencil<R, ORD, Op>(c->tiles, g, make_w<R>(c->p), (const C<R>*)u, op, c->partial, c->stream)));
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(rec.b, c->stream));
+        c->recs.push_back(rec);
+    }
+    if (Op::kNRed > 0) {
+        HIPCHK(c, launch_reduce_final(Op::kMode, c->partial, nb, Op::kNRed, c->dres, c->stream));
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Steppers (src/solve.f90:57-154)
+
+template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, double dre, double dim) {
+    const RhsC<R> k = make_rhs<R>(c);
+    C<R>* psi = (C<R>*)c->psi;
+    C<R>* uA = (C<R>*)c->uA;
+    C<R>* uB = (C<R>*)c->uB;
+    int rc;
+    if (scheme == GPE_EULER) {
+        // new = old + dt*rhs
+        OpAxpySelf<R> op{k, uA, cdt<R>(dre, dim, 1.0)};
+        if ((rc = run_op<R, ORD>(c, psi, op))) return rc;
+        std::swap(c->psi, c->uA);
+    } else if (scheme == GPE_RK2) {
+        // tmp = old + 0.5 dt kk1 ; new = old + dt kk2
+        OpAxpySelf<R> s1{k, uA, cdt<R>(dre, dim, 0.5)};
+        if ((rc = run_op<R, ORD>(c, psi, s1))) return rc;
+        OpAxpyP0<R> s2{k, psi, uB, cdt<R>(dre, dim, 1.0)};
+        if ((rc = run_op<R, ORD>(c, uA, s2))) return rc;
+        std::swap(c->psi, c->uB);
+    } else {
+        // classic RK4; acc carries psi0 + dt (k1/6 + k2/3 + k3/3) through the stages
+        C<R>* acc = (C<R>*)c->acc;
+        OpRk4First<R> s1{k, uA, acc, cdt<R>(dre, dim, 0.5), cdt<R>(dre, dim, 1.0 / 6.0)};
+        if ((rc = run_op<R, ORD>(c, psi, s1))) return rc;
+        OpRk4Mid<R> s2{k, psi, acc, uB, cdt<R>(dre, dim, 0.5), cdt<R>(dre, dim, 1.0 / 3.0)};
+        if ((rc = run_op<R, ORD>(c, uA, s2))) return rc;
+        OpRk4Mid<R> s3{k, psi, acc, uA, cdt<R>(dre, dim, 1.0), cdt<R>(dre, dim, 1.0 / 3.0)};
+        if ((rc = run_op<R, ORD>(c, uB, s3))) return rc;
+        OpRk4Last<R> s4{k, acc, psi, cdt<R>(dre, dim, 1.0 / 6.0)};
+        if ((rc = run_op<R, ORD>(c, uA, s4))) return rc;
+    }
+    return 0;
+}
+
+static int need_rk_buffers(gpe_ctx* c, int scheme) {
+    int rc;
+    const size_t fb = field_bytes(c);
+    if ((rc = dalloc(c, &c->uA, fb, "uA"))) return rc;
+    if (scheme == GPE_EULER) return 0;
+    if ((rc = dalloc(c, &c->uB, fb, "uB"))) return rc;
+    if (scheme == GPE_RK4 && (rc = dalloc(c, &c->acc, fb, "acc"))) return rc;
+    if (scheme == GPE_RK45) {
+        if ((rc = dalloc(c, &c->f0, fb, "f0"))) return rc;
+        if ((rc = dalloc(c, &c->k2, fb, "k2"))) return rc;
+        if ((rc = dalloc(c, &c->k3, fb, "k3"))) return rc;
+        if ((rc = dalloc(c, &c->k4, fb, "k4"))) return rc;
+        if ((rc = dalloc(c, &c->k5, fb, "k5"))) return rc;
+        if ((rc = dalloc(c, &c->pnew, fb, "pnew"))) return rc;
+    }
+    return 0;
+}
+
+// One Cash-Karp attempt (src/solve.f90:248-378) with the fused error
+// max-norm of rkqs (src/solve.f90:176-185).
+template <typename R, int ORD>
+static int ck_attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    const RhsC<R> k = make_rhs<R>(c);
+    const CkTab<R> t = make_tab<R>();
+    const C<R> dt = cdt<R>(dre, dim, 1.0);
+    C<R>* psi = (C<R>*)c->psi;
+    C<R>* uA = (C<R>*)c->uA;
+    C<R>* uB = (C<R>*)c->uB;
+    C<R>* f0 = (C<R>*)c->f0;
+    C<R>* k2 = (C<R>*)c->k2;
+    C<R>* k3 = (C<R>*)c->k3;
+    C<R>* k4 = (C<R>*)c->k4;
+    C<R>* k5 = (C<R>*)c->k5;
+    int rc;
+    if (!c->f0_valid) {
+        OpCk1<R> s1{k, f0, uA, dt, t.b21};
+        if ((rc = run_op<R, ORD>(c, psi, s1))) return rc;
+        c->f0_valid = true;
+    } else {
+        HIPCHK(c, launch_ck1_retry<R>(psi, f0, uA, dt, t.b21, c->n, c->stream));
+    }
+    OpCkMid<R, 2> s2{k, psi, f0, nullptr, nullptr, nullptr, k2, uB, dt, t};
+    if ((rc = run_op<R, ORD>(c, uA, s2))) return rc;
+    OpCkMid<R, 3> s3{k, psi, f0, k2, nullptr, nullptr, k3, uA, dt, t};
+    if ((rc = run_op<R, ORD>(c, uB, s3))) return rc;
+    OpCkMid<R, 4> s4{k, psi, f0, k2, k3, nullptr, k4, uB, dt, t};
+    if ((rc = run_op<R, ORD>(c, uA, s4))) return rc;
+    OpCkMid<R, 5> s5{k, psi, f0, k2, k3, k4, k5, uA, dt, t};
+    if ((rc = run_op<R, ORD>(c, uB, s5))) return rc;
+    OpCkLast<R> s6{k, psi, f0, k3, k4, k5, (C<R>*)c->pnew, dt, d0re, d0im, t};
+    if ((rc = run_op<R, ORD>(c, uA, s6))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = c->hres[0] / c->p.eps;
+    out[1] = c->hres[1];
+    c->cand_valid = true;
+    return 0;
+}
+
+template <typename R> static int dispatch_fixed(gpe_ctx* c, int scheme, double dre, double dim) {
+    return c->p.order == 2 ? step_fixed<R, 2>(c, scheme, dre, dim) : step_fixed<R, 4>(c, scheme, dre, dim);
+}
+template <typename R>
+static int dispatch_attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    return c->p.order == 2 ? ck_attempt<R, 2>(c, dre, dim, d0re, d0im, out)
+                           : ck_attempt<R, 4>(c, dre, dim, d0re, d0im, out);
+}
+
+static int attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    int rc = need_rk_buffers(c, GPE_RK45);
+    if (rc) return rc;
+    return c->p.storage == GPE_C128 ? dispatch_attempt<double>(c, dre, dim, d0re, d0im, out)
+                                    : dispatch_attempt<float>(c, dre, dim, d0re, d0im, out);
+}
+
+static void accept(gpe_ctx* c) {
+    std::swap(c->psi, c->pnew);
+    c->f0_valid = false;
+    c->cand_valid = false;
+}
+
+static double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+// rkqs (src/solve.f90:158-244): attempts until errmax <= 1, then the
+// elementary step-size controller of the reference.
+static int rkqs(gpe_ctx* c, gpe_step_info* io) {
+    const gpe_params& p = c->p;
+    const double errcon = pow(5.0 / p.safety, 1.0 / p.dt_increase);
+    double dre = io->dt_re, dim = io->dt_im;
+    const double d0re = dre, d0im = dim;  // psi_scal uses the dt held on entry
+    double out[2] = {0, 0};
+    int rc;
+    for (;;) {
+        const bool had_f0 = c->f0_valid;
+        if ((rc = attempt(c, dre, dim, d0re, d0im, out))) return rc;
+        io->n_rhs += had_f0 ? 5 : 6;
+        if (!(out[0] > 1.0)) break;
+        io->n_reject += 1;
+        // dt_temp = safety*dt*errmax**dt_decrease; no more than a factor 10 decrease
+        const double f = pow(out[0], p.dt_decrease);
+        const double mt = hypot((p.safety * dre) * f, (p.safety * dim) * f);
+        const double ad = hypot(dre, dim);
+        const double mag = fsign(fmax(mt, 0.1 * ad), ad);
+        if (!p.real_time) { dre = 0.0; dim = -mag; }
+        else { dre = mag; dim = 0.0; }
+        if (!isfinite(out[0]) || mag == 0.0) return fail(c, GPE_EINVAL, "rkqs: step size underflow or non-finite error");
+    }
+    double nre, nim;
+    if (out[0] > errcon) {
+        const double f = pow(out[0], p.dt_increase);
+        nre = (p.safety * dre) * f;
+        nim = (p.safety * dim) * f;
+    } else {
+        nre = 5.0 * dre;
+        nim = 5.0 * dim;
+    }
+    io->t += dre;              // t = t + real(dt)
+    io->dt_re = nre;           // dt = dt_next
+    io->dt_im = nim;
+    io->im_t += fabs(nim);     // im_t = im_t + abs(aimag(dt)) with the new dt (src/solve.f90:229)
+    io->errmax = out[0];
+    io->psi2max = out[1];
+    io->n_accept += 1;
+    accept(c);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host <-> device transfers
+
+static size_t host_esz(int host_dtype) { return host_dtype == GPE_HOST_C64 ? 8 : 16; }
+
+static int ensure_stage(gpe_ctx* c, size_t bytes) {
+    if (c->stage_bytes >= bytes) return 0;
+    if (c->stage) (void)hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    int rc = dalloc(c, &c->stage, bytes, "staging");
+    if (rc) return rc;
+    c->stage_bytes = bytes;
+    return 0;
+}
+
+static int upload_to(gpe_ctx* c, void* dst, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off,
+                     int64_t z_off) {
+    if (!host) return fail(c, GPE_EINVAL, "upload: host pointer is NULL");
+    if (host_dtype != GPE_HOST_C128 && host_dtype != GPE_HOST_C64) return fail(c, GPE_EINVAL, "upload: bad host dtype");
+    if (ny_alloc < c->p.ny || y_off < 0 || y_off + c->p.ny > ny_alloc || z_off < 0)
+        return fail(c, GPE_EINVAL, "upload: bad halo layout (ny_alloc, y_off, z_off)");
+    const size_t he = host_esz(host_dtype);
+    const bool same = (he == c->esz);
+    void* target = dst;
+    if (!same) {
+        int rc = ensure_stage(c, (size_t)c->n * he);
+        if (rc) return rc;
+        target = c->stage;
+    }
+    const char* src = (const char*)host + ((size_t)z_off * ny_alloc + y_off) * c->p.nx * he;
+    const size_t width = (size_t)c->p.nx * c->p.ny * he;
+    const size_t spitch = (size_t)c->p.nx * ny_alloc * he;
+    HIPCHK(c, hipMemcpy2DAsync(target, width, src, spitch, width, c->p.nz, hipMemcpyHostToDevice, c->stream));
+    if (!same) {
+        if (c->p.storage == GPE_C128) {
+            HIPCHK(c, launch_from_c64<double>((const float2*)c->stage, (double2*)dst, c->n, c->stream));
+        } else {
+            HIPCHK(c, launch_from_c128<float>((const double2*)c->stage, (float2*)dst, c->n, c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int download_from(gpe_ctx* c, const void* src, void* host, int host_dtype) {
+    if (!host) return fail(c, GPE_EINVAL, "download: host pointer is NULL");
+    if (host_dtype != GPE_HOST_C128 && host_dtype != GPE_HOST_C64) return fail(c, GPE_EINVAL, "download: bad host dtype");
+    const size_t he = host_esz(host_dtype);
+    const void* from = src;
+    if (he != c->esz) {
+        int rc = ensure_stage(c, (size_t)c->n * he);
+        if (rc) return rc;
+        if (c->p.storage == GPE_C128) {
+            HIPCHK(c, launch_to_c64<double>((const double2*)src, (float2*)c->stage, c->n, c->stream));
+        } else {
+            HIPCHK(c, launch_to_c128<float>((const float2*)src, (double2*)c->stage, c->n, c->stream));
+        }
+        from = c->stage;
+    }
+    HIPCHK(c, hipMemcpyAsync(host, from, (size_t)c->n * he, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+template <typename R> static int build_dtab(gpe_ctx* c) {
+    // 1 + tanh(s - a) tanh(s + a) with a = x(nx-3) etc. (src/solve.f90:397-409)
+    const gpe_params& p = c->p;
+    double d[3];
+    spacing(p, d);
+    const int ns[3] = {p.nx, p.ny, p.nz};
+    const double lo[3] = {-p.xr, -p.yr, -p.zr};
+    std::string tmp;
+    R* h = new R[(size_t)p.nx + p.ny + p.nz];
+    size_t o = 0;
+    for (int ax = 0; ax < 3; ++ax) {
+        const double a = lo[ax] + (double)(ns[ax] - 3) * d[ax];
+        for (int i = 0; i < ns[ax]; ++i) {
+            const double s = lo[ax] + (double)i * d[ax];
+            h[o++] = (R)(1.0 + tanh(s - a) * tanh(s + a));
+        }
+    }
+    int rc = dalloc(c, &c->dtab, o * sizeof(R), "diss tables");
+    if (!rc) {
+        hipError_t e = hipMemcpy(c->dtab, h, o * sizeof(R), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(c, GPE_EHIP, std::string("hipMemcpy(dtab): ") + hipGetErrorString(e));
+    }
+    delete[] h;
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Public entry points
+
+template <typename R> static int deriv_impl(gpe_ctx* c, int op, const void* u, void* out, bool rhs) {
+    if (rhs) {
+        OpRhs<R> o{make_rhs<R>(c), (C<R>*)out};
+        return c->p.order == 2 ? run_op<R, 2>(c, u, o) : run_op<R, 4>(c, u, o);
+    }
+    OpDeriv<R> o{(C<R>*)out, op};
+    return c->p.order == 2 ? run_op<R, 2>(c, u, o) : run_op<R, 4>(c, u, o);
+}
+
+static int deriv_common(gpe_ctx* c, int op, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off,
+                        void* out, int host_dtype, bool rhs) {
+    if (!out) return fail(c, GPE_EINVAL, "output pointer is NULL");
+    int rc;
+    if ((rc = dalloc(c, &c->sout, field_bytes(c), "deriv out"))) return rc;
+    const void* u = c->psi;
+    if (in) {
+        if ((rc = dalloc(c, &c->sin, field_bytes(c), "deriv in"))) return rc;
+        if ((rc = upload_to(c, c->sin, in, host_dtype, ny_alloc, y_off, z_off))) return rc;
+        u = c->sin;
+    }
+    rc = c->p.storage == GPE_C128 ? deriv_impl<double>(c, op, u, c->sout, rhs) : deriv_impl<float>(c, op, u, c->sout, rhs);
+    if (rc) return rc;
+    return download_from(c, c->sout, out, host_dtype);
+}
+
+static int read_result(gpe_ctx* c, double* v) {
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *v = c->hres[0];
+    return 0;
+}
+
+static int norm_mass(gpe_ctx* c, double* v, int weighted) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "norm/mass: ctx is NULL");
+    if (!v) return fail(c, GPE_EINVAL, "norm/mass: output is NULL");
+    int rc = ensure_partial(c, kNormBlocks);
+    if (rc) return rc;
+    if (c->p.storage == GPE_C128) {
+        HIPCHK(c, launch_norm_partial<double>((const double2*)c->psi, c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
+                                              kNormBlocks, c->stream));
+    } else {
+        HIPCHK(c, launch_norm_partial<float>((const float2*)c->psi, c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
+                                             kNormBlocks, c->stream));
+    }
+    HIPCHK(c, launch_reduce_final(RED_SUM, c->partial, kNormBlocks, 1, c->dres, c->stream));
+    double s;
+    if ((rc = read_result(c, &s))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    *v = s * d[0] * d[1] * d[2];
+    return 0;
+}
+
+static void prof_clear(gpe_ctx* c) {
+    for (auto& r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    c->recs.clear();
+}
+
+extern "C" {
+
+int gpe_abi_version(void) { return GPE_ABI_VERSION; }
+
+const char* gpe_last_error(const gpe_ctx* ctx) { return ctx ? ctx->err.c_str() : g_thread_error.c_str(); }
+
+int gpe_ctx_create(const gpe_params* p, gpe_ctx** out) {
+    if (!out) return fail(nullptr, GPE_EINVAL, "gpe_ctx_create: out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (validate(p, why)) return fail(nullptr, GPE_EINVAL, "gpe_ctx_create: " + why);
+    gpe_ctx* c = new gpe_ctx();
+    c->p = *p;
+    c->n = (long long)p->nx * p->ny * p->nz;
+    c->esz = p->storage == GPE_C128 ? 16 : 8;
+    c->tiles = default_tiles(*p);
+    int rc = 0;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        rc = fail(nullptr, GPE_EHIP, std::string("gpe_ctx_create: ") + hipGetErrorString(e));
+        delete c;
+        return rc;
+    }
+    if (!rc) rc = dalloc(c, &c->psi, field_bytes(c), "psi");
+    if (!rc) rc = dalloc(c, (void**)&c->dres, 8 * sizeof(double), "results");
+    if (!rc) {
+        e = hipHostMalloc((void**)&c->hres, 8 * sizeof(double), hipHostMallocDefault);
+        if (e != hipSuccess) rc = fail(c, GPE_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    if (!rc && p->diss_amp != 0.0 && (p->eqn_to_solve == 1 || p->eqn_to_solve == 3))
+        rc = p->storage == GPE_C128 ? build_dtab<double>(c) : build_dtab<float>(c);
+    if (!rc) {
+        e = hipMemsetAsync(c->psi, 0, field_bytes(c), c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, GPE_EHIP, std::string("init: ") + hipGetErrorString(e));
+    }
+    if (rc) {
+        g_thread_error = c->err;
+        gpe_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+void gpe_ctx_destroy(gpe_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    prof_clear(c);
+    void* bufs[] = {c->psi, c->uA, c->uB, c->acc, c->f0, c->k2, c->k3, c->k4, c->k5, c->pnew,
+                    c->sin, c->sout, c->stage, c->partial, c->dres, c->dtab};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->hres) (void)hipHostFree(c->hres);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int gpe_set_tiles(gpe_ctx* c, int tx, int ty, int tz) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_set_tiles: ctx is NULL");
+    if (!tile_supported(tx, ty)) return fail(c, GPE_EINVAL, "gpe_set_tiles: unsupported tile shape");
+    if (tz < 0) return fail(c, GPE_EINVAL, "gpe_set_tiles: tz must be >= 0");
+    Tiles t{tx, ty, tz};
+    if (tz == 0) {
+        Tiles d = default_tiles(c->p);
+        const int ntxy = ((c->p.nx + tx - 1) / tx) * ((c->p.ny + ty - 1) / ty);
+        long long z = ((long long)c->p.nz * ntxy) / 2048;
+        t.tz = (int)(z < 8 ? 8 : z > c->p.nz ? c->p.nz : z);
+        (void)d;
+    }
+    if (t.tz > c->p.nz) t.tz = c->p.nz;
+    c->tiles = t;
+    return 0;
+}
+
+int gpe_get_tiles(const gpe_ctx* c, int* tx, int* ty, int* tz) {
+    if (!c || !tx || !ty || !tz) return fail(nullptr, GPE_EINVAL, "gpe_get_tiles: NULL argument");
+    *tx = c->tiles.tx;
+    *ty = c->tiles.ty;
+    *tz = c->tiles.tz;
+    return 0;
+}
+
+int gpe_upload(gpe_ctx* c, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off, int64_t z_off) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_upload: ctx is NULL");
+    int rc = upload_to(c, c->psi, host, host_dtype, ny_alloc, y_off, z_off);
+    if (!rc) {
+        c->f0_valid = false;
+        c->cand_valid = false;
+    }
+    return rc;
+}
+
+int gpe_download(gpe_ctx* c, void* host, int host_dtype) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_download: ctx is NULL");
+    return download_from(c, c->psi, host, host_dtype);
+}
+
+int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_step: ctx is NULL");
+    if (!io) return fail(c, GPE_EINVAL, "gpe_step: io is NULL");
+    if (scheme < GPE_EULER || scheme > GPE_RK45)
+        return fail(c, GPE_EINVAL, "ERROR: Unrecognised time stepping scheme.");  // src/solve.f90:50-51
+    if (nsteps < 0) return fail(c, GPE_EINVAL, "gpe_step: nsteps < 0");
+    int rc = need_rk_buffers(c, scheme);
+    if (rc) return rc;
+    if (scheme == GPE_RK45) {
+        for (int s = 0; s < nsteps; ++s)
+            if ((rc = rkqs(c, io))) return rc;
+        return 0;
+    }
+    const int nrhs = scheme == GPE_EULER ? 1 : scheme == GPE_RK2 ? 2 : 4;
+    for (int s = 0; s < nsteps; ++s) {
+        rc = c->p.storage == GPE_C128 ? dispatch_fixed<double>(c, scheme, io->dt_re, io->dt_im)
+                                      : dispatch_fixed<float>(c, scheme, io->dt_re, io->dt_im);
+        if (rc) return rc;
+        io->t += io->dt_re;             // t = t + real(dt)
+        io->im_t += fabs(io->dt_im);    // im_t = im_t + abs(aimag(dt))
+        io->n_accept += 1;
+        io->n_rhs += nrhs;
+    }
+    c->f0_valid = false;
+    c->cand_valid = false;
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int gpe_solver(gpe_ctx* c, int scheme, const void* var_in, int64_t ny_alloc, int64_t y_off, int64_t z_off,
+               void* var_out, int host_dtype, gpe_step_info* io) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_solver: ctx is NULL");
+    int rc = gpe_upload(c, var_in, host_dtype, ny_alloc, y_off, z_off);
+    if (!rc) rc = gpe_step(c, scheme, 1, io);
+    if (!rc) rc = gpe_download(c, var_out, host_dtype);
+    return rc;
+}
+
+int gpe_rk45_attempt(gpe_ctx* c, double dt_re, double dt_im, double dt0_re, double dt0_im, double out[2]) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_rk45_attempt: ctx is NULL");
+    if (!out) return fail(c, GPE_EINVAL, "gpe_rk45_attempt: out is NULL");
+    if (!isfinite(dt_re) || !isfinite(dt_im) || (dt_re == 0.0 && dt_im == 0.0))
+        return fail(c, GPE_EINVAL, "gpe_rk45_attempt: dt must be finite and non-zero");
+    return attempt(c, dt_re, dt_im, dt0_re, dt0_im, out);
+}
+
+int gpe_rk45_accept(gpe_ctx* c) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_rk45_accept: ctx is NULL");
+    if (!c->cand_valid) return fail(c, GPE_ESTATE, "gpe_rk45_accept: no attempt to accept");
+    accept(c);
+    return 0;
+}
+
+int gpe_deriv(gpe_ctx* c, int op, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off, void* out,
+              int host_dtype) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_deriv: ctx is NULL");
+    if (op < GPE_D_X || op > GPE_D_LAP) return fail(c, GPE_EINVAL, "gpe_deriv: unknown operator");
+    return deriv_common(c, op, in, ny_alloc, y_off, z_off, out, host_dtype, false);
+}
+
+int gpe_rhs(gpe_ctx* c, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off, void* out, int host_dtype) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_rhs: ctx is NULL");
+    return deriv_common(c, 0, in, ny_alloc, y_off, z_off, out, host_dtype, true);
+}
+
+int gpe_norm(gpe_ctx* c, double* norm) { return norm_mass(c, norm, 1); }
+int gpe_mass(gpe_ctx* c, double* mass) { return norm_mass(c, mass, 0); }
+
+int gpe_energy(gpe_ctx* c, double* energy) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_energy: ctx is NULL");
+    if (!energy) return fail(c, GPE_EINVAL, "gpe_energy: output is NULL");
+    int rc;
+    if (c->p.storage == GPE_C128) {
+        OpEnergy<double> o;
+        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+    } else {
+        OpEnergy<float> o;
+        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+    }
+    if (rc) return rc;
+    double s;
+    if ((rc = read_result(c, &s))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    *energy = s * d[0] * d[1] * d[2];  // E = E*dx*dy*dz (src/variables.f90:584)
+    return 0;
+}
+
+int gpe_renormalise(gpe_ctx* c, double norm) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_renormalise: ctx is NULL");
+    if (!(norm > 0.0) || !isfinite(norm)) return fail(c, GPE_EINVAL, "gpe_renormalise: norm must be positive");
+    const double snn = sqrt(c->p.nn), sn = sqrt(norm);
+    if (c->p.storage == GPE_C128) {
+        HIPCHK(c, launch_scale<double>((double2*)c->psi, c->n, snn, sn, c->stream));
+    } else {
+        HIPCHK(c, launch_scale<float>((float2*)c->psi, c->n, snn, sn, c->stream));
+    }
+    c->f0_valid = false;
+    c->cand_valid = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int gpe_prof_enable(gpe_ctx* c, int on) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_prof_enable: ctx is NULL");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    prof_clear(c);
+    c->prof = on != 0;
+    return 0;
+}
+
+int gpe_prof_read(gpe_ctx* c, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_prof_read: ctx is NULL");
+    if (kind < 0 || kind >= GPE_NKINDS || !total_ms || !launches || !bytes_per_launch)
+        return fail(c, GPE_EINVAL, "gpe_prof_read: bad argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double ms = 0.0, bytes = 0.0;
+    int64_t n = 0;
+    for (auto& r : c->recs) {
+        if (r.kind != kind) continue;
+        float e = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&e, r.a, r.b));
+        ms += e;
+        bytes = r.bytes;
+        ++n;
+    }
+    *total_ms = ms;
+    *launches = n;
+    *bytes_per_launch = bytes;
+    return 0;
+}
+
+const char* gpe_kernel_kind_name(int kind) { return (kind >= 0 && kind < GPE_NKINDS) ? kKindNames[kind] : ""; }
+
+int gpe_bench_copy(void* dst, const void* src, int64_t nbytes, void* stream) {
+    if (!dst || !src || nbytes < 0 || (nbytes % 16) != 0)
+        return fail(nullptr, GPE_EINVAL, "gpe_bench_copy: pointers must be non-NULL and nbytes a multiple of 16");
+    hipError_t e = launch_copy16(dst, src, nbytes / 16, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(nullptr, GPE_EHIP, std::string("gpe_bench_copy: ") + hipGetErrorString(e));
+    return 0;
+}
+
+void* gpe_state_ptr(gpe_ctx* c) { return c ? c->psi : nullptr; }
+void* gpe_stream(gpe_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int gpe_sync(gpe_ctx* c) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_sync: ctx is NULL");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,401 @@
+// The z-marching stencil kernel and the fused stage operators.
+#pragma once
+
+#include "gpe_device.h"
+
+namespace gpe {
+
+// ---------------------------------------------------------------------------
+// Block reductions (deterministic: fixed butterfly within a wavefront, fixed
+// wave order across the block).
+enum { RED_NONE = 0, RED_SUM = 1, RED_MAX = 2 };
+
+template <int MODE> __device__ __forceinline__ double red_op(double a, double b) {
+    if constexpr (MODE == RED_SUM) return a + b;
+    else return a > b ? a : b;
+}
+
+template <int NT, int MODE> __device__ __forceinline__ double block_reduce(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = red_op<MODE>(v, __shfl_xor(v, o, 64));
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) r = red_op<MODE>(r, sh[i]);
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// The stencil kernel.  Op supplies:
+//   kNRed (0..2 reduction values), kMode (RED_SUM / RED_MAX),
+//   struct Pre; Pre load(long long i) const   -- pointwise inputs of cell i
+//   void apply(long long i, int x, int y, int z, const Sten& st, const Pre& p, double* red) const
+template <typename R, int ORD, int TX, int TY, class Op>
+__global__ __launch_bounds__(TX* TY) void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
+                                                     double* __restrict__ partial) {
+    constexpr int H = ORD / 2, NT = TX * TY, LW = TX + 2 * H, LH = TY + 2 * H;
+    constexpr int NHALO = 2 * H * TX + 2 * H * TY;
+    constexpr int NSLOT = (NHALO + NT - 1) / NT;
+    __shared__ C<R> lds[2][LH * LW];
+    __shared__ double red_sh[NT / 64];
+
+    const int tid = threadIdx.x;
+    const int bid = xcd_remap(blockIdx.x, g.nblocks);
+    const int ntxy = g.ntx * g.nty;
+    const int txy = bid % ntxy, zc = bid / ntxy;
+    const int x0 = (txy % g.ntx) * TX, y0 = (txy / g.ntx) * TY;
+    const int z0 = zc * g.tz;
+    const int z1 = min(z0 + g.tz, g.nz);
+    const int lx = tid % TX, ly = tid / TX;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool valid = (x < g.nx) && (y < g.ny);
+    const long long col = (long long)bcmap(x, g.nx, g.bcs) + (long long)g.nx * bcmap(y, g.ny, g.bcs);
+    const long long own = (long long)x + (long long)g.nx * y;  // cell offset within a plane (valid threads)
+    const int lc = (ly + H) * LW + lx + H;
+
+    // Halo slots: first the 2H rows above/below the tile (coalesced rows of
+    // TX), then the 2H columns left/right of each tile row.
+    long long hoff[NSLOT];
+    int hl[NSLOT];
+    bool hv[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        const int e = tid + s * NT;
+        hv[s] = e < NHALO;
+        int gx = x0, gy = y0, lxx = H, lyy = H;
+        if (e < 2 * H * TX) {
+            const int r = e / TX, c = e % TX;
+            lxx = c + H;
+            lyy = r < H ? r : TY + r;
+            gx = x0 + c;
+            gy = y0 - H + lyy;
+        } else if (e < NHALO) {
+            const int e2 = e - 2 * H * TX;
+            const int r = e2 / (2 * H), c = e2 % (2 * H);
+            lyy = r + H;
+            lxx = c < H ? c : TX + c;
+            gx = x0 - H + lxx;
+            gy = y0 + r;
+        }
+        hl[s] = lyy * LW + lxx;
+        hoff[s] = (long long)bcmap(gx, g.nx, g.bcs) + (long long)g.nx * bcmap(gy, g.ny, g.bcs);
+    }
+
+    // Register z-window q[0..2H] = planes z-H..z+H, qn = plane z+H+1 (prefetch).
+    C<R> q[2 * H + 1];
+#pragma unroll
+    for (int d = 0; d < 2 * H; ++d) q[d] = u[g.plane * bcmap(z0 - H + d, g.nz, g.bcs) + col];
+    C<R> qn = mkc<R>(R(0), R(0));
+    C<R> hr[NSLOT];
+    typename Op::Pre pc{};
+    if (z0 < z1) {
+        qn = u[g.plane * bcmap(z0 + H, g.nz, g.bcs) + col];
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+            if (hv[s]) hr[s] = u[g.plane * z0 + hoff[s]];
+        if (valid) pc = op.load(g.plane * z0 + own);
+    }
+
+    double red[2] = {0.0, 0.0};
+    for (int z = z0; z < z1; ++z) {
+        q[2 * H] = qn;
+        C<R>* L = lds[(z - z0) & 1];
+        L[lc] = q[H];
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+            if (hv[s]) L[hl[s]] = hr[s];
+        __syncthreads();
+        typename Op::Pre pn{};
+        if (z + 1 < z1) {
+            qn = u[g.plane * bcmap(z + 1 + H, g.nz, g.bcs) + col];
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s)
+                if (hv[s]) hr[s] = u[g.plane * (z + 1) + hoff[s]];
+            if (valid) pn = op.load(g.plane * (z + 1) + own);
+        }
+        if (valid) {
+            Sten<R, ORD, LW> st;
+            st.L = L;
+            st.lc = lc;
+#pragma unroll
+            for (int d = 0; d <= 2 * H; ++d) st.q[d] = q[d];
+            st.w = w;
+            op.apply(g.plane * z + own, x, y, z, st, pc, red);
+        }
+        pc = pn;
+#pragma unroll
+        for (int d = 0; d < 2 * H; ++d) q[d] = q[d + 1];
+    }
+
+    if constexpr (Op::kNRed > 0) {
+#pragma unroll
+        for (int r = 0; r < Op::kNRed; ++r) {
+            const double v = block_reduce<NT, Op::kMode>(red[r], red_sh);
+            if (tid == 0) partial[(long long)r * g.nblocks + blockIdx.x] = v;
+        }
+    }
+}
+
+// Final reduction of per-block partials (one block, fixed order).
+template <int MODE>
+__global__ __launch_bounds__(1024) void k_reduce_final(const double* __restrict__ partial, int n, int nval,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[16];
+    for (int r = 0; r < nval; ++r) {
+        double v = MODE == RED_SUM ? 0.0 : 0.0;
+        for (int i = threadIdx.x; i < n; i += 1024) v = red_op<MODE>(v, partial[(long long)r * n + i]);
+        v = block_reduce<1024, MODE>(v, sh);
+        if (threadIdx.x == 0) out[r] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage operators.  Complex dt factors are passed premultiplied (e.g. dt/2).
+
+struct NoPre {};
+
+// out = psi + a*f(psi)                              (euler; rk2 stage 1)
+template <typename R> struct OpAxpySelf {
+    static constexpr int kKind = 0, kAccess = 2;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    RhsC<R> k;
+    C<R>* __restrict__ out;
+    C<R> a;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), st.c());
+    }
+};
+
+// out = psi0 + a*f(u)                               (rk2 stage 2)
+template <typename R> struct OpAxpyP0 {
+    static constexpr int kKind = 1, kAccess = 3;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    struct Pre { C<R> p0; };
+    RhsC<R> k;
+    const C<R>* __restrict__ p0;
+    C<R>* __restrict__ out;
+    C<R> a;
+    __device__ Pre load(long long i) const { return {p0[i]}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.p0);
+    }
+};
+
+// RK4 stage 1: u1 = psi0 + (dt/2) f ; acc = psi0 + (dt/6) f
+template <typename R> struct OpRk4First {
+    static constexpr int kKind = 2, kAccess = 3;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    RhsC<R> k;
+    C<R>* __restrict__ unext;
+    C<R>* __restrict__ acc;
+    C<R> a1, a2;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+        const C<R> f = rhs_eval<R>(k, st, x, y, z);
+        const C<R> c = st.c();
+        unext[i] = cfma<R>(a1, f, c);
+        acc[i] = cfma<R>(a2, f, c);
+    }
+};
+
+// RK4 stages 2 and 3: unext = psi0 + a1 f ; acc += a2 f  (acc read and written
+// by the same thread through one pointer)
+template <typename R> struct OpRk4Mid {
+    static constexpr int kKind = 3, kAccess = 5;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    struct Pre { C<R> p0, acc; };
+    RhsC<R> k;
+    const C<R>* __restrict__ p0;
+    C<R>* acc;
+    C<R>* __restrict__ unext;
+    C<R> a1, a2;
+    __device__ Pre load(long long i) const { return {p0[i], acc[i]}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+        const C<R> f = rhs_eval<R>(k, st, x, y, z);
+        unext[i] = cfma<R>(a1, f, p.p0);
+        acc[i] = cfma<R>(a2, f, p.acc);
+    }
+};
+
+// RK4 stage 4: out = acc + (dt/6) f
+template <typename R> struct OpRk4Last {
+    static constexpr int kKind = 4, kAccess = 3;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    struct Pre { C<R> acc; };
+    RhsC<R> k;
+    const C<R>* __restrict__ acc;
+    C<R>* __restrict__ out;
+    C<R> a;
+    __device__ Pre load(long long i) const { return {acc[i]}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.acc);
+    }
+};
+
+// Cash-Karp tableau (src/solve.f90:255-281), stored in the field precision.
+template <typename R> struct CkTab {
+    R b21, b31, b32, b41, b42, b43, b51, b52, b53, b54, b61, b62, b63, b64, b65;
+    R c1, c3, c4, c6;
+    double dc1, dc3, dc4, dc5, dc6;
+};
+
+// Cash-Karp stage 1 (first attempt): f0 = f(psi0) ; u2 = psi0 + b21 (dt f0)
+template <typename R> struct OpCk1 {
+    static constexpr int kKind = 5, kAccess = 3;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    RhsC<R> k;
+    C<R>* __restrict__ f0;
+    C<R>* __restrict__ unext;
+    C<R> dt;
+    R b21;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+        const C<R> f = rhs_eval<R>(k, st, x, y, z);
+        f0[i] = f;
+        const C<R> k1 = cmul<R>(dt, f);
+        unext[i] = caxpy<R>(b21, k1, st.c());
+    }
+};
+
+// Cash-Karp stage s = 2..5: k_s = dt f(u_s) ; u_{s+1} = psi0 + sum_j b_{s+1,j} k_j
+template <typename R, int S_> struct OpCkMid {
+    // kernel s reads u_s, psi0, f0, k2..k_{s-1} and writes k_s, u_{s+1}
+    static constexpr int kKind = 4 + S_, kAccess = S_ + 3;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    struct Pre { C<R> p0, f0, k2, k3, k4; };
+    RhsC<R> k;
+    const C<R>* __restrict__ p0;
+    const C<R>* __restrict__ f0;
+    const C<R>* __restrict__ k2;
+    const C<R>* __restrict__ k3;
+    const C<R>* __restrict__ k4;
+    C<R>* __restrict__ kout;
+    C<R>* __restrict__ unext;
+    C<R> dt;
+    CkTab<R> t;
+    __device__ Pre load(long long i) const {
+        Pre p;
+        p.p0 = p0[i];
+        p.f0 = f0[i];
+        if constexpr (S_ >= 3) p.k2 = k2[i];
+        if constexpr (S_ >= 4) p.k3 = k3[i];
+        if constexpr (S_ >= 5) p.k4 = k4[i];
+        return p;
+    }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+        const C<R> ks = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
+        const C<R> k1 = cmul<R>(dt, p.f0);
+        kout[i] = ks;
+        C<R> u;
+        if constexpr (S_ == 2) {
+            u = caxpy<R>(t.b32, ks, caxpy<R>(t.b31, k1, p.p0));
+        } else if constexpr (S_ == 3) {
+            u = caxpy<R>(t.b43, ks, caxpy<R>(t.b42, p.k2, caxpy<R>(t.b41, k1, p.p0)));
+        } else if constexpr (S_ == 4) {
+            u = caxpy<R>(t.b54, ks, caxpy<R>(t.b53, p.k3, caxpy<R>(t.b52, p.k2, caxpy<R>(t.b51, k1, p.p0))));
+        } else {
+            u = caxpy<R>(t.b65, ks,
+                         caxpy<R>(t.b64, p.k4, caxpy<R>(t.b63, p.k3, caxpy<R>(t.b62, p.k2, caxpy<R>(t.b61, k1, p.p0)))));
+        }
+        unext[i] = u;
+    }
+};
+
+// Cash-Karp stage 6 + 5th-order solution + embedded error + scaled max-norm
+// (src/solve.f90:176-185, 365-375).  red[0] = max |err| / |psi_scal| with
+// psi_scal = |psi0| + dt0 |f0| + 1e-30 (complex), red[1] = max |psi_new|^2.
+// The error combination and scale are evaluated in FP64.
+template <typename R> struct OpCkLast {
+    static constexpr int kKind = 10, kAccess = 7;
+    static constexpr int kNRed = 2, kMode = RED_MAX;
+    struct Pre { C<R> p0, f0, k3, k4, k5; };
+    RhsC<R> k;
+    const C<R>* __restrict__ p0;
+    const C<R>* __restrict__ f0;
+    const C<R>* __restrict__ k3;
+    const C<R>* __restrict__ k4;
+    const C<R>* __restrict__ k5;
+    C<R>* __restrict__ out;
+    C<R> dt;
+    double dt0_re, dt0_im;
+    CkTab<R> t;
+    __device__ Pre load(long long i) const { return {p0[i], f0[i], k3[i], k4[i], k5[i]}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double* red) const {
+        const C<R> k6 = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
+        const C<R> k1 = cmul<R>(dt, p.f0);
+        const C<R> nw = caxpy<R>(t.c6, k6, caxpy<R>(t.c4, p.k4, caxpy<R>(t.c3, p.k3, caxpy<R>(t.c1, k1, p.p0))));
+        out[i] = nw;
+        const double er = t.dc1 * (double)k1.x + t.dc3 * (double)p.k3.x + t.dc4 * (double)p.k4.x +
+                          t.dc5 * (double)p.k5.x + t.dc6 * (double)k6.x;
+        const double ei = t.dc1 * (double)k1.y + t.dc3 * (double)p.k3.y + t.dc4 * (double)p.k4.y +
+                          t.dc5 * (double)p.k5.y + t.dc6 * (double)k6.y;
+        const double ap = sqrt((double)p.p0.x * p.p0.x + (double)p.p0.y * p.p0.y);
+        const double af = sqrt((double)p.f0.x * p.f0.x + (double)p.f0.y * p.f0.y);
+        const double sr = ap + dt0_re * af + 1e-30, si = dt0_im * af;
+        const double e = sqrt((er * er + ei * ei) / (sr * sr + si * si));
+        red[0] = e > red[0] ? e : red[0];
+        const double a2 = (double)nw.x * nw.x + (double)nw.y * nw.y;
+        red[1] = a2 > red[1] ? a2 : red[1];
+    }
+};
+
+// Standalone derivative (derivs module drop-in): out = D_op(u)
+template <typename R> struct OpDeriv {
+    static constexpr int kKind = 11, kAccess = 2;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    C<R>* __restrict__ out;
+    int op;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long i, int, int, int, const S& st, const Pre&, double*) const {
+        C<R> v;
+        switch (op) {
+            case 0: v = st.d1x(); break;
+            case 1: v = st.d1y(); break;
+            case 2: v = st.d1z(); break;
+            case 3: v = st.d2x(); break;
+            case 4: v = st.d2y(); break;
+            case 5: v = st.d2z(); break;
+            default: v = st.lap(); break;
+        }
+        out[i] = v;
+    }
+};
+
+// Standalone get_rhs: out = f(u)
+template <typename R> struct OpRhs {
+    static constexpr int kKind = 12, kAccess = 2;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    RhsC<R> k;
+    C<R>* __restrict__ out;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+        out[i] = rhs_eval<R>(k, st, x, y, z);
+    }
+};
+
+// Energy density of the reference (src/variables.f90:563-587):
+// Re(-lap(psi) conj(psi)) + 0.5 |psi|^4, summed in FP64.
+template <typename R> struct OpEnergy {
+    static constexpr int kKind = 13, kAccess = 1;
+    static constexpr int kNRed = 1, kMode = RED_SUM;
+    using Pre = NoPre;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long, int, int, int, const S& st, const Pre&, double* red) const {
+        const C<R> L = st.lap();
+        const C<R> p = st.c();
+        const double pr = p.x, pi = p.y;
+        const double a2 = pr * pr + pi * pi;
+        red[0] += -((double)L.x * pr + (double)L.y * pi) + 0.5 * (a2 * a2);
+    }
+};
+
+}  // namespace gpe

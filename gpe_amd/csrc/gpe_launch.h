@@ -1,0 +1,78 @@
+// Host-side launch interface of the stencil kernels and the pointwise
+// kernels.  The stencil launchers are explicitly instantiated per
+// (precision, order) in gpe_k_<p><order>.hip so those translation units
+// compile in parallel.
+#pragma once
+
+#include "gpe_kernels.h"
+
+namespace gpe {
+
+struct Tiles {
+    int tx, ty, tz;
+};
+
+// Tile shapes compiled into the library (tx, ty).
+inline bool tile_supported(int tx, int ty) {
+    return (tx == 64 && (ty == 4 || ty == 8 || ty == 16)) || (tx == 32 && (ty == 4 || ty == 8 || ty == 16));
+}
+
+// Fills the tiling fields of g for tile shape t and returns the block count.
+inline int tile_geom(const Tiles& t, Geom& g) {
+    g.ntx = (g.nx + t.tx - 1) / t.tx;
+    g.nty = (g.ny + t.ty - 1) / t.ty;
+    g.tz = t.tz;
+    const int nzc = (g.nz + t.tz - 1) / t.tz;
+    g.nblocks = g.ntx * g.nty * nzc;
+    return g.nblocks;
+}
+
+template <typename R> using OpCk2 = OpCkMid<R, 2>;
+template <typename R> using OpCk3 = OpCkMid<R, 3>;
+template <typename R> using OpCk4 = OpCkMid<R, 4>;
+template <typename R> using OpCk5 = OpCkMid<R, 5>;
+
+template <typename R, int ORD, class Op>
+hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u, const Op& op, double* partial,
+                          hipStream_t s);
+
+#define GPE_FOR_EACH_OP(X, R)                                                                                      \
+    X(R, OpAxpySelf<R>)                                                                                            \
+    X(R, OpAxpyP0<R>)                                                                                              \
+    X(R, OpRk4First<R>)                                                                                            \
+    X(R, OpRk4Mid<R>)                                                                                              \
+    X(R, OpRk4Last<R>)                                                                                             \
+    X(R, OpCk1<R>)                                                                                                 \
+    X(R, OpCk2<R>)                                                                                       \
+    X(R, OpCk3<R>)                                                                                       \
+    X(R, OpCk4<R>)                                                                                       \
+    X(R, OpCk5<R>)                                                                                       \
+    X(R, OpCkLast<R>)                                                                                              \
+    X(R, OpDeriv<R>)                                                                                               \
+    X(R, OpRhs<R>)                                                                                                 \
+    X(R, OpEnergy<R>)
+
+#define GPE_DECLARE_EXTERN_ORD(R, OP, ORD) \
+    extern template hipError_t launch_stencil<R, ORD, OP>(const Tiles&, Geom, const StW<R>&, const C<R>*, const OP&, \
+                                                          double*, hipStream_t);
+#define GPE_DECLARE_EXTERN(R, OP) GPE_DECLARE_EXTERN_ORD(R, OP, 2) GPE_DECLARE_EXTERN_ORD(R, OP, 4)
+GPE_FOR_EACH_OP(GPE_DECLARE_EXTERN, double)
+GPE_FOR_EACH_OP(GPE_DECLARE_EXTERN, float)
+
+// Pointwise kernels (gpe_pointwise.hip).
+template <typename R>
+hipError_t launch_ck1_retry(const C<R>* p0, const C<R>* f0, C<R>* unext, C<R> dt, R b21, long long n, hipStream_t s);
+template <typename R>
+hipError_t launch_norm_partial(const C<R>* u, int nx, int ny, int nz, int weighted, double* partial, int nblocks,
+                               hipStream_t s);
+template <typename R> hipError_t launch_scale(C<R>* u, long long n, double snn, double sn, hipStream_t s);
+template <typename R> hipError_t launch_from_c128(const double2* src, C<R>* dst, long long n, hipStream_t s);
+template <typename R> hipError_t launch_from_c64(const float2* src, C<R>* dst, long long n, hipStream_t s);
+template <typename R> hipError_t launch_to_c128(const C<R>* src, double2* dst, long long n, hipStream_t s);
+template <typename R> hipError_t launch_to_c64(const C<R>* src, float2* dst, long long n, hipStream_t s);
+hipError_t launch_copy16(void* dst, const void* src, long long n16, hipStream_t s);
+hipError_t launch_reduce_final(int mode, const double* partial, int n, int nval, double* out, hipStream_t s);
+
+constexpr int kNormBlocks = 2048;
+
+}  // namespace gpe

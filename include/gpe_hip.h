@@ -1,0 +1,186 @@
+/*
+ * gpe_hip.h -- C ABI of the MI355X (gfx950) GPE time-stepping library
+ * (libgpehip.so).
+ *
+ * The library replaces the time-stepping hot path of the reference Fortran
+ * solver (liangjj/gpe): the derivs module (src/derivs.f90), get_rhs and the
+ * euler/rk2/rk4/rkqs/rkck steppers of the solve module (src/solve.f90), the
+ * field reductions of the variables module (src/variables.f90) and the MPI
+ * ghost-plane exchange (src/variables.f90:186-358), which becomes in-kernel
+ * periodic or reflective indexing over one whole-domain grid on one GPU.
+ *
+ * Conventions (all entry points):
+ *  - return 0 on success, a negative GPE_E* code on failure; the message is
+ *    available from gpe_last_error(ctx) (or gpe_last_error(NULL) when no
+ *    context exists).  The Fortran host turns a failure into
+ *    emergency_stop(message), the reference's error convention
+ *    (src/error.f90:24-46).
+ *  - host arrays are borrowed for the duration of the call only; the context
+ *    owns all device memory.  Fields are x-fastest (Fortran column-major),
+ *    complex values interleaved (re, im): complex(c_double_complex) <->
+ *    double _Complex, complex(c_float_complex) <-> float _Complex.
+ *  - one host thread per context; every kernel of a context runs on that
+ *    context's HIP stream (gpe_stream).  Calls that return host values
+ *    synchronise the stream.
+ */
+#ifndef GPE_HIP_H
+#define GPE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPE_ABI_VERSION 1
+
+/* error codes */
+#define GPE_OK 0
+#define GPE_EINVAL (-1)   /* invalid argument */
+#define GPE_EHIP (-2)     /* HIP runtime error */
+#define GPE_ENOMEM (-3)   /* device allocation failed */
+#define GPE_ESTATE (-4)   /* call out of order (e.g. accept without attempt) */
+
+/* storage precision of device fields */
+#define GPE_C128 0        /* complex128 fields, FP64 stencil arithmetic */
+#define GPE_C64 1         /* complex64 fields, FP32 stencil, FP64 reductions and time */
+
+/* host array element type */
+#define GPE_HOST_C128 0
+#define GPE_HOST_C64 1
+
+/* time-stepping schemes: the reference's `scheme` values (src/solve.f90:34-50) */
+#define GPE_EULER 0       /* 'euler' */
+#define GPE_RK2 1         /* 'rk2'   */
+#define GPE_RK4 2         /* 'rk4'   */
+#define GPE_RK45 3        /* 'rk45'  (rkqs controller with Cash-Karp rkck) */
+
+/* derivative operators (src/derivs.f90:20-21, src/variables.f90:55-71) */
+#define GPE_D_X 0
+#define GPE_D_Y 1
+#define GPE_D_Z 2
+#define GPE_D_XX 3
+#define GPE_D_YY 4
+#define GPE_D_ZZ 5
+#define GPE_D_LAP 6
+
+/* Run-time parameters: the subset of the reference's module parameters and
+ * its run_params / eqn_params namelists the path reads
+ * (src/parameters.f90:25-55, 89-93; src/parameters.in:22-29). */
+typedef struct gpe_params {
+    int32_t nx, ny, nz;     /* grid points per axis (compile-time nx,ny,nz in the reference) */
+    int32_t order;          /* 2 or 4 */
+    int32_t bcs;            /* 1 periodic, 2 reflective */
+    int32_t eqn_to_solve;   /* 1..4 (src/solve.f90:414-437) */
+    int32_t real_time;      /* 1: dt real, 0: imaginary time */
+    int32_t storage;        /* GPE_C128 or GPE_C64 */
+    double xr, yr, zr;      /* half widths: dx = 2 xr/(nx-1), x(i) = -xr + i dx (src/ic.f90:43-56) */
+    double Urhs, diss_amp, g, mu, nn, omx, omy, omz;
+    double vscale;          /* coefficient of the trap V in eqn 1: i/2(lap + (1-|psi|^2 - vscale V)psi);
+                               0 reproduces the reference's eqn 1 */
+    double eps, safety, dt_decrease, dt_increase; /* rkqs constants; reference: 1e-6, 0.9, -0.25, -0.2 */
+} gpe_params;
+
+/* In/out record of gpe_step.  t, im_t, dt are the reference's module
+ * variables of the same names (src/parameters.f90:115-117). */
+typedef struct gpe_step_info {
+    double t, im_t;         /* in/out: advanced as the reference does */
+    double dt_re, dt_im;    /* in/out: complex dt; for RK45 the dt to try next */
+    double errmax;          /* RK45: errmax of the last accepted attempt */
+    double psi2max;         /* RK45: max |psi|^2 of the last accepted state */
+    int64_t n_accept;       /* accumulated accepted steps */
+    int64_t n_reject;       /* accumulated rejected RK45 attempts */
+    int64_t n_rhs;          /* accumulated RHS evaluations */
+} gpe_step_info;
+
+typedef struct gpe_ctx gpe_ctx;
+
+int gpe_abi_version(void);
+
+/* Last error message of ctx, or of the calling thread when ctx == NULL. */
+const char* gpe_last_error(const gpe_ctx* ctx);
+
+/* Create a context on the current HIP device: validates p, allocates the
+ * state field, creates the context's stream.  Replaces the reference's
+ * start-up of the path (get_grid src/ic.f90:32-69; array allocation
+ * src/gpe.f90:57-62; decomposition set-up src/variables.f90:98-182). */
+int gpe_ctx_create(const gpe_params* p, gpe_ctx** out);
+void gpe_ctx_destroy(gpe_ctx* ctx);
+
+/* Tile shape of the stencil kernels (tx in {32, 64}, ty in {4, 8, 16},
+ * tz >= 1 planes per block, or 0 for the default).  Used by the autotuner. */
+int gpe_set_tiles(gpe_ctx* ctx, int tx, int ty, int tz);
+int gpe_get_tiles(const gpe_ctx* ctx, int* tx, int* ty, int* tz);
+
+/* Copy a host field into the device state.  The host array has extent
+ * nx * ny_alloc * (any) and the interior starts at (0, y_off, z_off): pass
+ * (ny, 0, 0) for an interior array and (ny+4, 2, 2) for the reference's
+ * halo array var(0:nx1, js-2:je+2, ks-2:ke+2) with one rank.  Replaces the
+ * main-loop copy psi%old = psi%new and ghost fill (src/gpe.f90:165-174). */
+int gpe_upload(gpe_ctx* ctx, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off, int64_t z_off);
+/* Copy the device state into an interior host array (nx*ny*nz values). */
+int gpe_download(gpe_ctx* ctx, void* host, int host_dtype);
+
+/* Advance the device state.  scheme GPE_EULER/RK2/RK4: nsteps fixed steps
+ * of dt = (io->dt_re, io->dt_im) (src/solve.f90:57-154).  GPE_RK45: nsteps
+ * accepted steps of rkqs (src/solve.f90:158-244) starting from io->dt; on
+ * return io->dt holds dt_next.  t and im_t advance as in the reference. */
+int gpe_step(gpe_ctx* ctx, int scheme, int nsteps, gpe_step_info* io);
+
+/* Host-array drop-ins of solver(var_in, var_out) (src/solve.f90:25-53):
+ * upload var_in (halo layout as in gpe_upload), one step, download var_out. */
+int gpe_solver(gpe_ctx* ctx, int scheme, const void* var_in, int64_t ny_alloc, int64_t y_off, int64_t z_off,
+               void* var_out, int host_dtype, gpe_step_info* io);
+
+/* One Cash-Karp attempt from the current state with step dt; dt0 is the
+ * step held on entry to rkqs, used for the error scale
+ * psi_scal = |psi| + dt0 |f(psi)| + 1e-30 (src/solve.f90:176-178).  f(psi)
+ * is evaluated on the first attempt after each accept (or upload) and reused
+ * on retries.  out[0] = errmax = max|err/psi_scal|/eps (src/solve.f90:184),
+ * out[1] = max |psi_new|^2.  The candidate is kept until gpe_rk45_accept. */
+int gpe_rk45_attempt(gpe_ctx* ctx, double dt_re, double dt_im, double dt0_re, double dt0_im, double out[2]);
+/* Make the last attempt's candidate the state (src/solve.f90:232). */
+int gpe_rk45_accept(gpe_ctx* ctx);
+
+/* derivs module drop-in (src/derivs.f90:25-272) and laplacian
+ * (src/variables.f90:55-71): out = D_op(in).  in == NULL uses the device
+ * state; otherwise in has the gpe_upload layout.  out is an interior array. */
+int gpe_deriv(gpe_ctx* ctx, int op, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off, void* out,
+              int host_dtype);
+/* get_rhs (src/solve.f90:382-440), same argument conventions as gpe_deriv. */
+int gpe_rhs(gpe_ctx* ctx, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off, void* out,
+            int host_dtype);
+
+/* Reductions of the device state, accumulated in FP64 in a fixed order:
+ * get_norm with the end-corrected weights (src/variables.f90:537-559,
+ * 713-823), energy (563-587), mass (591-611). */
+int gpe_norm(gpe_ctx* ctx, double* norm);
+int gpe_energy(gpe_ctx* ctx, double* energy);
+int gpe_mass(gpe_ctx* ctx, double* mass);
+/* psi <- sqrt(nn) psi / sqrt(norm) (src/variables.f90:698-709). */
+int gpe_renormalise(gpe_ctx* ctx, double norm);
+
+/* Device pointer of the state field (nx*ny*nz complex of the storage
+ * precision) and the context's stream (a hipStream_t). */
+/* Per-launch timing: with profiling on, every stencil launch of ctx is
+ * bracketed by HIP events on the context's stream.  gpe_prof_read returns,
+ * for one kernel kind, the summed duration, the launch count and the
+ * algorithmic HBM bytes of one launch (fields read + written per cell times
+ * the cell count).  No reference counterpart: measurement support. */
+#define GPE_NKINDS 14
+int gpe_prof_enable(gpe_ctx* ctx, int on);
+int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
+const char* gpe_kernel_kind_name(int kind);
+
+/* Plain device copy of nbytes (a multiple of 16) on a HIP stream: the
+ * calibration kernel for HBM byte counters.  No reference counterpart. */
+int gpe_bench_copy(void* dst, const void* src, int64_t nbytes, void* stream);
+
+void* gpe_state_ptr(gpe_ctx* ctx);
+void* gpe_stream(gpe_ctx* ctx);
+int gpe_sync(gpe_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,229 @@
+"""Parity of the HIP kernels (called through the C ABI of libgpehip.so) with
+the CPU oracle on the same seeded inputs.
+
+Tolerances: complex128 storage must agree with the FP64 oracle to 1e-12
+relative L2 (single operators to 1e-13); complex64 storage is compared with
+the FP64 oracle at the accuracy of FP32 arithmetic (stated per test).
+"""
+import numpy as np
+import pytest
+
+from gpe_amd import Context, GpeError, RunParams, StepInfo
+from gpe_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(np.ravel(b)))
+
+
+def rand_field(p, seed=0, amp=1.0):
+    rng = np.random.default_rng(seed)
+    base = np.exp(1j * rng.uniform(0, 2 * np.pi, p.shape)) * (1 + 0.2 * rng.standard_normal(p.shape))
+    return np.ascontiguousarray(amp * base)
+
+
+def smooth_field(p, seed=0):
+    """A smooth non-trivial field: a few random plane waves on a background."""
+    rng = np.random.default_rng(seed)
+    x, y, z = p.axes()
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    psi = np.ones(p.shape, dtype=np.complex128)
+    for _ in range(4):
+        k = 2 * np.pi * rng.integers(-2, 3, size=3) / np.array([p.nx * p.dx, p.ny * p.dy, p.nz * p.dz])
+        psi += 0.2 * (rng.standard_normal() + 1j * rng.standard_normal()) * np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z))
+    return np.ascontiguousarray(psi)
+
+
+SHAPES = [(16, 12, 10), (37, 21, 19), (64, 8, 9), (70, 33, 12)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("bcs", [1, 2])
+def test_derivs_match_oracle(orc, shape, order, bcs):
+    nx, ny, nz = shape
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=3.1, yr=2.2, zr=1.9, order=order, bcs=bcs)
+    f = rand_field(p, seed=nx + ny)
+    with Context(p) as ctx:
+        for name, op in [("x", 0), ("y", 1), ("z", 2), ("xx", 3), ("yy", 4), ("zz", 5), ("lap", 6)]:
+            got = ctx.deriv(name, f)
+            exp = orc.deriv(p, op, f)
+            assert rel(got, exp) < 1e-13, (name, shape, order, bcs)
+
+
+@pytest.mark.parametrize("tiles", [(64, 8, 3), (64, 4, 5), (32, 8, 7), (32, 16, 4), (64, 16, 0), (32, 4, 100)])
+def test_tile_shapes_agree(orc, tiles):
+    p = RunParams(nx=45, ny=29, nz=23, xr=3.0, yr=2.0, zr=2.5, order=4, bcs=1)
+    f = rand_field(p, seed=5)
+    with Context(p) as ctx:
+        ctx.set_tiles(*tiles)
+        assert rel(ctx.deriv("lap", f), orc.deriv(p, 6, f)) < 1e-13
+
+
+@pytest.mark.parametrize("eqn,extra", [
+    (1, {}), (1, {"Urhs": 0.3}), (1, {"diss_amp": 0.02}), (1, {"vscale": 1.0, "omx": 0.2, "omy": 0.1, "omz": 0.3}),
+    (2, {}), (2, {"Urhs": -0.4}), (3, {}), (4, {"g": 1.0, "mu": 10.0, "omx": 0.05, "omy": 0.07, "omz": 0.05}),
+])
+@pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
+def test_rhs_matches_oracle(orc, eqn, extra, order, bcs):
+    p = RunParams(nx=20, ny=18, nz=16, xr=14.0, yr=12.0, zr=13.0, order=order, bcs=bcs, eqn_to_solve=eqn, **extra)
+    f = smooth_field(p, seed=eqn)
+    with Context(p) as ctx:
+        assert rel(ctx.rhs(f), orc.rhs(p, f)) < 1e-13
+
+
+@pytest.mark.parametrize("scheme", ["euler", "rk2", "rk4"])
+@pytest.mark.parametrize("order,bcs,eqn", [(2, 1, 1), (4, 2, 4), (4, 1, 2)])
+def test_fixed_steppers_match_oracle(orc, scheme, order, bcs, eqn):
+    p = RunParams(nx=24, ny=20, nz=18, xr=8.0, yr=7.0, zr=6.0, order=order, bcs=bcs, eqn_to_solve=eqn,
+                  g=1.0, mu=1.0, omx=0.1, omy=0.1, omz=0.1)
+    psi = smooth_field(p, seed=2)
+    dt = 0.01 + 0j
+    exp = psi
+    t = (0.0, 0.0)
+    for _ in range(5):
+        exp, t = orc.step(p, scheme, exp, dt, t)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        info = StepInfo(dt)
+        ctx.step(scheme, 5, info)
+        got = ctx.download()
+    assert rel(got, exp) < 1e-12
+    assert abs(info.t - t[0]) < 1e-15 and info.im_t == t[1]
+
+
+def test_imaginary_time_rk4(orc):
+    p = RunParams(nx=20, ny=20, nz=20, xr=6.0, yr=6.0, zr=6.0, order=4, bcs=2, eqn_to_solve=4, g=1.0, mu=2.0,
+                  omx=0.3, omy=0.3, omz=0.3, real_time=False, tau=0.005)
+    psi = np.ascontiguousarray(ics.fermi(p) + 0.01)
+    exp, t = orc.rk4_n(p, psi, p.dt0(), 10)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        info = StepInfo(p.dt0())
+        ctx.step("rk4", 10, info)
+        got = ctx.download()
+    assert rel(got, exp) < 1e-12
+    assert abs(info.im_t - t[1]) < 1e-15
+
+
+def test_ring64_rk4_200_steps(orc):
+    """The 64^3 plain method: periodic, dx = 0.5, 2nd order, eqn 1, RK4
+    dt = 0.01 for 200 steps from the noisy vortex ring; <= 1e-12 relative L2."""
+    p, psi = ics.config_ring64(seed=0)
+    exp, t = orc.rk4_n(p, psi, 0.01 + 0j, 200)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        info = StepInfo(0.01 + 0j)
+        ctx.step("rk4", 200, info)
+        got = ctx.download()
+        n_gpu = ctx.norm()
+    err = rel(got, exp)
+    print(f"ring64 rk4 200 steps: rel L2 = {err:.3e}")
+    assert err <= 1e-12
+    assert abs(n_gpu - orc.norm(p, exp)) <= 1e-12 * abs(n_gpu)
+    assert abs(info.t - t[0]) < 1e-12
+
+
+@pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
+def test_rkck_attempt_and_rkqs_match_oracle(orc, order, bcs):
+    p = RunParams(nx=24, ny=22, nz=20, xr=8.0, yr=8.0, zr=8.0, order=order, bcs=bcs, eqn_to_solve=1)
+    psi = smooth_field(p, seed=4)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        # single attempt: errmax and the candidate
+        new_o, err_o = orc.rkck(p, psi, 0.05 + 0j)
+        f0 = orc.rhs(p, psi)
+        scal = np.abs(psi) + 0.05 * np.abs(f0) + 1e-30
+        em_o = np.max(np.abs(err_o) / np.abs(scal)) / 1e-6
+        em, a2 = ctx.rk45_attempt(0.05 + 0j, 0.05 + 0j)
+        assert abs(em - em_o) <= 1e-9 * em_o
+        assert abs(a2 - np.max(np.abs(new_o) ** 2)) <= 1e-12 * a2
+        ctx.rk45_accept()
+        assert rel(ctx.download(), new_o) < 1e-13
+        # several rkqs steps from a too-large dt (forces rejections)
+        ctx.upload(psi)
+        info = StepInfo(0.4 + 0j)
+        cur, dt, t = psi, 0.4 + 0j, (0.0, 0.0)
+        nrej_o = 0
+        for _ in range(4):
+            cur, dt, t, em, nrej, _ = orc.rkqs(p, cur, dt, t)
+            nrej_o += nrej
+        ctx.step("rk45", 4, info)
+        got = ctx.download()
+    assert info.n_reject == nrej_o and nrej_o > 0
+    assert abs(info.dt - dt) <= 1e-9 * abs(dt)
+    assert abs(info.t - t[0]) <= 1e-12
+    assert rel(got, cur) < 1e-12
+
+
+def test_reductions_match_oracle(orc):
+    for order, bcs in [(2, 1), (4, 2)]:
+        p = RunParams(nx=33, ny=27, nz=19, xr=5.0, yr=4.0, zr=3.0, order=order, bcs=bcs, nn=2.5)
+        psi = smooth_field(p, seed=7)
+        with Context(p) as ctx:
+            ctx.upload(psi)
+            assert abs(ctx.norm() - orc.norm(p, psi)) <= 1e-13 * orc.norm(p, psi)
+            assert abs(ctx.mass() - orc.mass(p, psi)) <= 1e-13 * orc.mass(p, psi)
+            e = orc.energy(p, psi)
+            assert abs(ctx.energy() - e) <= 1e-12 * abs(e)
+            nrm = ctx.norm()
+            ctx.renormalise(nrm)
+            assert rel(ctx.download(), orc.renormalise(p, psi, nrm)) < 1e-15
+
+
+def test_halo_array_upload_and_c64_host(orc):
+    p = RunParams(nx=16, ny=12, nz=10, xr=3.0, yr=3.0, zr=3.0, order=4, bcs=1)
+    psi = rand_field(p, seed=9)
+    halo = np.zeros((p.nz + 4, p.ny + 4, p.nx), dtype=np.complex128)
+    halo[2:-2, 2:-2, :] = psi
+    with Context(p) as ctx:
+        ctx.upload(halo)
+        assert np.array_equal(ctx.download(), psi)
+        assert rel(ctx.deriv("lap", halo), orc.deriv(p, 6, psi)) < 1e-13
+        ctx.upload(psi.astype(np.complex64))
+        assert np.array_equal(ctx.download(np.complex64), psi.astype(np.complex64))
+
+
+def test_c64_storage_drift_against_fp64_oracle(orc):
+    """complex64 storage / FP32 stencil against the FP64 oracle: 20 RK4 steps
+    of the 64^3 ring agree to 1e-5 relative L2; the norm (FP64 reduction of
+    FP32 values) to 1e-6."""
+    p, psi = ics.config_ring64(seed=1)
+    exp, _ = orc.rk4_n(p, psi, 0.01 + 0j, 20)
+    with Context(p.replace(storage="c64")) as ctx:
+        ctx.upload(psi)
+        ctx.step("rk4", 20, StepInfo(0.01 + 0j))
+        got = ctx.download()
+        nrm = ctx.norm()
+    err = rel(got, exp)
+    print(f"c64 drift after 20 steps: {err:.3e}")
+    assert err < 1e-5
+    assert abs(nrm - orc.norm(p, exp)) < 1e-6 * nrm
+
+
+def test_deterministic_reruns():
+    p, psi = ics.config_ring64(seed=2)
+    outs = []
+    for _ in range(2):
+        with Context(p.replace(scheme="rk45")) as ctx:
+            ctx.upload(psi)
+            info = StepInfo(0.05 + 0j)
+            ctx.step("rk45", 3, info)
+            outs.append((ctx.download(), info.dt, info.errmax, ctx.energy(), ctx.norm()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1:] == outs[1][1:]
+
+
+def test_error_paths():
+    p = RunParams(nx=16, ny=16, nz=16)
+    with Context(p) as ctx:
+        with pytest.raises(GpeError, match="Unrecognised"):
+            ctx._chk(ctx.lib.gpe_step(ctx.h, 9, 1, StepInfo().c), "gpe_step")
+        with pytest.raises(GpeError, match="no attempt"):
+            ctx.rk45_accept()
+        with pytest.raises(GpeError):
+            ctx.set_tiles(48, 8, 0)
+        with pytest.raises(ValueError):
+            ctx.upload(np.zeros((3, 3, 3), dtype=np.complex128))
