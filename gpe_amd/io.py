@@ -1,0 +1,105 @@
+"""The reference's end_state restart file (src/io.f90:840-891, read back by
+state_restart src/ic.f90:117-186): stream-unformatted int32 nx, ny, nz, p;
+float64 t, im_t; complex128 dt; complex128 psi (x fastest).  Used to hand
+initial states to the Fortran host (and to the reference build) and to read
+their final states back."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def write_end_state(path: str, psi: np.ndarray, p: int = 0, t: float = 0.0, im_t: float = 0.0,
+                    dt: complex = 0j, extra: tuple = ()) -> None:
+    nz, ny, nx = psi.shape
+    with open(path, "wb") as f:
+        np.array([nx, ny, nz, p], dtype="<i4").tofile(f)
+        np.array([t, im_t, dt.real, dt.imag], dtype="<f8").tofile(f)
+        np.ascontiguousarray(psi, dtype="<c16").tofile(f)
+        if extra:
+            np.array(extra, dtype="<f8").tofile(f)
+
+
+def read_end_state(path: str, n_extra: int = 0):
+    """Returns (psi, p, t, im_t, dt[, extras])."""
+    with open(path, "rb") as f:
+        nx, ny, nz, p = (int(v) for v in np.fromfile(f, dtype="<i4", count=4))
+        t, im_t, dre, dim = np.fromfile(f, dtype="<f8", count=4)
+        psi = np.fromfile(f, dtype="<c16", count=nx * ny * nz).reshape(nz, ny, nx)
+        out = (psi, p, float(t), float(im_t), complex(dre, dim))
+        if n_extra:
+            out = out + (np.fromfile(f, dtype="<f8", count=n_extra),)
+    return out
+
+
+def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10) -> str:
+    """run.in with the reference's four namelists (src/io.f90:78-114) plus
+    grid_params for the Fortran host of this package."""
+    def b(v):
+        return ".true." if v else ".false."
+
+    def r(v):
+        return repr(float(v))
+
+    return f"""&run_params
+  tau = {r(p.tau)}
+  end_time = {r(end_time)}
+  xr = {r(p.xr)}
+  yr = {r(p.yr)}
+  zr = {r(p.zr)}
+  scheme = '{p.scheme}'
+  eqn_to_solve = {p.eqn_to_solve}
+  bcs = {p.bcs}
+  order = {p.order}
+  restart = {b(restart)}
+  saved_restart = .false.
+  renorm = .false.
+  imprint_vl = .false.
+  stop_imag = .false.
+  real_time = {b(p.real_time)}
+/
+&eqn_params
+  Urhs = {r(p.Urhs)}
+  diss_amp = {r(p.diss_amp)}
+  scal = 1.0
+  nv = 0.75
+  enerv = 0.75
+  g = {r(p.g)}
+  mu = {r(p.mu)}
+  nn = {r(p.nn)}
+  omx = {r(p.omx)}
+  omy = {r(p.omy)}
+  omz = {r(p.omz)}
+/
+&io_params
+  save_rate = {save_rate}
+  save_rate2 = 1.0e30
+  save_rate3 = 1.0e30
+  p_save = 1.0e30
+  save_contour = .false.
+  save_3d = .false.
+  save_filter = .false.
+  filter_kc = 10.0
+  save_average = .false.
+  save_spectrum = .false.
+  save_pdf = .false.
+  save_vcf = .false.
+  save_ll = .false.
+  save_zeros = .false.
+/
+&misc_params
+  nbins = 256
+  diagnostic = .false.
+  pp_filtered_surface = .false.
+  nlines = 3
+  nfilter = 1
+  fscale = 1.0
+/
+&grid_params
+  nx = {p.nx}
+  ny = {p.ny}
+  nz = {p.nz}
+  storage = '{p.storage}'
+  vscale = {r(p.vscale)}
+  host_mode = 'resident'
+/
+"""

@@ -1,0 +1,63 @@
+#!/bin/bash
+# Builds the reference Fortran time-stepping path (oracle/_ref), a checker
+# for the CPU oracle.  The reference modules on the path -- parameters,
+# constants, error, derivs, variables, ic, solve -- are compiled UNMODIFIED
+# from /root/reference (through the examples/ring symlinks, whose directory
+# holds no parameters.in, so the compile-time configuration written here is
+# found on the include path, as the reference's own setup script intends).
+# mpif.h and the MPI library are the image's MPICH (/opt/conda).  The
+# driver is oracle/ref/ref_driver.f90 (this project's harness).  Unused
+# code (FFT diagnostics, IC generators) is dropped at link time with
+# --gc-sections, so the absent FFTW2 library is never referenced.
+# Outputs go to oracle/_ref/<nx>x<ny>x<nz>/ref_driver only.
+set -euo pipefail
+REF=${GPE_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$HERE/_ref
+FC=${FC:-/opt/rocm/lib/llvm/bin/amdflang}
+MPI=${MPI_HOME:-/opt/conda}
+[ -d "$REF/examples/ring" ] || { echo "build_ref.sh: $REF not present, skipping"; exit 0; }
+
+build_grid() {
+  local nx=$1 ny=$2 nz=$3
+  local dir=$OUT/${nx}x${ny}x${nz}
+  mkdir -p "$dir"
+  cat > "$dir/parameters.in" <<PIN
+  integer, parameter :: pr = selected_real_kind(15,307)
+  integer, parameter :: nyprocs = 1
+  integer, parameter :: nzprocs = 1
+  integer, parameter :: nx      = $nx
+  integer, parameter :: ny      = $ny
+  integer, parameter :: nz      = $nz
+  type :: line_param
+    real (pr)    :: x0, y0, z0, amp1, amp2, ll, sgn
+    character(1) :: dir
+    logical      :: imprint_phase
+  end type line_param
+  type (line_param), parameter :: vl1 = line_param(0.0_pr, 0.0_pr, 0.0_pr, &
+    0.0_pr, 0.0_pr, 0.0_pr, 1.0_pr, 'z', .false.)
+  type :: ring_param
+    character(2) :: plane
+    real (pr)    :: x0, y0, z0, r0, amp
+    integer      :: mm
+    real (pr)    :: r1
+    integer      :: kk
+    real (pr)    :: dir
+  end type ring_param
+  type (ring_param), parameter :: vr1 = ring_param('xy', 0.0_pr, 0.0_pr, &
+    0.0_pr, 8.0_pr, 0.0_pr, 5, 0.0_pr, 10, 1.0_pr)
+PIN
+  # the run starts from a restart file, so the compile-time IC is a constant
+  echo "      init_cond = cmplx(1.0_pr, 0.0_pr, pr)" > "$dir/ic.in"
+  local flags="-O2 -flto -I$dir -I$MPI/include -module-dir $dir"
+  for m in parameters constants error derivs ic variables solve; do
+    $FC $flags -c "$REF/examples/ring/$m.f90" -o "$dir/$m.o"
+  done
+  $FC $flags -c "$HERE/ref/ref_driver.f90" -o "$dir/ref_driver.o"
+  $FC -O2 -flto -o "$dir/ref_driver" "$dir"/{parameters,constants,error,derivs,ic,variables,solve,ref_driver}.o \
+      -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -flto -fuse-ld=lld
+  echo "built $dir/ref_driver"
+}
+
+build_grid 64 64 64
+build_grid 24 20 18

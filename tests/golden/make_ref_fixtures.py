@@ -1,0 +1,116 @@
+"""Writes tests/golden/ref_*.npz: outputs of the reference Fortran
+time-stepping path (compiled unmodified by oracle/build_ref.sh into
+oracle/_ref) on seeded inputs.  Each fixture holds the input field, the run
+parameters, the reference's final field, t, im_t, dt, p, its norm and energy
+(get_norm, energy) and the per-step (t, im_t, dt) trace.
+
+Run in the build container (the reference is not present on the GPU box):
+    python tests/golden/make_ref_fixtures.py
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from gpe_amd import RunParams, ics  # noqa: E402
+from gpe_amd.io import read_end_state, run_in_text, write_end_state  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def smooth_field(p, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = p.axes()
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    psi = np.ones(p.shape, dtype=np.complex128)
+    for _ in range(4):
+        k = 2 * np.pi * rng.integers(-2, 3, size=3) / np.array([p.nx * p.dx, p.ny * p.dy, p.nz * p.dz])
+        psi += 0.2 * (rng.standard_normal() + 1j * rng.standard_normal()) * np.exp(1j * (k[0] * X + k[1] * Y + k[2] * Z))
+    return np.ascontiguousarray(psi)
+
+
+def run_reference(name, p, psi, nsteps=None, end_time=None):
+    exe = os.path.join(REF, f"{p.nx}x{p.ny}x{p.nz}", "ref_driver")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_ref.sh first")
+    run = os.path.join(REF, "run_" + name)
+    shutil.rmtree(run, ignore_errors=True)
+    os.makedirs(run)
+    if end_time is None:
+        end_time = (nsteps - 0.5) * p.tau  # exactly nsteps steps of the loop t+im_t <= end_time
+    with open(os.path.join(run, "run.in"), "w") as f:
+        f.write(run_in_text(p, end_time))
+    write_end_state(os.path.join(run, "in.dat"), psi, 0, 0.0, 0.0, p.dt0())
+    t0 = time.perf_counter()
+    # the reference keeps its work arrays on the stack (automatic arrays)
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {exe} in.dat out.dat trace.dat"], cwd=run, check=True,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+    secs = time.perf_counter() - t0
+    out, pp, t, im_t, dt, extra = read_end_state(os.path.join(run, "out.dat"), 2)
+    tr = np.fromfile(os.path.join(run, "trace.dat"), dtype="<f8").reshape(-1, 4)
+    shutil.rmtree(run)
+    print(f"{name}: {pp} steps, t={t:.6f}, im_t={im_t:.6f}, {secs:.1f} s", flush=True)
+    return dict(out=out, p=pp, t=t, im_t=im_t, dt=dt, norm=extra[0], energy=extra[1], trace=tr, secs=secs)
+
+
+def save(name, p, psi, res, store_input=True):
+    d = dict(params=json.dumps(dataclasses.asdict(p)), out=res["out"], p=res["p"], t=res["t"], im_t=res["im_t"],
+             dt=res["dt"], norm=res["norm"], energy=res["energy"], trace=res["trace"], ref_seconds=res["secs"])
+    if store_input:
+        d["inp"] = psi
+    np.savez_compressed(os.path.join(OUT, f"ref_{name}.npz"), **d)
+
+
+def small_cases():
+    base = dict(nx=24, ny=20, nz=18, xr=8.0, yr=7.0, zr=6.5)
+    cases = [
+        ("eq1_per_o2_rk4", RunParams(**base, order=2, bcs=1, eqn_to_solve=1, scheme="rk4", tau=0.01), 20, 1),
+        ("eq4_ref_o4_rk4", RunParams(**base, order=4, bcs=2, eqn_to_solve=4, scheme="rk4", tau=0.005, g=1.0, mu=2.0,
+                                     omx=0.3, omy=0.25, omz=0.2), 20, "trap"),
+        ("eq2_per_o4_euler", RunParams(**base, order=4, bcs=1, eqn_to_solve=2, scheme="euler", tau=0.002), 10, 3),
+        ("eq1_ref_o2_rk2_u_diss", RunParams(**base, order=2, bcs=2, eqn_to_solve=1, scheme="rk2", tau=0.01, Urhs=0.3,
+                                            diss_amp=0.02), 10, 4),
+        ("eq4_ref_o2_imag_rk4", RunParams(**base, order=2, bcs=2, eqn_to_solve=4, scheme="rk4", tau=0.005, g=1.0,
+                                          mu=2.0, omx=0.3, omy=0.3, omz=0.3, real_time=False), 10, "trap"),
+        ("eq3_per_o4_rk4", RunParams(nx=24, ny=20, nz=18, xr=14.0, yr=13.0, zr=15.0, order=4, bcs=1, eqn_to_solve=3,
+                                     scheme="rk4", tau=0.01), 10, 6),
+    ]
+    for name, p, n, seed in cases:
+        if seed == "trap":
+            psi = np.ascontiguousarray(ics.fermi(p) * ics.vortex_line_z(p) + 0.01)
+        else:
+            psi = smooth_field(p, seed)
+        save(name, p, psi, run_reference(name, p, psi, nsteps=n))
+    # rk45 from a too-large first dt (rejections), to t ~ 0.6
+    p = RunParams(**base, order=4, bcs=1, eqn_to_solve=1, scheme="rk45", tau=0.4)
+    psi = smooth_field(p, 7)
+    save("eq1_per_o4_rk45", p, psi, run_reference("eq1_per_o4_rk45", p, psi, end_time=0.6))
+    p = RunParams(**base, order=2, bcs=2, eqn_to_solve=4, scheme="rk45", tau=0.05, g=1.0, mu=2.0, omx=0.3, omy=0.25,
+                  omz=0.2)
+    psi = np.ascontiguousarray(ics.fermi(p) * ics.vortex_line_z(p) + 0.01)
+    save("eq4_ref_o2_rk45", p, psi, run_reference("eq4_ref_o2_rk45", p, psi, end_time=0.5))
+
+
+def ring64():
+    p, psi = ics.config_ring64(seed=0)
+    res = run_reference("ring64_rk4_200", p, psi, nsteps=200)
+    save("ring64_rk4_200", p, psi, res, store_input=False)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["small", "ring64"]
+    if "small" in which:
+        small_cases()
+    if "ring64" in which:
+        ring64()
