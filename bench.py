@@ -51,6 +51,27 @@ class Hip:
     def device_sync(self):
         assert self.h.hipDeviceSynchronize() == 0
 
+    def malloc(self, nbytes):
+        p = ct.c_void_p()
+        assert self.h.hipMalloc(ct.byref(p), ct.c_size_t(nbytes)) == 0
+        return p
+
+    def free(self, p):
+        assert self.h.hipFree(p) == 0
+
+
+def copy_probe(hip, nbytes):
+    """One plain copy kernel of known size (read nbytes + write nbytes): the
+    calibration point for the FETCH_SIZE / WRITE_SIZE counters."""
+    from gpe_amd import _lib
+
+    lib = _lib.load()
+    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
+    assert lib.gpe_bench_copy(b, a, nbytes, None) == 0
+    hip.device_sync()
+    hip.free(a)
+    hip.free(b)
+
 
 def timed_steps(ctx, hip, scheme, steps, info):
     """K steps between two events on the context's stream, bracketed by device syncs."""
@@ -135,6 +156,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
     ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one plain copy kernel of this size first (counter calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -148,6 +171,8 @@ def main(argv=None) -> int:
     p, psi = ics.config_turb(n=args.n, storage="c128", seed=0, scheme="rk4", tau=0.005)
     print(f"[bench] IC {args.n}^3 built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     hip = Hip()
+    if args.copy_probe_mib:
+        copy_probe(hip, args.copy_probe_mib << 20)
 
     ms, prof, tiles = run_storage(p, psi, args, hip)
     print(f"[bench] c128: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)

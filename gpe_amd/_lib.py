@@ -68,6 +68,7 @@ PROTOTYPES = [
     ("gpe_prof_read", _I, [_VP, _I, ct.POINTER(_D), ct.POINTER(_I64), ct.POINTER(_D)]),
     ("gpe_kernel_kind_name", ct.c_char_p, [_I]),
     ("gpe_bench_copy", _I, [_VP, _VP, _I64, _VP]),
+    ("gpe_bench_stream5", _I, [_VP, _VP, _VP, _VP, _VP, _I64, _VP]),
     ("gpe_state_ptr", _VP, [_VP]),
     ("gpe_stream", _VP, [_VP]),
     ("gpe_sync", _I, [_VP]),

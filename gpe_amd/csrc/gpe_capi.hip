@@ -7,6 +7,7 @@
 
 #include <string>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gpe_hip.h"
@@ -61,6 +62,9 @@ struct gpe_ctx {
     };
     bool prof = false;
     std::vector<ProfRec> recs;
+    // allocation bases of offset fields (see dalloc)
+    std::unordered_map<void*, void*> bases;
+    int nalloc = 0;
 };
 
 static const char* const kKindNames[GPE_NKINDS] = {
@@ -80,15 +84,33 @@ static int fail(gpe_ctx* c, int code, const std::string& msg) {
             return fail(ctx, GPE_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));                         \
     } while (0)
 
+// Field allocations.  With GPE_PAD_FIELDS each allocation starts at a
+// different offset (a multiple of 64 KiB + 256 B) so that equal indices of
+// different fields do not share cache sets.
+#ifndef GPE_PAD_FIELDS
+#define GPE_PAD_FIELDS 1
+#endif
 static int dalloc(gpe_ctx* c, void** p, size_t bytes, const char* what) {
     if (*p) return 0;
-    hipError_t e = hipMalloc(p, bytes);
+    const size_t pad = GPE_PAD_FIELDS ? (size_t)(c->nalloc % 16) * (65536 + 256) : 0;
+    void* base = nullptr;
+    hipError_t e = hipMalloc(&base, bytes + (GPE_PAD_FIELDS ? (1u << 20) : 0));
     if (e != hipSuccess) {
         *p = nullptr;
         return fail(c, GPE_ENOMEM, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) + " B): " +
                                        hipGetErrorString(e));
     }
+    *p = (char*)base + pad;
+    c->bases[*p] = base;
+    c->nalloc++;
     return 0;
+}
+
+static void dfree(gpe_ctx* c, void* p) {
+    if (!p) return;
+    auto it = c->bases.find(p);
+    (void)hipFree(it != c->bases.end() ? it->second : p);
+    if (it != c->bases.end()) c->bases.erase(it);
 }
 
 static size_t field_bytes(const gpe_ctx* c) { return (size_t)c->n * c->esz; }
@@ -107,15 +129,22 @@ static int validate(const gpe_params* p, std::string& why) {
     return 0;
 }
 
-static Tiles default_tiles(const gpe_params& p) {
-    Tiles t{64, 8, 0};
-    if (p.nx <= 32) t.tx = 32;
-    const int ntxy = ((p.nx + t.tx - 1) / t.tx) * ((p.ny + t.ty - 1) / t.ty);
-    // aim for ~2048 blocks while keeping the z-halo re-read small
-    long long tz = ((long long)p.nz * ntxy) / 2048;
-    if (tz < 8) tz = 8;
+// Default tiling: 64x16 tiles for wide planes (one 1024-thread block per CU
+// marching long z-ranges, so neighbouring tiles advance together and share
+// halo rows through L2), 32x8 tiles for small planes; tz is chosen so the
+// launch has at least 256 blocks (one per CU).
+static int tz_for(const gpe_params& p, int tx, int ty) {
+    const long long ntxy = (long long)((p.nx + tx - 1) / tx) * ((p.ny + ty - 1) / ty);
+    long long tz = (p.nz * ntxy) / 256;
+    if (tz < 4) tz = 4;
     if (tz > p.nz) tz = p.nz;
-    t.tz = (int)tz;
+    return (int)tz;
+}
+
+static Tiles default_tiles(const gpe_params& p) {
+    Tiles t{64, 16, 0};
+    if ((long long)p.nx * p.ny < 128LL * 128LL) t = Tiles{32, 8, 0};
+    t.tz = tz_for(p, t.tx, t.ty);
     return t;
 }
 
@@ -216,7 +245,7 @@ template <typename R> static CkTab<R> make_tab() {
 
 static int ensure_partial(gpe_ctx* c, size_t ndoubles) {
     if (c->partial_cap >= ndoubles) return 0;
-    if (c->partial) (void)hipFree(c->partial);
+    dfree(c, c->partial);
     c->partial = nullptr;
     c->partial_cap = 0;
     int rc = dalloc(c, (void**)&c->partial, ndoubles * sizeof(double), "partials");
@@ -419,7 +448,7 @@ static size_t host_esz(int host_dtype) { return host_dtype == GPE_HOST_C64 ? 8 :
 
 static int ensure_stage(gpe_ctx* c, size_t bytes) {
     if (c->stage_bytes >= bytes) return 0;
-    if (c->stage) (void)hipFree(c->stage);
+    dfree(c, c->stage);
     c->stage = nullptr;
     c->stage_bytes = 0;
     int rc = dalloc(c, &c->stage, bytes, "staging");
@@ -619,8 +648,7 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     prof_clear(c);
     void* bufs[] = {c->psi, c->uA, c->uB, c->acc, c->f0, c->k2, c->k3, c->k4, c->k5, c->pnew,
                     c->sin, c->sout, c->stage, c->partial, c->dres, c->dtab};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
+    for (void* b : bufs) dfree(c, b);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -631,13 +659,7 @@ int gpe_set_tiles(gpe_ctx* c, int tx, int ty, int tz) {
     if (!tile_supported(tx, ty)) return fail(c, GPE_EINVAL, "gpe_set_tiles: unsupported tile shape");
     if (tz < 0) return fail(c, GPE_EINVAL, "gpe_set_tiles: tz must be >= 0");
     Tiles t{tx, ty, tz};
-    if (tz == 0) {
-        Tiles d = default_tiles(c->p);
-        const int ntxy = ((c->p.nx + tx - 1) / tx) * ((c->p.ny + ty - 1) / ty);
-        long long z = ((long long)c->p.nz * ntxy) / 2048;
-        t.tz = (int)(z < 8 ? 8 : z > c->p.nz ? c->p.nz : z);
-        (void)d;
-    }
+    if (tz == 0) t.tz = tz_for(c->p, tx, ty);
     if (t.tz > c->p.nz) t.tz = c->p.nz;
     c->tiles = t;
     return 0;
@@ -805,6 +827,15 @@ int gpe_bench_copy(void* dst, const void* src, int64_t nbytes, void* stream) {
         return fail(nullptr, GPE_EINVAL, "gpe_bench_copy: pointers must be non-NULL and nbytes a multiple of 16");
     hipError_t e = launch_copy16(dst, src, nbytes / 16, (hipStream_t)stream);
     if (e != hipSuccess) return fail(nullptr, GPE_EHIP, std::string("gpe_bench_copy: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int gpe_bench_stream5(void* c_out, void* d_out, const void* a, const void* b, const void* e, int64_t nbytes,
+                      void* stream) {
+    if (!c_out || !d_out || !a || !b || !e || nbytes < 0 || (nbytes % 16) != 0)
+        return fail(nullptr, GPE_EINVAL, "gpe_bench_stream5: pointers must be non-NULL and nbytes a multiple of 16");
+    hipError_t err = launch_stream5(c_out, d_out, a, b, e, nbytes / 16, (hipStream_t)stream);
+    if (err != hipSuccess) return fail(nullptr, GPE_EHIP, std::string("gpe_bench_stream5: ") + hipGetErrorString(err));
     return 0;
 }
 

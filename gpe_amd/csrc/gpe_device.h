@@ -53,6 +53,33 @@ template <typename R> __device__ __forceinline__ C<R> caxpy(R s, C<R> a, C<R> c)
     return mkc<R>(c.x + s * a.x, c.y + s * a.y);
 }
 
+// Streaming (non-temporal) access for the pointwise fields of a stage: they
+// are touched once per kernel, so they bypass cache retention and leave L2 to
+// the stencil field's halo re-reads.
+#ifndef GPE_STREAM_NT
+#define GPE_STREAM_NT 1
+#endif
+template <typename R> struct vec2;
+template <> struct vec2<double> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct vec2<float> { typedef float T __attribute__((ext_vector_type(2))); };
+
+template <typename R> __device__ __forceinline__ C<R> ld_stream(const C<R>* p) {
+#if GPE_STREAM_NT
+    const typename vec2<R>::T v = __builtin_nontemporal_load((const typename vec2<R>::T*)p);
+    return mkc<R>(v.x, v.y);
+#else
+    return *p;
+#endif
+}
+template <typename R> __device__ __forceinline__ void st_stream(C<R>* p, C<R> a) {
+#if GPE_STREAM_NT
+    typename vec2<R>::T v = {a.x, a.y};
+    __builtin_nontemporal_store(v, (typename vec2<R>::T*)p);
+#else
+    *p = a;
+#endif
+}
+
 // Map a global index into [0, n): periodic wrap (bcs = 1) or even reflection
 // about the end nodes (bcs = 2): -1 -> 1, -2 -> 2, n -> n-2, n+1 -> n-3.
 // Reflection is periodic with period 2(n-1), which makes the map total.

@@ -33,8 +33,24 @@ template <int NT, int MODE> __device__ __forceinline__ double block_reduce(doubl
 //   kNRed (0..2 reduction values), kMode (RED_SUM / RED_MAX),
 //   struct Pre; Pre load(long long i) const   -- pointwise inputs of cell i
 //   void apply(long long i, int x, int y, int z, const Sten& st, const Pre& p, double* red) const
+// GPE_WAVES_PER_EU > 0 asks the compiler for at least that many waves per
+// SIMD (caps VGPRs); GPE_PW_PREFETCH = 1 prefetches the pointwise inputs of
+// the next plane (double-buffered registers), 0 loads them at the top of the
+// plane's iteration.
+#ifndef GPE_WAVES_PER_EU
+#define GPE_WAVES_PER_EU 0
+#endif
+#ifndef GPE_PW_PREFETCH
+#define GPE_PW_PREFETCH 1
+#endif
+#if GPE_WAVES_PER_EU > 0
+#define GPE_OCC __attribute__((amdgpu_waves_per_eu(GPE_WAVES_PER_EU)))
+#else
+#define GPE_OCC
+#endif
+
 template <typename R, int ORD, int TX, int TY, class Op>
-__global__ __launch_bounds__(TX* TY) void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
+__global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
                                                      double* __restrict__ partial) {
     constexpr int H = ORD / 2, NT = TX * TY, LW = TX + 2 * H, LH = TY + 2 * H;
     constexpr int NHALO = 2 * H * TX + 2 * H * TY;
@@ -96,12 +112,13 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Geom g, StW<R> w, const C<R>
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
             if (hv[s]) hr[s] = u[g.plane * z0 + hoff[s]];
-        if (valid) pc = op.load(g.plane * z0 + own);
+        if (GPE_PW_PREFETCH && valid) pc = op.load(g.plane * z0 + own);
     }
 
     double red[2] = {0.0, 0.0};
     for (int z = z0; z < z1; ++z) {
         q[2 * H] = qn;
+        if (!GPE_PW_PREFETCH && valid) pc = op.load(g.plane * z + own);
         C<R>* L = lds[(z - z0) & 1];
         L[lc] = q[H];
 #pragma unroll
@@ -114,7 +131,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Geom g, StW<R> w, const C<R>
 #pragma unroll
             for (int s = 0; s < NSLOT; ++s)
                 if (hv[s]) hr[s] = u[g.plane * (z + 1) + hoff[s]];
-            if (valid) pn = op.load(g.plane * (z + 1) + own);
+            if (GPE_PW_PREFETCH && valid) pn = op.load(g.plane * (z + 1) + own);
         }
         if (valid) {
             Sten<R, ORD, LW> st;
@@ -125,7 +142,7 @@ __global__ __launch_bounds__(TX* TY) void k_stencil(Geom g, StW<R> w, const C<R>
             st.w = w;
             op.apply(g.plane * z + own, x, y, z, st, pc, red);
         }
-        pc = pn;
+        if (GPE_PW_PREFETCH) pc = pn;
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) q[d] = q[d + 1];
     }
@@ -167,7 +184,7 @@ template <typename R> struct OpAxpySelf {
     C<R> a;
     __device__ Pre load(long long) const { return {}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
-        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), st.c());
+        st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), st.c()));
     }
 };
 
@@ -180,9 +197,9 @@ template <typename R> struct OpAxpyP0 {
     const C<R>* __restrict__ p0;
     C<R>* __restrict__ out;
     C<R> a;
-    __device__ Pre load(long long i) const { return {p0[i]}; }
+    __device__ Pre load(long long i) const { return {ld_stream<R>(p0 + i)}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
-        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.p0);
+        st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.p0));
     }
 };
 
@@ -199,8 +216,8 @@ template <typename R> struct OpRk4First {
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
         const C<R> f = rhs_eval<R>(k, st, x, y, z);
         const C<R> c = st.c();
-        unext[i] = cfma<R>(a1, f, c);
-        acc[i] = cfma<R>(a2, f, c);
+        st_stream<R>(unext + i, cfma<R>(a1, f, c));
+        st_stream<R>(acc + i, cfma<R>(a2, f, c));
     }
 };
 
@@ -215,11 +232,11 @@ template <typename R> struct OpRk4Mid {
     C<R>* acc;
     C<R>* __restrict__ unext;
     C<R> a1, a2;
-    __device__ Pre load(long long i) const { return {p0[i], acc[i]}; }
+    __device__ Pre load(long long i) const { return {ld_stream<R>(p0 + i), ld_stream<R>(acc + i)}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         const C<R> f = rhs_eval<R>(k, st, x, y, z);
-        unext[i] = cfma<R>(a1, f, p.p0);
-        acc[i] = cfma<R>(a2, f, p.acc);
+        st_stream<R>(unext + i, cfma<R>(a1, f, p.p0));
+        st_stream<R>(acc + i, cfma<R>(a2, f, p.acc));
     }
 };
 
@@ -232,9 +249,9 @@ template <typename R> struct OpRk4Last {
     const C<R>* __restrict__ acc;
     C<R>* __restrict__ out;
     C<R> a;
-    __device__ Pre load(long long i) const { return {acc[i]}; }
+    __device__ Pre load(long long i) const { return {ld_stream<R>(acc + i)}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
-        out[i] = cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.acc);
+        st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.acc));
     }
 };
 
@@ -258,9 +275,9 @@ template <typename R> struct OpCk1 {
     __device__ Pre load(long long) const { return {}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
         const C<R> f = rhs_eval<R>(k, st, x, y, z);
-        f0[i] = f;
+        st_stream<R>(f0 + i, f);
         const C<R> k1 = cmul<R>(dt, f);
-        unext[i] = caxpy<R>(b21, k1, st.c());
+        st_stream<R>(unext + i, caxpy<R>(b21, k1, st.c()));
     }
 };
 
@@ -282,17 +299,17 @@ template <typename R, int S_> struct OpCkMid {
     CkTab<R> t;
     __device__ Pre load(long long i) const {
         Pre p;
-        p.p0 = p0[i];
-        p.f0 = f0[i];
-        if constexpr (S_ >= 3) p.k2 = k2[i];
-        if constexpr (S_ >= 4) p.k3 = k3[i];
-        if constexpr (S_ >= 5) p.k4 = k4[i];
+        p.p0 = ld_stream<R>(p0 + i);
+        p.f0 = ld_stream<R>(f0 + i);
+        if constexpr (S_ >= 3) p.k2 = ld_stream<R>(k2 + i);
+        if constexpr (S_ >= 4) p.k3 = ld_stream<R>(k3 + i);
+        if constexpr (S_ >= 5) p.k4 = ld_stream<R>(k4 + i);
         return p;
     }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         const C<R> ks = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
         const C<R> k1 = cmul<R>(dt, p.f0);
-        kout[i] = ks;
+        st_stream<R>(kout + i, ks);
         C<R> u;
         if constexpr (S_ == 2) {
             u = caxpy<R>(t.b32, ks, caxpy<R>(t.b31, k1, p.p0));
@@ -304,7 +321,7 @@ template <typename R, int S_> struct OpCkMid {
             u = caxpy<R>(t.b65, ks,
                          caxpy<R>(t.b64, p.k4, caxpy<R>(t.b63, p.k3, caxpy<R>(t.b62, p.k2, caxpy<R>(t.b61, k1, p.p0)))));
         }
-        unext[i] = u;
+        st_stream<R>(unext + i, u);
     }
 };
 
@@ -326,12 +343,15 @@ template <typename R> struct OpCkLast {
     C<R> dt;
     double dt0_re, dt0_im;
     CkTab<R> t;
-    __device__ Pre load(long long i) const { return {p0[i], f0[i], k3[i], k4[i], k5[i]}; }
+    __device__ Pre load(long long i) const {
+        return {ld_stream<R>(p0 + i), ld_stream<R>(f0 + i), ld_stream<R>(k3 + i), ld_stream<R>(k4 + i),
+                ld_stream<R>(k5 + i)};
+    }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double* red) const {
         const C<R> k6 = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
         const C<R> k1 = cmul<R>(dt, p.f0);
         const C<R> nw = caxpy<R>(t.c6, k6, caxpy<R>(t.c4, p.k4, caxpy<R>(t.c3, p.k3, caxpy<R>(t.c1, k1, p.p0))));
-        out[i] = nw;
+        st_stream<R>(out + i, nw);
         const double er = t.dc1 * (double)k1.x + t.dc3 * (double)p.k3.x + t.dc4 * (double)p.k4.x +
                           t.dc5 * (double)p.k5.x + t.dc6 * (double)k6.x;
         const double ei = t.dc1 * (double)k1.y + t.dc3 * (double)p.k3.y + t.dc4 * (double)p.k4.y +
@@ -365,7 +385,7 @@ template <typename R> struct OpDeriv {
             case 5: v = st.d2z(); break;
             default: v = st.lap(); break;
         }
-        out[i] = v;
+        st_stream<R>(out + i, v);
     }
 };
 
@@ -378,7 +398,7 @@ template <typename R> struct OpRhs {
     C<R>* __restrict__ out;
     __device__ Pre load(long long) const { return {}; }
     template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
-        out[i] = rhs_eval<R>(k, st, x, y, z);
+        st_stream<R>(out + i, rhs_eval<R>(k, st, x, y, z));
     }
 };
 

@@ -71,6 +71,7 @@ template <typename R> hipError_t launch_from_c64(const float2* src, C<R>* dst, l
 template <typename R> hipError_t launch_to_c128(const C<R>* src, double2* dst, long long n, hipStream_t s);
 template <typename R> hipError_t launch_to_c64(const C<R>* src, float2* dst, long long n, hipStream_t s);
 hipError_t launch_copy16(void* dst, const void* src, long long n16, hipStream_t s);
+hipError_t launch_stream5(void* c, void* d, const void* a, const void* b, const void* e, long long n16, hipStream_t s);
 hipError_t launch_reduce_final(int mode, const double* partial, int n, int nval, double* out, hipStream_t s);
 
 constexpr int kNormBlocks = 2048;

@@ -116,6 +116,24 @@ __global__ __launch_bounds__(kPwThreads) void k_copy16(const uint4* __restrict__
         dst[i] = src[i];
 }
 
+// streaming variant: 3 reads + 2 writes per element (the pointwise traffic
+// pattern of an RK4 middle stage without the stencil)
+__global__ __launch_bounds__(kPwThreads) void k_stream5(const uint4* __restrict__ a, const uint4* __restrict__ b,
+                                                        uint4* __restrict__ c, uint4* __restrict__ d,
+                                                        const uint4* __restrict__ e, long long n) {
+    for (long long i = blockIdx.x * (long long)kPwThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kPwThreads) {
+        const uint4 x = a[i], y = b[i], z = e[i];
+        c[i] = make_uint4(x.x ^ y.x, x.y ^ y.y, x.z ^ z.z, x.w ^ z.w);
+        d[i] = make_uint4(y.x ^ z.x, y.y ^ z.y, x.z, y.w);
+    }
+}
+
+hipError_t launch_stream5(void* c, void* d, const void* a, const void* b, const void* e, long long n16, hipStream_t s) {
+    hipLaunchKernelGGL(k_stream5, dim3(pw_blocks(n16)), dim3(kPwThreads), 0, s, (const uint4*)a, (const uint4*)b,
+                       (uint4*)c, (uint4*)d, (const uint4*)e, n16);
+    return hipGetLastError();
+}
+
 hipError_t launch_copy16(void* dst, const void* src, long long n16, hipStream_t s) {
     hipLaunchKernelGGL(k_copy16, dim3(pw_blocks(n16)), dim3(kPwThreads), 0, s, (const uint4*)src, (uint4*)dst, n16);
     return hipGetLastError();

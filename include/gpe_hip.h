@@ -175,6 +175,10 @@ const char* gpe_kernel_kind_name(int kind);
 /* Plain device copy of nbytes (a multiple of 16) on a HIP stream: the
  * calibration kernel for HBM byte counters.  No reference counterpart. */
 int gpe_bench_copy(void* dst, const void* src, int64_t nbytes, void* stream);
+/* Streaming kernel with three reads (a, b, e) and two writes (c, d) of
+ * nbytes each: the achievable-bandwidth reference for the pointwise traffic
+ * of a stage kernel.  No reference counterpart. */
+int gpe_bench_stream5(void* c, void* d, const void* a, const void* b, const void* e, int64_t nbytes, void* stream);
 
 void* gpe_state_ptr(gpe_ctx* ctx);
 void* gpe_stream(gpe_ctx* ctx);
