@@ -91,7 +91,7 @@ def kernel_profile(ctx, scheme, steps, info):
     assert lib.gpe_prof_enable(ctx.h, 1) == 0
     ctx.step(scheme, steps, info)
     out = {}
-    for k in range(14):
+    for k in range(15):
         ms, n, b = ct.c_double(), ct.c_int64(), ct.c_double()
         assert lib.gpe_prof_read(ctx.h, k, ct.byref(ms), ct.byref(n), ct.byref(b)) == 0
         if n.value:

@@ -63,6 +63,7 @@ PROTOTYPES = [
     ("gpe_norm", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_energy", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_mass", _I, [_VP, ct.POINTER(_D)]),
+    ("gpe_hamiltonian", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_renormalise", _I, [_VP, _D]),
     ("gpe_prof_enable", _I, [_VP, _I]),
     ("gpe_prof_read", _I, [_VP, _I, ct.POINTER(_D), ct.POINTER(_I64), ct.POINTER(_D)]),

@@ -161,6 +161,11 @@ class Context:
         self._chk(self.lib.gpe_energy(self.h, ct.byref(v)), "gpe_energy")
         return v.value
 
+    def hamiltonian(self) -> float:
+        v = ct.c_double()
+        self._chk(self.lib.gpe_hamiltonian(self.h, ct.byref(v)), "gpe_hamiltonian")
+        return v.value
+
     def mass(self) -> float:
         v = ct.c_double()
         self._chk(self.lib.gpe_mass(self.h, ct.byref(v)), "gpe_mass")

@@ -69,7 +69,7 @@ struct gpe_ctx {
 
 static const char* const kKindNames[GPE_NKINDS] = {
     "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2_3", "rk4_stage4", "rkck_stage1",
-    "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy"};
+    "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy", "hamiltonian"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -773,6 +773,28 @@ int gpe_energy(gpe_ctx* c, double* energy) {
     double d[3];
     spacing(c->p, d);
     *energy = s * d[0] * d[1] * d[2];  // E = E*dx*dy*dz (src/variables.f90:584)
+    return 0;
+}
+
+int gpe_hamiltonian(gpe_ctx* c, double* h) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_hamiltonian: ctx is NULL");
+    if (!h) return fail(c, GPE_EINVAL, "gpe_hamiltonian: output is NULL");
+    int rc;
+    if (c->p.storage == GPE_C128) {
+        RhsC<double> k = make_rhs<double>(c);
+        OpHamiltonian<double> o{k, k.pim >= 0 ? 1.0 : -1.0};
+        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+    } else {
+        RhsC<float> k = make_rhs<float>(c);
+        OpHamiltonian<float> o{k, k.pim >= 0 ? 1.0 : -1.0};
+        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+    }
+    if (rc) return rc;
+    double s;
+    if ((rc = read_result(c, &s))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    *h = s * d[0] * d[1] * d[2];
     return 0;
 }
 

@@ -418,4 +418,33 @@ template <typename R> struct OpEnergy {
     }
 };
 
+// Conserved energy of the equation being solved (no reference counterpart:
+// the reference's energy() omits V and mu).  For f = i*pim*(a lap + (b + c|psi|^2
+// + dV) psi) the density is -sgn(pim) (a Re(conj(psi) lap psi) + b|psi|^2
+// + c|psi|^4/2 + dV|psi|^2): eqn 1 gives |grad psi|^2 - |psi|^2 + |psi|^4/2,
+// eqn 4 gives |grad psi|^2/2 + V|psi|^2 + g|psi|^4/2 - mu|psi|^2 (after
+// summation by parts of the discrete Laplacian).  Summed in FP64.
+template <typename R> struct OpHamiltonian {
+    static constexpr int kKind = 14, kAccess = 1;
+    static constexpr int kNRed = 1, kMode = RED_SUM;
+    using Pre = NoPre;
+    RhsC<R> k;
+    double sgn;
+    __device__ Pre load(long long) const { return {}; }
+    template <class S> __device__ void apply(long long, int x, int y, int z, const S& st, const Pre&, double* red) const {
+        const C<R> L = st.lap();
+        const C<R> p = st.c();
+        const double pr = p.x, pi = p.y;
+        const double a2 = pr * pr + pi * pi;
+        double v = 0.0;
+        if (k.flags & RHS_V) {
+            const double xc = (double)k.xl + x * (double)k.dx, yc = (double)k.yl + y * (double)k.dy,
+                         zc = (double)k.zl + z * (double)k.dz;
+            v = (double)k.d * ((double)k.vx * xc * xc + (double)k.vy * yc * yc + (double)k.vz * zc * zc);
+        }
+        const double re_pl = (double)L.x * pr + (double)L.y * pi;
+        red[0] += -sgn * ((double)k.a * re_pl + ((double)k.b + v) * a2 + 0.5 * (double)k.c * a2 * a2);
+    }
+};
+
 }  // namespace gpe

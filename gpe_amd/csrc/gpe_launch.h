@@ -50,7 +50,8 @@ hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u
     X(R, OpCkLast<R>)                                                                                              \
     X(R, OpDeriv<R>)                                                                                               \
     X(R, OpRhs<R>)                                                                                                 \
-    X(R, OpEnergy<R>)
+    X(R, OpEnergy<R>)                                                                                              \
+    X(R, OpHamiltonian<R>)
 
 #define GPE_DECLARE_EXTERN_ORD(R, OP, ORD) \
     extern template hipError_t launch_stencil<R, ORD, OP>(const Tiles&, Geom, const StW<R>&, const C<R>*, const OP&, \

@@ -1,0 +1,132 @@
+"""Studies of the north-star additions, run on one GPU:
+
+  drift      complex64 storage vs complex128 on the trapped-vortex workload
+             (256^3 reflective, dx = 0.2, 4th order, eqn 4, RK4 dt = 0.002,
+             2000 steps): drift of the norm and of the conserved energy
+             (gpe_hamiltonian) for each, and the c64 field's distance from the
+             FP64 (c128) run.
+  time_to_t  RKF45 to a fixed final time on the random-phase workload:
+             an FP64 reference at a tight tolerance, plain c128 at 1e-6,
+             c64 at 1e-6 (and the learned controller when a model is given);
+             wall time, steps, rejections and relative L2 error at T.
+
+Each prints one JSON object per line.  Usage:
+    python -m gpe_amd.studies drift [--n 256] [--steps 2000]
+    python -m gpe_amd.studies time_to_t [--n 512] [--T 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import ics
+from .context import Context, StepInfo
+from .controller import PlainController, integrate
+
+
+def rel_l2(a, b) -> float:
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def emit(d):
+    print(json.dumps(d), flush=True)
+
+
+def drift(n: int, steps: int, every: int):
+    out = {}
+    finals = {}
+    for storage in ("c128", "c64"):
+        p, psi = ics.config_trap256(storage=storage, n=n)
+        with Context(p) as ctx:
+            ctx.upload(psi)
+            n0, e0 = ctx.norm(), ctx.hamiltonian()
+            info = StepInfo(p.dt0())
+            series = [(0, n0, e0)]
+            t0 = time.perf_counter()
+            done = 0
+            while done < steps:
+                k = min(every, steps - done)
+                ctx.step("rk4", k, info)
+                done += k
+                series.append((done, ctx.norm(), ctx.hamiltonian()))
+            ctx.sync()
+            wall = time.perf_counter() - t0
+            finals[storage] = ctx.download()
+        nf, ef = series[-1][1], series[-1][2]
+        out[storage] = dict(norm0=n0, energy0=e0, norm_drift=(nf - n0) / n0, energy_drift=(ef - e0) / abs(e0),
+                            max_norm_drift=max(abs(s[1] - n0) / n0 for s in series),
+                            max_energy_drift=max(abs(s[2] - e0) / abs(e0) for s in series), wall_s=wall,
+                            gcell_steps_s=p.cells * steps / wall / 1e9)
+        emit(dict(study="drift", storage=storage, n=n, steps=steps, **out[storage]))
+    s64, s128 = out["c64"], out["c128"]
+    emit(dict(study="drift_c64_vs_fp64", n=n, steps=steps,
+              field_rel_l2=rel_l2(finals["c64"].astype(np.complex128), finals["c128"]),
+              norm_rel_diff=abs(s64["norm0"] * (1 + s64["norm_drift"]) - s128["norm0"] * (1 + s128["norm_drift"]))
+              / (s128["norm0"] * (1 + s128["norm_drift"])),
+              energy_rel_diff=abs(s64["energy0"] * (1 + s64["energy_drift"]) - s128["energy0"] * (1 + s128["energy_drift"]))
+              / abs(s128["energy0"] * (1 + s128["energy_drift"]))))
+
+
+def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=None):
+    pp = p.replace(storage=storage, eps=eps)
+    ctrl = controller or PlainController()
+    with Context(pp) as ctx:
+        ctx.upload(psi)
+        st = integrate(ctx, T, dt0, ctrl, clamp=True, energy=energy)
+        final = ctx.download()
+    st.method = name or f"{ctrl.name}-{storage}-eps{eps:g}"
+    return st, final
+
+
+def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None):
+    p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=dt0)
+    ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
+    emit(dict(study="time_to_t", n=n, T=T, method=ref.method, wall_s=ref.wall_s, steps=ref.n_accept,
+              rejects=ref.n_reject, rhs=ref.n_rhs))
+    rows = []
+    runs = []
+    for eps in eps_list:
+        runs.append(("c128", eps, None, False))
+        runs.append(("c64", eps, None, False))
+        if learned_model is not None:
+            from .learned import LearnedController
+            runs.append(("c64", eps, LearnedController(learned_model), True))
+    for storage, eps, ctrl, energy in runs:
+        st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy)
+        err = rel_l2(fin.astype(np.complex128), ref_final)
+        row = dict(study="time_to_t", n=n, T=T, method=st.method, storage=storage, eps=eps, wall_s=st.wall_s,
+                   steps=st.n_accept, rejects=st.n_reject, rhs=st.n_rhs, rel_l2_vs_oracle=err)
+        rows.append(row)
+        emit(row)
+    return rows
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["drift", "time_to_t"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--T", type=float, default=20.0)
+    ap.add_argument("--dt0", type=float, default=0.01)
+    ap.add_argument("--oracle-eps", type=float, default=1e-10)
+    ap.add_argument("--eps", default="1e-6")
+    ap.add_argument("--model", default=None, help="learned controller weights (.npz)")
+    a = ap.parse_args(argv)
+    if a.study == "drift":
+        drift(a.n or 256, a.steps, a.every)
+    else:
+        model = None
+        if a.model:
+            from .learned import load_model
+            model = load_model(a.model)
+        time_to_t(a.n or 512, a.T, a.dt0, a.oracle_eps, [float(v) for v in a.eps.split(",")], model)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -157,6 +157,11 @@ int gpe_rhs(gpe_ctx* ctx, const void* in, int64_t ny_alloc, int64_t y_off, int64
 int gpe_norm(gpe_ctx* ctx, double* norm);
 int gpe_energy(gpe_ctx* ctx, double* energy);
 int gpe_mass(gpe_ctx* ctx, double* mass);
+/* Conserved energy of the equation solved (eqn 1: sum |grad psi|^2 - |psi|^2
+ * + |psi|^4/2; eqn 4: sum |grad psi|^2/2 + V|psi|^2 + g|psi|^4/2 - mu|psi|^2,
+ * times dx dy dz), for drift reports.  No reference counterpart (the
+ * reference's energy() omits V and mu). */
+int gpe_hamiltonian(gpe_ctx* ctx, double* h);
 /* psi <- sqrt(nn) psi / sqrt(norm) (src/variables.f90:698-709). */
 int gpe_renormalise(gpe_ctx* ctx, double norm);
 
@@ -167,7 +172,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 14
+#define GPE_NKINDS 15
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

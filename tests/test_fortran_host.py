@@ -1,0 +1,74 @@
+"""The Fortran host (gpe_amd/fortran): the reference-compatible modules
+solve/derivs/variables over the ISO_C_BINDING layer, and a driver with the
+reference's main loop.  GPU tests run the driver on the inputs of the
+reference-build fixtures and compare its end_state.dat with the reference's
+final state; the CPU test checks the error convention (emergency_stop writes
+ERROR and removes RUNNING) when no device is available."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from gpe_amd import RunParams, ics
+from gpe_amd.io import read_end_state, run_in_text, write_end_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "gpe_amd", "fortran", "gpe_hip_main")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run_host(tmp_path, p, psi, nsteps, mode="resident"):
+    text = run_in_text(p, (nsteps - 0.5) * p.tau).replace("host_mode = 'resident'", f"host_mode = '{mode}'")
+    (tmp_path / "run.in").write_text(text)
+    write_end_state(str(tmp_path / "end_state00.dat"), psi, 0, 0.0, 0.0, p.dt0())
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    return r
+
+
+def test_host_binary_built():
+    assert os.access(EXE, os.X_OK), "build with __graft_entry__.build() (make -C gpe_amd/fortran)"
+
+
+def test_host_error_convention_without_device(tmp_path):
+    """No GPU here: context creation fails and the host stops the way the
+    reference does (message, ERROR file, RUNNING removed, non-zero exit)."""
+    try:
+        import torch  # noqa: F401  (only to see whether a device exists)
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present; the failure path is not reachable")
+    p = RunParams(nx=16, ny=16, nz=16, xr=4.0, yr=4.0, zr=4.0)
+    r = run_host(tmp_path, p, np.ones(p.shape, dtype=np.complex128), 2)
+    assert r.returncode != 0
+    assert (tmp_path / "ERROR").exists()
+    assert "ERROR" in (tmp_path / "ERROR").read_text()
+    assert not (tmp_path / "RUNNING").exists()
+
+
+def load_fixture(name):
+    z = np.load(os.path.join(GOLD, f"ref_{name}.npz"), allow_pickle=False)
+    p = RunParams(**json.loads(str(z["params"])))
+    psi = z["inp"] if "inp" in z.files else ics.config_ring64(seed=0)[1]
+    return p, np.ascontiguousarray(psi), z
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,mode", [("ring64_rk4_200", "resident"), ("eq4_ref_o4_rk4", "parity"),
+                                       ("eq1_ref_o2_rk2_u_diss", "parity"), ("eq4_ref_o2_imag_rk4", "resident")])
+def test_host_matches_reference_build(tmp_path, name, mode):
+    p, psi, z = load_fixture(name)
+    nsteps = int(z["p"])
+    r = run_host(tmp_path, p, psi, nsteps, mode)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out, pp, t, im_t, dt = read_end_state(str(tmp_path / "end_state.dat"))
+    assert pp == nsteps
+    err = float(np.linalg.norm(out - z["out"]) / np.linalg.norm(z["out"]))
+    print(f"Fortran host ({mode}) vs reference build, {name}: rel L2 = {err:.3e}")
+    assert err <= 1e-12
+    assert abs(t - float(z["t"])) <= 1e-12 and abs(im_t - float(z["im_t"])) <= 1e-12
+    assert not (tmp_path / "RUNNING").exists()

@@ -227,3 +227,33 @@ def test_error_paths():
             ctx.set_tiles(48, 8, 0)
         with pytest.raises(ValueError):
             ctx.upload(np.zeros((3, 3, 3), dtype=np.complex128))
+
+
+def test_hamiltonian_closed_form_and_conservation():
+    """Conserved energy: plane-wave closed form for eqn 1, and RK4 keeps it to
+    O(dt^4) for eqn 4 on a periodic grid (where the discrete Laplacian is
+    self-adjoint under plain sums, so the discrete energy is an invariant)."""
+    p = RunParams(nx=24, ny=20, nz=18, xr=8.0, yr=7.0, zr=6.5, order=4, bcs=1, eqn_to_solve=1)
+    x, y, z = p.axes()
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    kx = 2 * np.pi * 2 / (p.nx * p.dx)
+    amp = 0.7 + 0.2j
+    f = np.ascontiguousarray(amp * np.exp(1j * kx * X))
+    th = kx * p.dx
+    lam = (-2 * np.cos(2 * th) + 32 * np.cos(th) - 30) / (12 * p.dx ** 2)
+    a2 = abs(amp) ** 2
+    exact = p.cells * p.dx * p.dy * p.dz * (-lam * a2 - a2 + 0.5 * a2 * a2)
+    with Context(p) as ctx:
+        ctx.upload(f)
+        assert abs(ctx.hamiltonian() - exact) <= 1e-12 * abs(exact)
+    q, psi = ics.config_trap256(n=32)
+    q = q.replace(xr=6.0, yr=6.0, zr=6.0, omx=0.3, omy=0.3, omz=0.3, mu=3.0, bcs=1)
+    psi = np.ascontiguousarray(ics.fermi(q) * ics.vortex_line_z(q))
+    drifts = []
+    for dt in (0.004, 0.002):
+        with Context(q) as ctx:
+            ctx.upload(psi)
+            h0 = ctx.hamiltonian()
+            ctx.step("rk4", int(round(0.2 / dt)), StepInfo(complex(dt)))
+            drifts.append(abs(ctx.hamiltonian() - h0) / abs(h0))
+    assert drifts[0] < 1e-6 and drifts[1] < drifts[0] / 8
