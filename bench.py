@@ -108,7 +108,7 @@ def load_traffic(name):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        return json.load(f)["traffic_bytes_per_launch"]
 
 
 def run_storage(p, psi, args, hip):
@@ -154,7 +154,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="grid points per axis (512 = the benchmark workload)")
     ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="RK4 steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (counter calibration)")

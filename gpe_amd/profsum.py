@@ -1,0 +1,122 @@
+"""Summarise rocprofv3 output of bench.py into the text tables kept under
+profiles/.
+
+    python -m gpe_amd.profsum KT_DIR FETCH_DIR WRITE_DIR --n 512 --out profiles/
+
+KT_DIR holds the --kernel-trace --stats run (kt_kernel_stats.csv), FETCH_DIR
+and WRITE_DIR the --pmc FETCH_SIZE and --pmc WRITE_SIZE runs (per-dispatch
+counters in KiB).  Both counter runs include one plain copy kernel of known
+size (bench.py --copy-probe-mib); each counter is scaled by
+(bytes the copy moved) / (what the counter read for it), which is how the
+per-dispatch HBM bytes ("traffic") of the stencil kernels are obtained.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+
+KINDS = {
+    "OpRk4First": "rk4_stage1", "OpRk4Mid": "rk4_stage2_3", "OpRk4Last": "rk4_stage4",
+    "OpAxpySelf": "euler_or_rk2_stage1", "OpAxpyP0": "rk2_stage2", "OpCk1": "rkck_stage1",
+    "OpCkLast": "rkck_stage6_error", "OpCkMid<double, 2>": "rkck_stage2", "OpCkMid<float, 2>": "rkck_stage2",
+}
+ACCESS = {"rk4_stage1": 3, "rk4_stage2_3": 5, "rk4_stage4": 3}
+
+
+def _one(pattern):
+    files = sorted(glob.glob(pattern, recursive=True))
+    if not files:
+        raise SystemExit(f"no file matches {pattern}")
+    return files[0]
+
+
+def kernel_label(name: str) -> str:
+    m = re.search(r"k_stencil<(\w+), (\d), (\d+), (\d+), gpe::(\w+)", name)
+    if m:
+        prec = "c128" if m.group(1) == "double" else "c64"
+        op = m.group(5)
+        kind = next((v for k, v in KINDS.items() if name.find(k) >= 0), op)
+        return f"{kind} {prec} order{m.group(2)} tile{m.group(3)}x{m.group(4)}"
+    m = re.search(r"gpe::(\w+)", name)
+    return m.group(1) if m else name[:60]
+
+
+def counters(path):
+    per = {}
+    for r in csv.DictReader(open(path)):
+        per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+    return {k: sum(v) / len(v) for k, v in per.items()}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("kt")
+    ap.add_argument("fetch")
+    ap.add_argument("write")
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--out", default="profiles")
+    ap.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+
+    stats = list(csv.DictReader(open(_one(os.path.join(a.kt, "**", "*kernel_stats.csv")))))
+    fetch = counters(_one(os.path.join(a.fetch, "**", "*counter_collection.csv")))
+    write = counters(_one(os.path.join(a.write, "**", "*counter_collection.csv")))
+    copy_bytes = a.copy_mib << 20
+    kcopy = next(k for k in fetch if "k_copy16" in k)
+    f_scale = copy_bytes / fetch[kcopy]
+    w_scale = copy_bytes / write[kcopy]
+    cells = a.n ** 3
+    os.makedirs(a.out, exist_ok=True)
+    lines = [f"# RK4 step kernels, {a.n}^3 random-phase workload (bench.py)", "",
+             "Commands (run from the repository root; TMPDIR=/tmp):", "",
+             "    rocprofv3 --kernel-trace --stats -d <KT_DIR> -o kt --output-format csv -- \\",
+             "        python3 bench.py --steps 20 --warmup 5 --cpu-steps 0",
+             "    rocprofv3 --pmc FETCH_SIZE -d <FETCH_DIR> -o fetch --output-format csv -- \\",
+             f"        python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}",
+             "    rocprofv3 --pmc WRITE_SIZE -d <WRITE_DIR> -o write --output-format csv -- \\",
+             f"        python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}",
+             f"    python3 -m gpe_amd.profsum <KT_DIR> <FETCH_DIR> <WRITE_DIR> --n {a.n}", "",
+             f"Counter calibration with a plain copy of {a.copy_mib} MiB (k_copy16): FETCH_SIZE read "
+             f"{fetch[kcopy] / 2**20:.1f} MiB -> scale {f_scale:.4f}; WRITE_SIZE read {write[kcopy] / 2**20:.1f} MiB "
+             f"-> scale {w_scale:.4f}.", "",
+             "| kernel | calls | avg ms | algorithmic GB/launch | algorithmic GB/s | % of 8 TB/s | HBM read GB (cal.) | "
+             "HBM write GB (cal.) | traffic / algorithmic |",
+             "|---|---|---|---|---|---|---|---|---|"]
+    traffic = {}
+    for r in stats:
+        name = r["Name"]
+        if "k_stencil" not in name:
+            continue
+        label = kernel_label(name)
+        kind = label.split()[0]
+        prec = label.split()[1]
+        esz = 16 if prec == "c128" else 8
+        alg = ACCESS.get(kind, 0) * esz * cells
+        avg_ms = float(r["AverageNs"]) / 1e6
+        fk = next((v for k, v in fetch.items() if k.startswith(name[:120])), None)
+        wk = next((v for k, v in write.items() if k.startswith(name[:120])), None)
+        rd = fk * f_scale if fk else None
+        wr = wk * w_scale if wk else None
+        t = (rd + wr) if (rd and wr) else None
+        if t and kind in ("rk4_stage2_3",):
+            traffic[prec] = t
+            json.dump({"traffic_bytes_per_launch": t, "read_bytes": rd, "write_bytes": wr,
+                       "algorithmic_bytes_per_launch": alg, "fetch_scale": f_scale, "write_scale": w_scale,
+                       "kernel": label},
+                      open(os.path.join(a.out, f"traffic_rk4_stage2_3_{prec}_{a.n}.json"), "w"), indent=1)
+        lines.append(f"| {label} | {r['Calls']} | {avg_ms:.4f} | {alg / 1e9:.3f} | "
+                     f"{alg / (avg_ms * 1e-3) / 1e9:.0f} | {100 * alg / (avg_ms * 1e-3) / 8e12:.1f} | "
+                     f"{(rd or 0) / 1e9:.3f} | {(wr or 0) / 1e9:.3f} | {(t / alg) if (t and alg) else float('nan'):.3f} |")
+    out = os.path.join(a.out, f"rk4_stencil_{a.n}.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print(open(out).read())
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
