@@ -8,9 +8,13 @@ module gpe_context
   use error, only : emergency_stop
   implicit none
   private
-  public :: the_ctx, check, host_kind, halo_args, close_ctx
+  public :: the_ctx, check, host_kind, halo_args, close_ctx, gpe_storage, gpe_vscale
 
   type(c_ptr), save :: ctx = c_null_ptr
+  ! device options with no counterpart in the reference's module parameters;
+  ! set them before the first call that creates the context
+  character(4), save :: gpe_storage = 'c128'   ! 'c128' or 'c64' device fields
+  real(c_double), save :: gpe_vscale = 0.0_c_double ! V term in eqn 1 (0 = reference)
 
 contains
 
@@ -21,10 +25,10 @@ contains
       gp%nx = nx; gp%ny = ny; gp%nz = nz
       gp%order = order; gp%bcs = bcs; gp%eqn_to_solve = eqn_to_solve
       gp%real_time = merge(1, 0, real_time)
-      gp%storage = merge(GPE_C64, GPE_C128, storage == 'c64')
+      gp%storage = merge(GPE_C64, GPE_C128, gpe_storage == 'c64')
       gp%xr = xr; gp%yr = yr; gp%zr = zr
       gp%Urhs = Urhs; gp%diss_amp = diss_amp; gp%g = g; gp%mu = mu; gp%nn = nn
-      gp%omx = omx; gp%omy = omy; gp%omz = omz; gp%vscale = vscale
+      gp%omx = omx; gp%omy = omy; gp%omz = omz; gp%vscale = gpe_vscale
       gp%eps = eps; gp%safety = safety; gp%dt_decrease = dt_decrease; gp%dt_increase = dt_increase
       if (gpe_ctx_create(gp, ctx) /= GPE_OK) &
         call emergency_stop('ERROR: '//c_string(gpe_last_error(c_null_ptr)))

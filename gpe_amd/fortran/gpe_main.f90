@@ -13,7 +13,7 @@ program gpe_hip_main
   use iso_c_binding, only : c_double
   use parameters
   use error
-  use gpe_context, only : close_ctx
+  use gpe_context, only : close_ctx, gpe_storage, gpe_vscale
   use solve
   use variables
   implicit none
@@ -40,6 +40,8 @@ program gpe_hip_main
   close (u)
   call set_extents()
   resident = host_mode /= 'parity'
+  gpe_storage = storage
+  gpe_vscale = vscale
 
   errcon = (5.0_pr/safety)**(1.0_pr/dt_increase)
   if (real_time) then

@@ -57,6 +57,28 @@ PIN
   $FC -O2 -flto -o "$dir/ref_driver" "$dir"/{parameters,constants,error,derivs,ic,variables,solve,ref_driver}.o \
       -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -flto -fuse-ld=lld
   echo "built $dir/ref_driver"
+
+  # Drop-in check: the same harness and the reference's own parameters,
+  # constants, error, ic and variables modules, with solve and derivs
+  # replaced by the GPU modules of gpe_amd/fortran (linked to libgpehip.so).
+  local gdir=$dir/dropin
+  local F=$HERE/../gpe_amd/fortran
+  local LIBDIR=$(cd "$HERE/../gpe_amd" && pwd)
+  mkdir -p "$gdir"
+  local gflags="-O2 -flto -I$dir -I$MPI/include -module-dir $gdir -I$gdir"
+  for m in parameters constants error; do
+    $FC $gflags -c "$REF/examples/ring/$m.f90" -o "$gdir/$m.o"
+  done
+  $FC $gflags -c "$F/gpe_hip_c.f90" -o "$gdir/gpe_hip_c.o"
+  $FC $gflags -c "$F/gpe_context.f90" -o "$gdir/gpe_context.o"
+  $FC $gflags -c "$F/derivs.f90" -o "$gdir/derivs.o"
+  $FC $gflags -c "$REF/examples/ring/ic.f90" -o "$gdir/ic.o"
+  $FC $gflags -c "$REF/examples/ring/variables.f90" -o "$gdir/variables.o"
+  $FC $gflags -c "$F/solve.f90" -o "$gdir/solve.o"
+  $FC $gflags -c "$HERE/ref/ref_driver.f90" -o "$gdir/ref_driver.o"
+  $FC -O2 -flto -o "$gdir/ref_driver_gpu" "$gdir"/{parameters,constants,error,gpe_hip_c,gpe_context,derivs,ic,variables,solve,ref_driver}.o \
+      -L$LIBDIR -lgpehip -Wl,-rpath,$LIBDIR -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -fuse-ld=lld
+  echo "built $gdir/ref_driver_gpu"
 }
 
 build_grid 64 64 64

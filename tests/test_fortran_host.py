@@ -72,3 +72,35 @@ def test_host_matches_reference_build(tmp_path, name, mode):
     assert err <= 1e-12
     assert abs(t - float(z["t"])) <= 1e-12 and abs(im_t - float(z["im_t"])) <= 1e-12
     assert not (tmp_path / "RUNNING").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["ring64_rk4_200", "eq4_ref_o4_rk4", "eq1_per_o4_rk45", "eq2_per_o4_euler"])
+def test_reference_program_with_gpu_dropin(tmp_path, name):
+    """oracle/_ref/<grid>/dropin/ref_driver_gpu is built from the reference's
+    own parameters/error/ic/variables modules and its main-loop sequence
+    (copy, MPI ghost exchange, solver) with this package's solve and derivs
+    modules in place of the reference's: the drop-in inside the reference's
+    program.  It must reproduce the reference build's results."""
+    p, psi, z = load_fixture(name)
+    exe = os.path.join(ROOT, "oracle", "_ref", f"{p.nx}x{p.ny}x{p.nz}", "dropin", "ref_driver_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("built by oracle/build_ref.sh where the reference sources are present")
+    nsteps = int(z["p"])
+    if p.scheme == "rk45":
+        # the loop ran while t <= end_time: end_time lies in [t_{n-1}, t_n)
+        end_time = float(z["trace"][-2, 0]) + 1e-12
+    else:
+        end_time = (nsteps - 0.5) * p.tau
+    (tmp_path / "run.in").write_text(run_in_text(p, end_time))
+    write_end_state(str(tmp_path / "in.dat"), psi, 0, 0.0, 0.0, p.dt0())
+    r = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {exe} in.dat out.dat trace.dat"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out, pp, t, im_t, dt, extra = read_end_state(str(tmp_path / "out.dat"), 2)
+    assert pp == nsteps
+    err = float(np.linalg.norm(out - z["out"]) / np.linalg.norm(z["out"]))
+    print(f"reference program + GPU drop-in, {name}: rel L2 = {err:.3e}")
+    assert err <= 1e-12
+    assert abs(t - float(z["t"])) <= 1e-12
+    assert abs(extra[0] - float(z["norm"])) <= 1e-12 * float(z["norm"])
