@@ -70,6 +70,16 @@ PROTOTYPES = [
     ("gpe_kernel_kind_name", ct.c_char_p, [_I]),
     ("gpe_bench_copy", _I, [_VP, _VP, _I64, _VP]),
     ("gpe_bench_stream5", _I, [_VP, _VP, _VP, _VP, _VP, _I64, _VP]),
+    ("gpe_ens_create", _I, [ct.POINTER(gpe_params), _I, ct.POINTER(_VP)]),
+    ("gpe_ens_destroy", None, [_VP]),
+    ("gpe_ens_last_error", ct.c_char_p, [_VP]),
+    ("gpe_ens_set_tiles", _I, [_VP, _I, _I, ct.POINTER(_I)]),
+    ("gpe_ens_upload", _I, [_VP, _I, _VP, _I]),
+    ("gpe_ens_download", _I, [_VP, _I, _VP, _I]),
+    ("gpe_ens_step", _I, [_VP, _I, _D, _D, _I]),
+    ("gpe_ens_work_items", _I, [_VP]),
+    ("gpe_ens_stream", _VP, [_VP]),
+    ("gpe_ens_sync", _I, [_VP]),
     ("gpe_state_ptr", _VP, [_VP]),
     ("gpe_stream", _VP, [_VP]),
     ("gpe_sync", _I, [_VP]),

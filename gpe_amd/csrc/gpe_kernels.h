@@ -49,9 +49,11 @@ template <int NT, int MODE> __device__ __forceinline__ double block_reduce(doubl
 #define GPE_OCC
 #endif
 
+// Body of the stencil kernel for the tile `bid` of grid g; the block's
+// reductions (if any) go to partial[r * g.nblocks + pidx].
 template <typename R, int ORD, int TX, int TY, class Op>
-__global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
-                                                     double* __restrict__ partial) {
+__device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, const C<R>* __restrict__ u, const Op& op,
+                                             double* __restrict__ partial, int bid, int pidx) {
     constexpr int H = ORD / 2, NT = TX * TY, LW = TX + 2 * H, LH = TY + 2 * H;
     constexpr int NHALO = 2 * H * TX + 2 * H * TY;
     constexpr int NSLOT = (NHALO + NT - 1) / NT;
@@ -59,7 +61,6 @@ __global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, co
     __shared__ double red_sh[NT / 64];
 
     const int tid = threadIdx.x;
-    const int bid = xcd_remap(blockIdx.x, g.nblocks);
     const int ntxy = g.ntx * g.nty;
     const int txy = bid % ntxy, zc = bid / ntxy;
     const int x0 = (txy % g.ntx) * TX, y0 = (txy / g.ntx) * TY;
@@ -151,9 +152,15 @@ __global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, co
 #pragma unroll
         for (int r = 0; r < Op::kNRed; ++r) {
             const double v = block_reduce<NT, Op::kMode>(red[r], red_sh);
-            if (tid == 0) partial[(long long)r * g.nblocks + blockIdx.x] = v;
+            if (tid == 0) partial[(long long)r * g.nblocks + pidx] = v;
         }
     }
+}
+
+template <typename R, int ORD, int TX, int TY, class Op>
+__global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
+                                                             double* __restrict__ partial) {
+    stencil_body<R, ORD, TX, TY, Op>(g, w, u, op, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
 }
 
 // Final reduction of per-block partials (one block, fixed order).

@@ -185,6 +185,27 @@ int gpe_bench_copy(void* dst, const void* src, int64_t nbytes, void* stream);
  * of a stage kernel.  No reference counterpart. */
 int gpe_bench_stream5(void* c, void* d, const void* a, const void* b, const void* e, int64_t nbytes, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched ensemble (no reference counterpart: the reference runs one field
+ * per process group).  Members share order, bcs, eqn_to_solve and storage;
+ * grid sizes and spacings differ.  RK4 only. */
+#define GPE_ENS_PER_MEMBER 0  /* one launch per member and stage */
+#define GPE_ENS_BATCHED 1     /* one persistent launch per stage for all members */
+#define GPE_ENS_GRAPH 2       /* GPE_ENS_BATCHED captured in a hipGraph and replayed */
+typedef struct gpe_ens gpe_ens;
+int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out);
+void gpe_ens_destroy(gpe_ens* ens);
+const char* gpe_ens_last_error(const gpe_ens* ens);
+/* tile shape of the batch (tx, ty) in {32, 64} x {4, 8}; tz per member (NULL: default) */
+int gpe_ens_set_tiles(gpe_ens* ens, int tx, int ty, const int* tz);
+/* member field in the storage precision, nx*ny*nz values, x fastest */
+int gpe_ens_upload(gpe_ens* ens, int member, const void* host, int host_dtype);
+int gpe_ens_download(gpe_ens* ens, int member, void* host, int host_dtype);
+int gpe_ens_step(gpe_ens* ens, int mode, double dt_re, double dt_im, int nsteps);
+int gpe_ens_work_items(const gpe_ens* ens);
+void* gpe_ens_stream(gpe_ens* ens);
+int gpe_ens_sync(gpe_ens* ens);
+
 void* gpe_state_ptr(gpe_ctx* ctx);
 void* gpe_stream(gpe_ctx* ctx);
 int gpe_sync(gpe_ctx* ctx);

@@ -1,0 +1,108 @@
+"""Batched ensemble (gpe_ens_*): every member must equal the single-context
+run of the same member in every mode (one launch per member, batched
+persistent launches, hipGraph replay), including graph replays interleaved
+with eager work."""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+from gpe_amd import Context, RunParams, StepInfo, _lib
+from gpe_amd.ensemble import Ensemble, ensemble_config
+
+
+def members(order, bcs, storage="c128"):
+    shapes = [(16, 20, 24), (24, 24, 24), (32, 16, 20), (20, 28, 12), (40, 8, 10)]
+    out = []
+    for i, (nx, ny, nz) in enumerate(shapes):
+        out.append(RunParams(nx=nx, ny=ny, nz=nz, xr=0.3 * (nx - 1) / 2, yr=0.3 * (ny - 1) / 2,
+                             zr=0.3 * (nz - 1) / 2, order=order, bcs=bcs, eqn_to_solve=1, storage=storage))
+    return out
+
+
+def fields(ms, seed=0):
+    rng = np.random.default_rng(seed)
+    return [np.ascontiguousarray(1.0 + 0.2 * (rng.standard_normal(m.shape) + 1j * rng.standard_normal(m.shape)))
+            for m in ms]
+
+
+def single(m, psi, nsteps, dt):
+    with Context(m) as ctx:
+        ctx.upload(psi)
+        ctx.step("rk4", nsteps, StepInfo(dt))
+        return ctx.download()
+
+
+def test_config_sizes_and_cells():
+    ms, fs = ensemble_config(n_members=6)
+    assert all(m.nx in (64, 96, 128, 160, 192) and m.nx == m.ny == m.nz for m in ms)
+    assert all(abs(m.dx - 0.3) < 1e-15 for m in ms)
+    assert all(abs(np.mean(np.abs(f) ** 2) - 1.0) < 1e-12 for f in fs)
+
+
+def test_create_rejects_mixed_members():
+    lib = _lib.load()
+    a = RunParams(nx=16, ny=16, nz=16, order=2).to_c()
+    b = RunParams(nx=16, ny=16, nz=16, order=4).to_c()
+    arr = (_lib.gpe_params * 2)(a, b)
+    h = ct.c_void_p()
+    assert lib.gpe_ens_create(arr, 2, ct.byref(h)) == -1
+    assert b"share" in lib.gpe_ens_last_error(None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
+@pytest.mark.parametrize("mode", ["per_member", "batched", "graph"])
+def test_modes_match_single_context(order, bcs, mode):
+    ms = members(order, bcs)
+    fs = fields(ms)
+    dt = 0.005 + 0j
+    exp = [single(m, f, 7, dt) for m, f in zip(ms, fs)]
+    with Ensemble(ms) as ens:
+        for i, f in enumerate(fs):
+            ens.upload(i, f)
+        ens.step(mode, dt, 7)
+        for i in range(len(ms)):
+            got = ens.download(i)
+            assert np.array_equal(got, exp[i]) or np.linalg.norm(got - exp[i]) <= 1e-14 * np.linalg.norm(exp[i])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tiles", [(32, 8), (64, 4), (64, 8), (32, 4)])
+def test_graph_replays_with_eager_work_between(tiles):
+    """Capture once, replay several times with eager batched steps and host
+    syncs in between; every replay must match the eager reference."""
+    ms = members(2, 1)
+    fs = fields(ms, seed=3)
+    dt = 0.004 + 0j
+    with Ensemble(ms) as eager, Ensemble(ms) as graphed:
+        for e in (eager, graphed):
+            e.set_tiles(*tiles, tz=[3, 5, 4, 2, 6])
+            for i, f in enumerate(fs):
+                e.upload(i, f)
+        for _ in range(3):
+            eager.step("batched", dt, 6)
+            graphed.step("graph", dt, 6)       # one chunk of 6 steps (captured on the first call)
+            graphed.sync()
+            eager.step("batched", dt, 1)       # eager work between replays
+            graphed.step("batched", dt, 1)
+            for i in range(len(ms)):
+                assert np.array_equal(graphed.download(i), eager.download(i))
+
+
+@pytest.mark.gpu
+def test_c64_ensemble_matches_single_context():
+    ms = members(2, 1, storage="c64")
+    fs = [f.astype(np.complex64) for f in fields(ms, seed=5)]
+    dt = 0.005 + 0j
+    with Ensemble(ms) as ens:
+        for i, f in enumerate(fs):
+            ens.upload(i, f)
+        ens.step("graph", dt, 5)
+        for i, (m, f) in enumerate(zip(ms, fs)):
+            with Context(m) as ctx:
+                ctx.upload(f)
+                ctx.step("rk4", 5, StepInfo(dt))
+                exp = ctx.download(np.complex64)
+            got = ens.download(i)
+            assert np.linalg.norm(got - exp) <= 1e-6 * np.linalg.norm(exp)
