@@ -80,6 +80,15 @@ PROTOTYPES = [
     ("gpe_ens_work_items", _I, [_VP]),
     ("gpe_ens_stream", _VP, [_VP]),
     ("gpe_ens_sync", _I, [_VP]),
+    ("gpe_mlp_create", _I, [_I, ct.POINTER(_VP)]),
+    ("gpe_mlp_destroy", None, [_VP]),
+    ("gpe_mlp_last_error", ct.c_char_p, [_VP]),
+    ("gpe_mlp_num_params", _I, [_VP]),
+    ("gpe_mlp_set_params", _I, [_VP, _VP, _I64]),
+    ("gpe_mlp_get_params", _I, [_VP, _VP, _I64]),
+    ("gpe_mlp_forward", _I, [_VP, _VP, _I, _VP]),
+    ("gpe_mlp_grad", _I, [_VP, _VP, _VP, _I, _VP, _VP]),
+    ("gpe_mlp_train", _I, [_VP, _VP, _VP, _I, _I, _I, ct.c_float, _VP]),
     ("gpe_state_ptr", _VP, [_VP]),
     ("gpe_stream", _VP, [_VP]),
     ("gpe_sync", _I, [_VP]),

@@ -59,7 +59,9 @@ class RunStats:
     n_reject: int = 0
     n_rhs: int = 0
     wall_s: float = 0.0
-    trace: list = field(default_factory=list)   # rows (t, dt, errmax, n_rej_this_step, psi2max, energy_drift)
+    # rows (t, dt, errmax, n_rej_this_step, psi2max, energy_drift, dt_first, errmax_first):
+    # the accepted step and the first attempt of that step
+    trace: list = field(default_factory=list)
 
 
 def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy: bool = False,
@@ -85,8 +87,11 @@ def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy:
             dt = complex(T - t, 0.0)
         dt_entry = dt
         rej = 0
+        first = None
         while True:
             errmax, a2 = ctx.rk45_attempt(dt, dt_entry)
+            if first is None:
+                first = (dt.real, errmax)
             st.n_rhs += 5 if rej else 6  # f(psi) is reused on retries
             if errmax <= 1.0:
                 break
@@ -104,7 +109,7 @@ def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy:
         if energy:
             E = ctx.energy()
             hist.energy_drift = (E - E0) / abs(E0) if E0 else 0.0
-        st.trace.append((t, dt.real, errmax, rej, a2, hist.energy_drift))
+        st.trace.append((t, dt.real, errmax, rej, a2, hist.energy_drift, first[0], first[1]))
         dt = controller.propose(dt, errmax, hist)
     ctx.sync()
     st.wall_s = time.perf_counter() - t_start
@@ -113,4 +118,4 @@ def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy:
 
 
 def trace_array(stats: RunStats) -> np.ndarray:
-    return np.array(stats.trace, dtype=np.float64).reshape(-1, 6)
+    return np.array(stats.trace, dtype=np.float64).reshape(-1, 8)

@@ -206,6 +206,26 @@ int gpe_ens_work_items(const gpe_ens* ens);
 void* gpe_ens_stream(gpe_ens* ens);
 int gpe_ens_sync(gpe_ens* ens);
 
+/* ------------------------------------------------------------------------
+ * FP32 MLP n_in -> 64 -> 64 -> 1 (tanh) of the learned RKF45 step-size
+ * proposer: inference, gradients and Adam training on the GPU.  No
+ * reference counterpart.  Parameters are packed W1[64][n_in], b1[64],
+ * W2[64][64], b2[64], W3[64], b3; the loss is the mean squared error. */
+typedef struct gpe_mlp gpe_mlp;
+int gpe_mlp_create(int n_in, gpe_mlp** out);
+void gpe_mlp_destroy(gpe_mlp* mlp);
+const char* gpe_mlp_last_error(const gpe_mlp* mlp);
+int gpe_mlp_num_params(const gpe_mlp* mlp);
+int gpe_mlp_set_params(gpe_mlp* mlp, const float* params, int64_t n);
+int gpe_mlp_get_params(gpe_mlp* mlp, float* params, int64_t n);
+/* y[s] = mlp(x[s, :]) for s < n (x row-major n x n_in) */
+int gpe_mlp_forward(gpe_mlp* mlp, const float* x, int n, float* y);
+/* gradient of mean((mlp(x) - y)^2) over the n samples, and the loss */
+int gpe_mlp_grad(gpe_mlp* mlp, const float* x, const float* y, int n, float* grad, float* loss);
+/* `steps` Adam steps over cyclic mini-batches of `batch` samples (the data
+ * set is taken in the given order); losses[s] receives each step's loss */
+int gpe_mlp_train(gpe_mlp* mlp, const float* x, const float* y, int n, int steps, int batch, float lr, float* losses);
+
 void* gpe_state_ptr(gpe_ctx* ctx);
 void* gpe_stream(gpe_ctx* ctx);
 int gpe_sync(gpe_ctx* ctx);

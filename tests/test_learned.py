@@ -1,0 +1,117 @@
+"""Learned step-size proposer: the FP32 MLP kernels against the FP64 numpy
+oracle (forward, analytic gradient, finite differences), deterministic GPU
+training, and the proposer's gain over the plain controller on an RKF45 run
+(acceptance still by the Cash-Karp estimate)."""
+import math
+
+import numpy as np
+import pytest
+
+from gpe_amd import learned
+from gpe_amd.params import RunParams
+from oracle import mlp_oracle as M
+
+NIN = learned.N_IN
+
+
+def test_oracle_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    p = learned.init_params(NIN, 1).astype(np.float64) * 3
+    X = rng.standard_normal((40, NIN))
+    Y = rng.standard_normal(40) * 0.3
+    g = M.grad(p, X, Y, NIN)
+    idx = rng.choice(len(p), 60, replace=False)
+    fd = M.fd_grad(p, X, Y, NIN, idx)
+    assert np.max(np.abs(g[idx] - fd)) <= 1e-7 * max(1.0, np.max(np.abs(fd)))
+
+
+def test_features_and_dataset():
+    p = RunParams(nx=64, ny=64, nz=64, xr=9.45, yr=9.45, zr=9.45, order=4)
+    dtr = learned.dt_ref(p)
+    assert abs(dtr - 1.0 / (0.5 * 3 * (64 / 12) / 0.3 ** 2)) < 1e-15
+    f = learned.raw_features([0.5], [0.02], 1.3, -1e-7, dtr)
+    assert f.shape == (NIN,) and abs(f[0] - math.log(0.5)) < 1e-15 and abs(f[3] - math.log(0.5)) < 1e-15
+    assert abs(f[4] - math.log(0.02 / dtr)) < 1e-15 and f[9] < 0
+    # synthetic trace: errmax scales as dt^5 with a constant, so the target
+    # recovers dt with errmax = rho on the next attempt
+    C = 0.6 / 0.02 ** 5
+    rows = []
+    t = 0.0
+    dt = 0.02
+    for _ in range(12):
+        e = C * dt ** 5
+        t += dt
+        rows.append((t, dt, e, 0, 1.0, 0.0, dt, e))
+        dt *= 1.05
+    X, Y = learned.dataset_from_trace(np.array(rows), dtr)
+    assert X.shape == (10, NIN)
+    dt_star = (learned.RHO / C) ** 0.2
+    for n in range(10):
+        assert abs(Y[n] - math.log(dt_star / rows[n][1])) < 1e-12
+
+
+
+
+@pytest.mark.gpu
+def test_forward_and_gradient_match_fp64_oracle():
+    rng = np.random.default_rng(1)
+    p = learned.init_params(NIN, 2) * 2
+    X = rng.standard_normal((300, NIN)).astype(np.float32)
+    Y = (0.3 * rng.standard_normal(300)).astype(np.float32)
+    mlp = learned.MLP(NIN)
+    mlp.set_params(p)
+    assert np.array_equal(mlp.get_params(), p)
+    y = mlp.forward(X)
+    y64 = M.forward(p, X.astype(np.float64), NIN)
+    assert np.max(np.abs(y - y64)) <= 2e-5 * max(1.0, np.max(np.abs(y64)))
+    g, loss = mlp.grad(X, Y)
+    g64 = M.grad(p, X.astype(np.float64), Y.astype(np.float64), NIN)
+    assert abs(loss - M.loss(p, X, Y, NIN)) <= 1e-5 * M.loss(p, X, Y, NIN)
+    assert np.linalg.norm(g - g64) <= 1e-4 * np.linalg.norm(g64)
+    idx = rng.choice(len(p), 50, replace=False)
+    fd = M.fd_grad(p, X.astype(np.float64), Y.astype(np.float64), NIN, idx)
+    assert np.max(np.abs(g[idx] - fd)) <= 1e-4 * max(1e-3, np.max(np.abs(fd)))
+
+
+@pytest.mark.gpu
+def test_training_is_deterministic_and_fits():
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((2000, NIN)).astype(np.float32)
+    Y = (0.4 * np.tanh(X[:, 0] - 0.5 * X[:, 4]) + 0.1 * X[:, 8]).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        mlp = learned.MLP(NIN)
+        mlp.set_params(learned.init_params(NIN, 0))
+        losses = mlp.train(X, Y, 1500, 200, 2e-3)
+        runs.append((mlp.get_params(), losses))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1][-1] < 0.05 * float(np.var(Y))
+
+
+@pytest.mark.gpu
+def test_learned_proposer_gain_over_plain():
+    """Traces from two small random-phase runs, GPU training, then the same
+    workload with a new seed: the learned proposer needs fewer RHS
+    evaluations than the plain controller, with the error at T within a
+    factor of 3 of the plain one's and rejections still decided by the
+    Cash-Karp estimate."""
+    from gpe_amd import Context, ics
+    from gpe_amd.controller import PlainController, integrate
+    from gpe_amd.studies import rel_l2
+
+    runs = [(64, 4, 0, 1e-6, 10.0, 0.35), (64, 4, 1, 1e-6, 10.0, 0.5), (64, 4, 2, 3e-6, 10.0, 0.35)]
+    data = learned.generate_traces(runs, log=lambda s: None)
+    X, Y = learned.build_dataset(data)
+    model = learned.train_model(X, Y, steps=2000, log=print)
+    p, psi = ics.config_turb(n=64, seed=9, scheme="rk45", tau=0.01)
+    res = {}
+    for name, ctrl, eps in [("oracle", PlainController(), 1e-10), ("plain", PlainController(), 1e-6),
+                            ("learned", learned.LearnedController(model, p), 1e-6)]:
+        with Context(p.replace(eps=eps)) as ctx:
+            ctx.upload(psi)
+            st = integrate(ctx, 10.0, 0.01, ctrl, clamp=True, energy=(name == "learned"))
+            res[name] = (st, ctx.download())
+    err = {k: rel_l2(res[k][1], res["oracle"][1]) for k in ("plain", "learned")}
+    print({k: (res[k][0].n_accept, res[k][0].n_reject, res[k][0].n_rhs) for k in res}, err)
+    assert res["learned"][0].n_rhs < res["plain"][0].n_rhs
+    assert err["learned"] <= 3 * err["plain"]
