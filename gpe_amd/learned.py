@@ -42,8 +42,9 @@ from .controller import PlainController, integrate, trace_array
 from .params import RunParams
 
 N_IN = 10
-RHO = 0.75
+RHO = 0.65
 CLIP = (math.log(0.1), math.log(5.0))
+PROPOSE_CLIP = (math.log(0.5), math.log(2.0))
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "models", "proposer.npz")
 
@@ -145,14 +146,30 @@ def raw_features(errs, dts, psi2max: float, drift: float, dtr: float) -> np.ndar
     return np.array(f, dtype=np.float64)
 
 
-def dataset_from_trace(tr: np.ndarray, dtr: float, rho: float = RHO):
+def error_exponent(traces) -> float:
+    """Effective order p of errmax ~ dt^p near the operating point, fitted by
+    least squares over consecutive first attempts of the traces (steps whose
+    dt changed by more than 5%)."""
+    num = den = 0.0
+    for tr in traces:
+        dl = np.diff(np.log(tr[:-1, 6]))
+        el = np.diff(np.log(np.maximum(tr[:-1, 7], 1e-12)))
+        m = np.abs(dl) > 0.05
+        num += float(np.sum(dl[m] * el[m]))
+        den += float(np.sum(dl[m] * dl[m]))
+    p = num / den if den > 0 else 5.0
+    return min(max(p, 5.0), 20.0)
+
+
+def dataset_from_trace(tr: np.ndarray, dtr: float, rho: float = RHO, p: float = 5.0):
     """(X, Y) pairs of one trace: features after accepted step n, target from
-    the first attempt of step n+1 (the last, clamped step is left out)."""
+    the first attempt of step n+1 (the last, clamped step is left out):
+    the step that would have given errmax = rho, with errmax ~ dt^p."""
     X, Y = [], []
     for n in range(len(tr) - 2):
         x = raw_features(tr[:n + 1, 2], tr[:n + 1, 1], tr[n, 4], tr[n, 5], dtr)
         dt_try, e_try = tr[n + 1, 6], tr[n + 1, 7]
-        target = dt_try * (rho / max(e_try, 1e-8)) ** 0.2
+        target = dt_try * (rho / max(e_try, 1e-8)) ** (1.0 / p)
         y = min(max(math.log(target / tr[n, 1]), CLIP[0]), CLIP[1])
         X.append(x)
         Y.append(y)
@@ -175,9 +192,13 @@ class ExploringController(PlainController):
 
 
 class LearnedController(PlainController):
+    """Proposes dt_next = dt exp(mlp(features)), clipped to [dt/2, 2 dt] and
+    without growth right after a rejected attempt; the reference's rules
+    shrink after a rejection and replace a non-finite proposal.  With
+    explore > 0 the proposal is perturbed (on-policy trace generation)."""
     name = "learned"
 
-    def __init__(self, model: dict, params: RunParams, mlp: MLP | None = None):
+    def __init__(self, model: dict, params: RunParams, mlp: MLP | None = None, explore: float = 0.0, seed: int = 0):
         super().__init__(params.safety, params.dt_decrease, params.dt_increase)
         self.model = model
         self.dtr = dt_ref(params)
@@ -185,18 +206,32 @@ class LearnedController(PlainController):
         self.mlp.set_params(model["params"])
         self.mu = model["mu"]
         self.sd = model["sd"]
+        self.explore = explore
+        self.rng = np.random.default_rng(seed)
         self.n_fallback = 0
+        self.rejected = False
 
     def reset(self):
         self.n_fallback = 0
+        self.rejected = False
+
+    def shrink(self, dt, errmax):
+        self.rejected = True
+        return super().shrink(dt, errmax)
 
     def propose(self, dt, errmax, state):
         x = (raw_features(state.errmax, state.dt, state.psi2max, state.energy_drift, self.dtr) - self.mu) / self.sd
         y = float(self.mlp.forward(x[None, :])[0])
         if not math.isfinite(y):
             self.n_fallback += 1
+            self.rejected = False
             return super().propose(dt, errmax, state)
-        return dt * math.exp(min(max(y, CLIP[0]), CLIP[1]))
+        hi = 0.0 if self.rejected else PROPOSE_CLIP[1]
+        self.rejected = False
+        y = min(max(y, PROPOSE_CLIP[0]), hi)
+        if self.explore > 0:
+            y += self.rng.uniform(-self.explore, self.explore)
+        return dt * math.exp(y)
 
 
 def save_model(path: str, model: dict) -> None:
@@ -221,31 +256,58 @@ TRACE_RUNS = [  # (n, order, seed, eps, T, exploring spread)
 ]
 
 
-def generate_traces(runs=TRACE_RUNS, log=print):
+def generate_traces(runs=TRACE_RUNS, log=print, model=None):
+    """RKF45 traces of the random-phase workload; with a model, the learned
+    controller drives the runs (exploring by `spread`), otherwise the plain
+    controller does (exploring by `spread`)."""
     data = []
     for n, order, seed, eps, T, spread in runs:
         p, psi = ics.config_turb(n=n, seed=seed, scheme="rk45", tau=0.01)
         p = p.replace(order=order, eps=eps)
-        ctrl = ExploringController(seed, spread) if spread > 0 else PlainController()
+        if model is not None:
+            ctrl = LearnedController(model, p, explore=spread, seed=seed)
+        else:
+            ctrl = ExploringController(seed, spread) if spread > 0 else PlainController()
         with Context(p) as ctx:
             ctx.upload(psi)
             st = integrate(ctx, T, 0.01, ctrl, clamp=True, energy=True)
         tr = trace_array(st)
         data.append((p, tr))
-        log(json.dumps({"learned": "trace", "n": n, "order": order, "seed": seed, "eps": eps, "T": T,
-                        "spread": spread, "steps": st.n_accept, "rejects": st.n_reject, "wall_s": st.wall_s}))
+        log(json.dumps({"learned": "trace", "policy": "learned" if model is not None else "plain", "n": n,
+                        "order": order, "seed": seed, "eps": eps, "T": T, "spread": spread, "steps": st.n_accept,
+                        "rejects": st.n_reject, "wall_s": st.wall_s}))
     return data
 
 
+def train_pipeline(runs=TRACE_RUNS, rounds: int = 2, steps: int = 4000, log=print):
+    """Round 1: traces of the plain/exploring controller, train.  Later
+    rounds: traces of the learned controller itself (small exploration),
+    added to the data set, retrain (on-policy aggregation)."""
+    data = generate_traces(runs, log=log)
+    model = None
+    for r in range(rounds):
+        if r > 0:
+            data += generate_traces([(n, o, s + 100 * r, e, T, 0.1) for n, o, s, e, T, _ in runs], log=log,
+                                    model=model)
+        X, Y = build_dataset(data)
+        model = train_model(X, Y, steps=steps, log=log)
+        model["meta"]["round"] = r
+        model["meta"]["error_exponent"] = build_dataset.last_exponent
+        log(json.dumps({"learned": "round", "round": r, "error_exponent": build_dataset.last_exponent}))
+    return model
+
+
 def build_dataset(data, rho=RHO, seed=0):
+    pexp = error_exponent([tr for _, tr in data])
     Xs, Ys = [], []
     for p, tr in data:
-        X, Y = dataset_from_trace(tr, dt_ref(p), rho)
+        X, Y = dataset_from_trace(tr, dt_ref(p), rho, pexp)
         Xs.append(X)
         Ys.append(Y)
     X = np.concatenate(Xs)
     Y = np.concatenate(Ys)
     perm = np.random.default_rng(seed).permutation(len(X))
+    build_dataset.last_exponent = pexp
     return X[perm], Y[perm]
 
 
@@ -307,9 +369,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4000)
     a = ap.parse_args(argv)
     if a.what in ("all", "train"):
-        data = generate_traces()
-        X, Y = build_dataset(data)
-        model = train_model(X, Y, steps=a.steps)
+        model = train_pipeline(steps=a.steps)
         save_model(a.model, model)
     if a.what in ("all", "evaluate"):
         evaluate(load_model(a.model), n=a.n, T=a.T)
