@@ -78,6 +78,8 @@ PROTOTYPES = [
     ("gpe_ens_download", _I, [_VP, _I, _VP, _I]),
     ("gpe_ens_step", _I, [_VP, _I, _D, _D, _I]),
     ("gpe_ens_work_items", _I, [_VP]),
+    ("gpe_ens_set_grouping", _I, [_VP, _I64, _I]),
+    ("gpe_ens_num_groups", _I, [_VP]),
     ("gpe_ens_stream", _VP, [_VP]),
     ("gpe_ens_sync", _I, [_VP]),
     ("gpe_mlp_create", _I, [_I, ct.POINTER(_VP)]),

@@ -2,16 +2,23 @@
 // advanced with RK4.  All members share the equation, order, boundaries and
 // storage precision; each has its own grid, spacing and fields.
 //
-// Three ways to run a step (gpe_ens_step):
+// Four ways to run a step (gpe_ens_step):
 //   GPE_ENS_PER_MEMBER  one launch per member and stage (4 x members launches)
 //   GPE_ENS_BATCHED     one launch per stage for all members: persistent
 //                       blocks walk a work list of (member, tile) items
 //   GPE_ENS_GRAPH       GPE_ENS_BATCHED captured once into a hipGraph of a
 //                       chunk of steps and replayed
+//   GPE_ENS_GROUPED     members packed into groups whose fields fit the
+//                       256 MiB Infinity Cache; each group advances a chunk
+//                       of steps back to back (members are independent, so
+//                       the order of their steps is free) while its fields
+//                       stay cache resident; the chunk is captured in a
+//                       hipGraph and replayed
 // Each member's arithmetic is the single-context kernel's (same operators,
 // same per-member tiling), so results do not depend on the mode.
 #include <math.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -100,11 +107,19 @@ struct gpe_ens {
     std::vector<int> tz;
     int total = 0;
     int grid = 0;
+    // grouped mode: members in descriptor order, groups of that order
+    size_t group_budget = (size_t)192 << 20;
+    int group_steps = 16;
+    std::vector<int> order;               // descriptor slot -> member
+    struct Group { int first, count, items, grid; };
+    std::vector<Group> groups;
+    void* gdesc = nullptr;                // descriptors in group order, begin relative to the group
     bool desc_dirty = true;
     // graph of `gsteps` batched steps with dt (gre, gim)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;
+    int gmode = -1;
     double gre = 0, gim = 0;
     std::string err;
 };
@@ -174,23 +189,64 @@ template <typename R> int build_desc(gpe_ens* e) {
     EHIP(e, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     per_cu = (1024 / (e->tx * e->ty)) * 2;
     e->grid = ncu * per_cu < e->total ? ncu * per_cu : e->total;
+
+    // groups for GPE_ENS_GROUPED: members by increasing size, packed while the
+    // group's four fields fit the budget (a larger member is a group alone)
+    std::vector<int> idx(e->n);
+    for (int m = 0; m < e->n; ++m) idx[m] = m;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return e->cells[a] < e->cells[b]; });
+    const size_t esz = e->mp[0].storage == GPE_C128 ? 16 : 8;
+    e->order = idx;
+    e->groups.clear();
+    std::vector<EnsMember<R>> g(e->n);
+    size_t used = 0;
+    for (int s = 0; s < e->n; ++s) {
+        const int m = idx[s];
+        const size_t ws = 4 * (size_t)e->cells[m] * esz;
+        if (e->groups.empty() || used + ws > e->group_budget) {
+            e->groups.push_back({s, 0, 0, 0});
+            used = 0;
+        }
+        auto& G = e->groups.back();
+        g[s] = h[m];
+        g[s].begin = G.items;
+        G.items += h[m].g.nblocks;
+        G.count += 1;
+        used += ws;
+    }
+    for (auto& G : e->groups) G.grid = ncu * per_cu < G.items ? ncu * per_cu : G.items;
+    if (!e->gdesc) EHIP(e, hipMalloc(&e->gdesc, sizeof(EnsMember<R>) * e->n));
+    EHIP(e, hipMemcpy(e->gdesc, g.data(), sizeof(EnsMember<R>) * e->n, hipMemcpyHostToDevice));
     e->desc_dirty = false;
     drop_graph(e);
     return 0;
 }
 
-template <typename R> int step_batched(gpe_ens* e, double dre, double dim) {
-    const C<R>* dummy = nullptr;
-    (void)dummy;
-    const EnsMember<R>* mem = (const EnsMember<R>*)e->ddesc;
+template <typename R>
+int step_launches(gpe_ens* e, const EnsMember<R>* mem, int nmem, int total, int grid, double dre, double dim) {
     const int ord = e->mp[0].order;
     struct St { int stage; double s1, s2; } st[4] = {{1, 0.5, 1.0 / 6.0}, {2, 0.5, 1.0 / 3.0}, {3, 1.0, 1.0 / 3.0}, {4, 1.0 / 6.0, 0.0}};
     for (auto& s : st) {
         const C<R> a1 = cdt<R>(dre, dim, s.s1), a2 = cdt<R>(dre, dim, s.s2);
-        hipError_t x = ord == 2 ? launch_batched<R, 2>(e->tx, e->ty, s.stage, mem, e->n, e->total, e->grid, a1, a2, e->stream)
-                                : launch_batched<R, 4>(e->tx, e->ty, s.stage, mem, e->n, e->total, e->grid, a1, a2, e->stream);
+        hipError_t x = ord == 2 ? launch_batched<R, 2>(e->tx, e->ty, s.stage, mem, nmem, total, grid, a1, a2, e->stream)
+                                : launch_batched<R, 4>(e->tx, e->ty, s.stage, mem, nmem, total, grid, a1, a2, e->stream);
         EHIP(e, x);
     }
+    return 0;
+}
+
+template <typename R> int step_batched(gpe_ens* e, double dre, double dim) {
+    return step_launches<R>(e, (const EnsMember<R>*)e->ddesc, e->n, e->total, e->grid, dre, dim);
+}
+
+// `steps` steps of every member, group after group
+template <typename R> int steps_grouped(gpe_ens* e, int steps, double dre, double dim) {
+    const EnsMember<R>* base = (const EnsMember<R>*)e->gdesc;
+    for (const auto& G : e->groups)
+        for (int s = 0; s < steps; ++s) {
+            int rc = step_launches<R>(e, base + G.first, G.count, G.items, G.grid, dre, dim);
+            if (rc) return rc;
+        }
     return 0;
 }
 
@@ -240,14 +296,17 @@ template <typename R> int ens_step(gpe_ens* e, int mode, double dre, double dim,
         }
         return 0;
     }
-    // GPE_ENS_GRAPH: capture a chunk of batched steps once per dt, replay it
-    const int chunk = nsteps < 50 ? nsteps : 50;
+    // GPE_ENS_GRAPH / GPE_ENS_GROUPED: capture a chunk once per (mode, dt), replay it
+    const bool grouped = mode == GPE_ENS_GROUPED;
+    const int chunk = grouped ? (nsteps < e->group_steps ? nsteps : e->group_steps) : (nsteps < 50 ? nsteps : 50);
     if (chunk <= 0) return 0;
-    if (!e->gexec || e->gsteps != chunk || e->gre != dre || e->gim != dim) {
+    if (!e->gexec || e->gsteps != chunk || e->gre != dre || e->gim != dim || e->gmode != mode) {
         drop_graph(e);
         EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
         int rc = 0;
-        for (int s = 0; s < chunk && !rc; ++s) rc = step_batched<R>(e, dre, dim);
+        if (grouped) rc = steps_grouped<R>(e, chunk, dre, dim);
+        else
+            for (int s = 0; s < chunk && !rc; ++s) rc = step_batched<R>(e, dre, dim);
         hipGraph_t g = nullptr;
         hipError_t x = hipStreamEndCapture(e->stream, &g);
         if (rc) {
@@ -260,14 +319,17 @@ template <typename R> int ens_step(gpe_ens* e, int mode, double dre, double dim,
         e->gsteps = chunk;
         e->gre = dre;
         e->gim = dim;
+        e->gmode = mode;
     }
     int done = 0;
     while (done + e->gsteps <= nsteps) {
         EHIP(e, hipGraphLaunch(e->gexec, e->stream));
         done += e->gsteps;
     }
-    for (; done < nsteps; ++done) {
-        int rc = step_batched<R>(e, dre, dim);
+    if (done < nsteps) {
+        const int rest = nsteps - done;
+        int rc = grouped ? steps_grouped<R>(e, rest, dre, dim) : 0;
+        for (int s = 0; !grouped && s < rest && !rc; ++s) rc = step_batched<R>(e, dre, dim);
         if (rc) return rc;
     }
     return 0;
@@ -338,6 +400,7 @@ void gpe_ens_destroy(gpe_ens* e) {
             if (p) (void)hipFree(p);
     }
     if (e->ddesc) (void)hipFree(e->ddesc);
+    if (e->gdesc) (void)hipFree(e->gdesc);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -378,13 +441,31 @@ int gpe_ens_download(gpe_ens* e, int m, void* host, int host_dtype) {
 
 int gpe_ens_step(gpe_ens* e, int mode, double dt_re, double dt_im, int nsteps) {
     if (!e) return efail(nullptr, GPE_EINVAL, "gpe_ens_step: ens is NULL");
-    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_GRAPH || nsteps < 0)
+    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_GROUPED || nsteps < 0)
         return efail(e, GPE_EINVAL, "gpe_ens_step: bad mode or nsteps");
     return e->mp[0].storage == GPE_C128 ? ens_step<double>(e, mode, dt_re, dt_im, nsteps)
                                         : ens_step<float>(e, mode, dt_re, dt_im, nsteps);
 }
 
 int gpe_ens_work_items(const gpe_ens* e) { return e ? e->total : -1; }
+
+int gpe_ens_set_grouping(gpe_ens* e, int64_t budget_bytes, int steps_per_visit) {
+    if (!e) return efail(nullptr, GPE_EINVAL, "gpe_ens_set_grouping: ens is NULL");
+    if (budget_bytes <= 0 || steps_per_visit <= 0) return efail(e, GPE_EINVAL, "gpe_ens_set_grouping: bad arguments");
+    e->group_budget = (size_t)budget_bytes;
+    e->group_steps = steps_per_visit;
+    e->desc_dirty = true;
+    return 0;
+}
+
+int gpe_ens_num_groups(gpe_ens* e) {
+    if (!e) return efail(nullptr, GPE_EINVAL, "gpe_ens_num_groups: ens is NULL");
+    if (e->desc_dirty) {
+        int rc = e->mp[0].storage == GPE_C128 ? build_desc<double>(e) : build_desc<float>(e);
+        if (rc) return rc;
+    }
+    return (int)e->groups.size();
+}
 
 void* gpe_ens_stream(gpe_ens* e) { return e ? (void*)e->stream : nullptr; }
 

@@ -10,7 +10,8 @@ RK4 with dt = 0.005 for 1000 steps.
     python -m gpe_amd.ensemble [--steps 1000] [--members 48] [--no-tune]
 
 prints one JSON line per mode (one launch per member, batched persistent
-launches, hipGraph replay) with total cell-steps/s, and the autotuner log.
+launches, hipGraph replay, cache-resident groups) with total cell-steps/s,
+and the autotuner log.
 """
 from __future__ import annotations
 
@@ -26,7 +27,7 @@ import numpy as np
 from . import _lib, ics
 from .params import RunParams, xr_for_spacing
 
-MODES = {"per_member": 0, "batched": 1, "graph": 2}
+MODES = {"per_member": 0, "batched": 1, "graph": 2, "grouped": 3}
 SIZES = (64, 96, 128, 160, 192)
 
 
@@ -78,6 +79,14 @@ class Ensemble:
     def set_tiles(self, tx: int, ty: int, tz=None):
         arr = None if tz is None else (ct.c_int * len(self.members))(*[int(v) for v in tz])
         self._chk(self.lib.gpe_ens_set_tiles(self.h, tx, ty, arr), "gpe_ens_set_tiles")
+
+    def set_grouping(self, budget_bytes: int, steps_per_visit: int):
+        self._chk(self.lib.gpe_ens_set_grouping(self.h, budget_bytes, steps_per_visit), "gpe_ens_set_grouping")
+
+    def num_groups(self) -> int:
+        n = self.lib.gpe_ens_num_groups(self.h)
+        self._chk(0 if n >= 0 else n, "gpe_ens_num_groups")
+        return n
 
     def step(self, mode: str, dt: complex, nsteps: int):
         self._chk(self.lib.gpe_ens_step(self.h, MODES[mode], dt.real, dt.imag, nsteps), "gpe_ens_step")
@@ -131,7 +140,7 @@ class Timer:
 
 
 def autotune(ens: Ensemble, dt: complex, steps: int = 10, shapes=((32, 8), (64, 4), (64, 8), (32, 4)),
-             tzs=(4, 8, 16, 32), log=None):
+             tzs=(4, 8, 16, 32), size_tzs=(4, 8, 16, 32, 64, 96, 192), log=None):
     """Tile search for the batched launch: global (tx, ty, tz), then tz per
     grid size.  The state is advanced while tuning, so callers re-upload.
     Returns (tx, ty, tz_per_member, log)."""
@@ -154,7 +163,7 @@ def autotune(ens: Ensemble, dt: complex, steps: int = 10, shapes=((32, 8), (64, 
     ms0, tx, ty, tzm = best
     # per-size refinement of tz
     for n in sorted({m.nz for m in ens.members}):
-        for tz in tzs:
+        for tz in [t for t in size_tzs if t <= n]:
             trial = [min(tz, m.nz) if m.nz == n else t for m, t in zip(ens.members, tzm)]
             ms = measure(tx, ty, trial)
             log.append({"tx": tx, "ty": ty, "size": n, "tz": tz, "ms_per_step": ms})
@@ -177,7 +186,8 @@ def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
             print(json.dumps({"ensemble": "autotune", "tx": tx, "ty": ty,
                               "tz_by_size": {str(m.nx): t for m, t in zip(members, tzm)}, "log": log}), flush=True)
         rows = []
-        for mode in ("per_member", "batched", "graph"):
+        print(json.dumps({"ensemble": "groups", "count": ens.num_groups()}), flush=True)
+        for mode in ("per_member", "batched", "graph", "grouped"):
             for m, f in enumerate(fields):
                 ens.upload(m, f)
             ens.step(mode, dt, 2)  # warm-up (and graph capture)

@@ -194,11 +194,17 @@ class ExploringController(PlainController):
 class LearnedController(PlainController):
     """Proposes dt_next = dt exp(mlp(features)), clipped to [dt/2, 2 dt] and
     without growth right after a rejected attempt; the reference's rules
-    shrink after a rejection and replace a non-finite proposal.  With
-    explore > 0 the proposal is perturbed (on-policy trace generation)."""
+    shrink after a rejection and replace a non-finite proposal.
+
+    The plain controller is the fallback and the judge: when more than
+    `max_reject` of the last `window` steps needed a retry, the plain rule
+    proposes for the next `cooldown` steps (a problem unlike the training
+    data cannot cost more than a few rejections).  With explore > 0 the
+    proposal is perturbed (on-policy trace generation)."""
     name = "learned"
 
-    def __init__(self, model: dict, params: RunParams, mlp: MLP | None = None, explore: float = 0.0, seed: int = 0):
+    def __init__(self, model: dict, params: RunParams, mlp: MLP | None = None, explore: float = 0.0, seed: int = 0,
+                 window: int = 16, max_reject: float = 0.15, cooldown: int = 64):
         super().__init__(params.safety, params.dt_decrease, params.dt_increase)
         self.model = model
         self.dtr = dt_ref(params)
@@ -208,18 +214,30 @@ class LearnedController(PlainController):
         self.sd = model["sd"]
         self.explore = explore
         self.rng = np.random.default_rng(seed)
-        self.n_fallback = 0
-        self.rejected = False
+        self.window, self.max_reject, self.cooldown = window, max_reject, cooldown
+        self.reset()
 
     def reset(self):
-        self.n_fallback = 0
+        self.n_fallback = 0        # non-finite proposals
+        self.n_plain_steps = 0     # steps proposed by the plain rule after the judge intervened
         self.rejected = False
+        self.recent = []
+        self.hold = 0
 
     def shrink(self, dt, errmax):
         self.rejected = True
         return super().shrink(dt, errmax)
 
     def propose(self, dt, errmax, state):
+        self.recent = (self.recent + [1 if self.rejected else 0])[-self.window:]
+        if self.hold == 0 and len(self.recent) == self.window and sum(self.recent) > self.max_reject * self.window:
+            self.hold = self.cooldown
+            self.recent = []
+        if self.hold > 0:
+            self.hold -= 1
+            self.n_plain_steps += 1
+            self.rejected = False
+            return super().propose(dt, errmax, state)
         x = (raw_features(state.errmax, state.dt, state.psi2max, state.energy_drift, self.dtr) - self.mu) / self.sd
         y = float(self.mlp.forward(x[None, :])[0])
         if not math.isfinite(y):
@@ -347,8 +365,11 @@ def evaluate(model, n=384, dx=0.4, T=40.0, eps=1e-6, oracle_eps=1e-10, log=print
             ctx.upload(psi)
             st = integrate(ctx, T, 0.01, ctrl, clamp=True, energy=(name == "learned"))
             finals[name] = ctx.download()
-        rows.append({"learned": "evaluate", "method": name, "n": n, "T": T, "eps": e, "steps": st.n_accept,
-                     "rejects": st.n_reject, "rhs": st.n_rhs, "wall_s": st.wall_s})
+        row = {"learned": "evaluate", "method": name, "n": n, "T": T, "eps": e, "steps": st.n_accept,
+               "rejects": st.n_reject, "rhs": st.n_rhs, "wall_s": st.wall_s}
+        if name == "learned":
+            row["plain_rule_steps"] = ctrl.n_plain_steps
+        rows.append(row)
     for r in rows:
         r["rel_l2_vs_oracle"] = rel_l2(finals[r["method"]], finals["oracle"]) if r["method"] != "oracle" else 0.0
         log(json.dumps(r))

@@ -192,6 +192,7 @@ int gpe_bench_stream5(void* c, void* d, const void* a, const void* b, const void
 #define GPE_ENS_PER_MEMBER 0  /* one launch per member and stage */
 #define GPE_ENS_BATCHED 1     /* one persistent launch per stage for all members */
 #define GPE_ENS_GRAPH 2       /* GPE_ENS_BATCHED captured in a hipGraph and replayed */
+#define GPE_ENS_GROUPED 3     /* cache-sized groups advanced a chunk of steps each, in a hipGraph */
 typedef struct gpe_ens gpe_ens;
 int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out);
 void gpe_ens_destroy(gpe_ens* ens);
@@ -203,6 +204,10 @@ int gpe_ens_upload(gpe_ens* ens, int member, const void* host, int host_dtype);
 int gpe_ens_download(gpe_ens* ens, int member, void* host, int host_dtype);
 int gpe_ens_step(gpe_ens* ens, int mode, double dt_re, double dt_im, int nsteps);
 int gpe_ens_work_items(const gpe_ens* ens);
+/* GPE_ENS_GROUPED: byte budget of a group's four fields (default 192 MiB)
+ * and the steps a group advances per visit (default 16) */
+int gpe_ens_set_grouping(gpe_ens* ens, int64_t budget_bytes, int steps_per_visit);
+int gpe_ens_num_groups(gpe_ens* ens);
 void* gpe_ens_stream(gpe_ens* ens);
 int gpe_ens_sync(gpe_ens* ens);
 

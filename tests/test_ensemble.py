@@ -52,13 +52,16 @@ def test_create_rejects_mixed_members():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
-@pytest.mark.parametrize("mode", ["per_member", "batched", "graph"])
+@pytest.mark.parametrize("mode", ["per_member", "batched", "graph", "grouped"])
 def test_modes_match_single_context(order, bcs, mode):
     ms = members(order, bcs)
     fs = fields(ms)
     dt = 0.005 + 0j
     exp = [single(m, f, 7, dt) for m, f in zip(ms, fs)]
     with Ensemble(ms) as ens:
+        # small groups (about two members each) visited 3 steps at a time: 7 = 3 + 3 + 1
+        ens.set_grouping(2 * 4 * 16 * 24 * 24 * 24, 3)
+        assert ens.num_groups() >= 2
         for i, f in enumerate(fs):
             ens.upload(i, f)
         ens.step(mode, dt, 7)
