@@ -9,7 +9,8 @@ is one RK4 step of the whole grid (4 fused stencil+stage kernels); inputs
 are resident in HBM when the timed region starts.
 
 Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
-dominant kernel (RK4 stages 2/3, timed live with HIP events around each
+dominant kernel (RK4 stage 3, 5 field accesses per cell of the step's 13,
+timed live with HIP events around each
 launch on the context's stream), the complex64-storage run, and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC).
@@ -24,6 +25,7 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+DOMINANT = "rk4_stage3"  # the RK4 kernel with the most traffic per launch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
@@ -87,11 +89,13 @@ def timed_steps(ctx, hip, scheme, steps, info):
 
 def kernel_profile(ctx, scheme, steps, info):
     """Per-kind launch count, average duration (ms) and algorithmic bytes per launch."""
+    from gpe_amd import _lib
+
     lib = ctx.lib
     assert lib.gpe_prof_enable(ctx.h, 1) == 0
     ctx.step(scheme, steps, info)
     out = {}
-    for k in range(15):
+    for k in range(_lib.GPE_NKINDS):
         ms, n, b = ct.c_double(), ct.c_int64(), ct.c_double()
         assert lib.gpe_prof_read(ctx.h, k, ct.byref(ms), ct.byref(n), ct.byref(b)) == 0
         if n.value:
@@ -177,8 +181,8 @@ def main(argv=None) -> int:
     ms, prof, tiles = run_storage(p, psi, args, hip)
     print(f"[bench] c128: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     value = p.cells * args.steps / (ms * 1e-3) / 1e9
-    step_bytes = sum(v["bytes"] for k, v in prof.items() if k.startswith("rk4")) + prof["rk4_stage2_3"]["bytes"]
-    rf = roofline(prof, "rk4_stage2_3", load_traffic(f"traffic_rk4_stage2_3_c128_{args.n}.json"))
+    step_bytes = sum(v["bytes"] for k, v in prof.items() if k.startswith("rk4"))
+    rf = roofline(prof, DOMINANT, load_traffic(f"traffic_{DOMINANT}_c128_{args.n}.json"))
     result = {
         "metric": "RK4 Gcell-steps/s + %HBM peak at 512³; time-to-T, learned vs plain RKF45",
         "value": round(value, 4),
@@ -207,8 +211,8 @@ def main(argv=None) -> int:
         print(f"[bench] c64: {ms64 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
         result["c64"] = {"value": round(p.cells * args.steps / (ms64 * 1e-3) / 1e9, 4),
                          "ms_per_step": round(ms64 / args.steps, 4), "dtype": "f32 stencil, f64 reductions",
-                         "roofline": roofline(prof64, "rk4_stage2_3",
-                                              load_traffic(f"traffic_rk4_stage2_3_c64_{args.n}.json"))}
+                         "roofline": roofline(prof64, DOMINANT,
+                                              load_traffic(f"traffic_{DOMINANT}_c64_{args.n}.json"))}
     if args.cpu_steps > 0:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:

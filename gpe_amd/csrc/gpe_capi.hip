@@ -68,8 +68,9 @@ struct gpe_ctx {
 };
 
 static const char* const kKindNames[GPE_NKINDS] = {
-    "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2_3", "rk4_stage4", "rkck_stage1",
-    "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy", "hamiltonian"};
+    "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2", "rk4_stage4", "rkck_stage1",
+    "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
+    "hamiltonian", "rk4_stage3"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -235,13 +236,14 @@ template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, dou
         if ((rc = run_op<R, ORD>(c, uA, s2))) return rc;
         std::swap(c->psi, c->uB);
     } else {
-        // classic RK4; acc carries psi0 + dt (k1/6 + k2/3 + k3/3) through the stages
+        // classic RK4 in 13 field accesses (see OpRk4Third): u1 -> uA, u2 -> uB,
+        // u3 over u1 in uA, accumulator in acc, psi_new over psi0
         C<R>* acc = (C<R>*)c->acc;
-        OpRk4First<R> s1{k, uA, acc, cdt<R>(dre, dim, 0.5), cdt<R>(dre, dim, 1.0 / 6.0)};
+        OpRk4First<R> s1{k, uA, cdt<R>(dre, dim, 0.5)};
         if ((rc = run_op<R, ORD>(c, psi, s1))) return rc;
-        OpRk4Mid<R> s2{k, psi, acc, uB, cdt<R>(dre, dim, 0.5), cdt<R>(dre, dim, 1.0 / 3.0)};
+        OpRk4Second<R> s2{k, psi, uB, cdt<R>(dre, dim, 0.5)};
         if ((rc = run_op<R, ORD>(c, uA, s2))) return rc;
-        OpRk4Mid<R> s3{k, psi, acc, uA, cdt<R>(dre, dim, 1.0), cdt<R>(dre, dim, 1.0 / 3.0)};
+        OpRk4Third<R> s3{k, psi, uA, acc, cdt<R>(dre, dim, 1.0), cdt<R>(dre, dim, 1.0 / 3.0)};
         if ((rc = run_op<R, ORD>(c, uB, s3))) return rc;
         OpRk4Last<R> s4{k, acc, psi, cdt<R>(dre, dim, 1.0 / 6.0)};
         if ((rc = run_op<R, ORD>(c, uA, s4))) return rc;

@@ -40,7 +40,8 @@ hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u
     X(R, OpAxpySelf<R>)                                                                                            \
     X(R, OpAxpyP0<R>)                                                                                              \
     X(R, OpRk4First<R>)                                                                                            \
-    X(R, OpRk4Mid<R>)                                                                                              \
+    X(R, OpRk4Second<R>)                                                                                           \
+    X(R, OpRk4Third<R>)                                                                                            \
     X(R, OpRk4Last<R>)                                                                                             \
     X(R, OpCk1<R>)                                                                                                 \
     X(R, OpCk2<R>)                                                                                       \

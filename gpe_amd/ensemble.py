@@ -198,7 +198,7 @@ def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
             wall = time.perf_counter() - t0
             row = {"ensemble": mode, "steps": steps, "ms_per_step": ms / steps,
                    "gcell_steps_s": cells * steps / (ms * 1e-3) / 1e9, "wall_s": wall,
-                   "hbm_frac": cells * 16 * (16 if storage == "c128" else 8) * steps / (ms * 1e-3) / 8e12}
+                   "hbm_frac": cells * 13 * (16 if storage == "c128" else 8) * steps / (ms * 1e-3) / 8e12}
             rows.append(row)
             print(json.dumps(row), flush=True)
         base = rows[0]["gcell_steps_s"]

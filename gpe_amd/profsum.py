@@ -20,11 +20,12 @@ import os
 import re
 
 KINDS = {
-    "OpRk4First": "rk4_stage1", "OpRk4Mid": "rk4_stage2_3", "OpRk4Last": "rk4_stage4",
+    "OpRk4First": "rk4_stage1", "OpRk4Second": "rk4_stage2", "OpRk4Third": "rk4_stage3", "OpRk4Last": "rk4_stage4",
     "OpAxpySelf": "euler_or_rk2_stage1", "OpAxpyP0": "rk2_stage2", "OpCk1": "rkck_stage1",
     "OpCkLast": "rkck_stage6_error", "OpCkMid<double, 2>": "rkck_stage2", "OpCkMid<float, 2>": "rkck_stage2",
 }
-ACCESS = {"rk4_stage1": 3, "rk4_stage2_3": 5, "rk4_stage4": 3}
+ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3}
+DOMINANT = "rk4_stage3"
 
 
 def _one(pattern):
@@ -103,12 +104,12 @@ def main(argv=None):
         rd = fk * f_scale if fk else None
         wr = wk * w_scale if wk else None
         t = (rd + wr) if (rd and wr) else None
-        if t and kind in ("rk4_stage2_3",):
+        if t and kind == DOMINANT:
             traffic[prec] = t
             json.dump({"traffic_bytes_per_launch": t, "read_bytes": rd, "write_bytes": wr,
                        "algorithmic_bytes_per_launch": alg, "fetch_scale": f_scale, "write_scale": w_scale,
                        "kernel": label},
-                      open(os.path.join(a.out, f"traffic_rk4_stage2_3_{prec}_{a.n}.json"), "w"), indent=1)
+                      open(os.path.join(a.out, f"traffic_{DOMINANT}_{prec}_{a.n}.json"), "w"), indent=1)
         lines.append(f"| {label} | {r['Calls']} | {avg_ms:.4f} | {alg / 1e9:.3f} | "
                      f"{alg / (avg_ms * 1e-3) / 1e9:.0f} | {100 * alg / (avg_ms * 1e-3) / 8e12:.1f} | "
                      f"{(rd or 0) / 1e9:.3f} | {(wr or 0) / 1e9:.3f} | {(t / alg) if (t and alg) else float('nan'):.3f} |")

@@ -29,6 +29,16 @@ def test_library_exports_every_declared_symbol():
     assert lib.gpe_abi_version() == 1
 
 
+def test_kernel_kind_names_match_header_count():
+    text = open(os.path.join(ROOT, "include", "gpe_hip.h")).read()
+    assert int(re.search(r"#define GPE_NKINDS (\d+)", text).group(1)) == _lib.GPE_NKINDS
+    lib = _lib.load()
+    names = [lib.gpe_kernel_kind_name(k).decode() for k in range(_lib.GPE_NKINDS)]
+    assert all(names) and len(set(names)) == len(names)
+    assert {"rk4_stage1", "rk4_stage2", "rk4_stage3", "rk4_stage4"} <= set(names)
+    assert lib.gpe_kernel_kind_name(_lib.GPE_NKINDS) == b""
+
+
 def test_struct_layouts_match_header():
     # gpe_params: 8 int32 then 16 doubles; gpe_step_info: 6 doubles + 3 int64
     assert ct.sizeof(_lib.gpe_params) == 8 * 4 + 16 * 8
