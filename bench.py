@@ -138,6 +138,11 @@ def roofline(prof, kind, traffic=None):
             "algorithmic_bytes_per_launch": k["bytes"]}
 
 
+def kernel_table(prof):
+    return {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
+                "GB_s": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k, v in prof.items()}
+
+
 def cpu_baseline(p, psi, steps):
     """The FP64 oracle (plain C restatement of the reference, OpenMP) on
     `steps` RK4 steps of the same workload."""
@@ -160,6 +165,7 @@ def main(argv=None) -> int:
     ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (counter calibration)")
     args = ap.parse_args(argv)
@@ -202,8 +208,7 @@ def main(argv=None) -> int:
                    "storage": "c128", "tiles": list(tiles)},
         "roofline": rf,
         "step_hbm_frac": round(step_bytes * args.steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "kernels": {k: {"avg_ms": round(v["avg_ms"], 5), "launches": v["launches"],
-                        "GB_s": round(v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9, 1)} for k, v in prof.items()},
+        "kernels": kernel_table(prof),
     }
     if not args.no_c64:
         p64 = p.replace(storage="c64")
@@ -212,7 +217,18 @@ def main(argv=None) -> int:
         result["c64"] = {"value": round(p.cells * args.steps / (ms64 * 1e-3) / 1e9, 4),
                          "ms_per_step": round(ms64 / args.steps, 4), "dtype": "f32 stencil, f64 reductions",
                          "roofline": roofline(prof64, DOMINANT,
-                                              load_traffic(f"traffic_{DOMINANT}_c64_{args.n}.json"))}
+                                              load_traffic(f"traffic_{DOMINANT}_c64_{args.n}.json")),
+                         "kernels": kernel_table(prof64)}
+    if not args.no_mixed:
+        pm = p.replace(storage="mixed")
+        msm, profm, _ = run_storage(pm, psi, args, hip)
+        print(f"[bench] mixed: {msm / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        result["mixed"] = {"value": round(p.cells * args.steps / (msm * 1e-3) / 1e9, 4),
+                           "ms_per_step": round(msm / args.steps, 4),
+                           "dtype": "f64 state and stencil, complex64 RK increments",
+                           "roofline": roofline(profm, "mx_rk4_stage4",
+                                                load_traffic(f"traffic_mx_rk4_stage4_mixed_{args.n}.json")),
+                           "kernels": kernel_table(profm)}
     if args.cpu_steps > 0:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:

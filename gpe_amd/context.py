@@ -59,7 +59,7 @@ class Context:
         h = ct.c_void_p()
         _lib.check(self.lib.gpe_ctx_create(ct.byref(self._cp), ct.byref(h)), None, "gpe_ctx_create")
         self.h = h
-        self.storage_dtype = np.complex128 if params.storage == "c128" else np.complex64
+        self.storage_dtype = np.complex64 if params.storage == "c64" else np.complex128
 
     # ---- lifetime
     def close(self) -> None:

@@ -7,6 +7,7 @@
 
 #include <string>
 #include <utility>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -39,6 +40,9 @@ struct gpe_ctx {
     void* k4 = nullptr;
     void* k5 = nullptr;
     void* pnew = nullptr;
+    // mixed storage (GPE_MIXED): complex64 increments K1..K3 (RK4) or
+    // F0, K2..K5, D_A, D_B (RKCK)
+    void* mx[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     bool f0_valid = false;
     bool cand_valid = false;
     // drop-in scratch
@@ -70,7 +74,8 @@ struct gpe_ctx {
 static const char* const kKindNames[GPE_NKINDS] = {
     "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2", "rk4_stage4", "rkck_stage1",
     "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
-    "hamiltonian", "rk4_stage3"};
+    "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
+    "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -120,11 +125,15 @@ static int validate(const gpe_params* p, std::string& why) {
     if (!p) { why = "params is NULL"; return GPE_EINVAL; }
     const int mn = p->order == 4 ? 6 : 6;
     if (p->nx < mn || p->ny < mn || p->nz < mn) { why = "nx, ny, nz must be >= 6"; return GPE_EINVAL; }
-    if ((long long)p->nx * p->ny > (1LL << 31) - 1) { why = "nx*ny too large"; return GPE_EINVAL; }
+    // the kernels address a plane with 32-bit byte offsets (16 B per value)
+    if ((long long)p->nx * p->ny > (1LL << 28)) { why = "nx*ny must be <= 2^28"; return GPE_EINVAL; }
     if (p->order != 2 && p->order != 4) { why = "order must be 2 or 4"; return GPE_EINVAL; }
     if (p->bcs != 1 && p->bcs != 2) { why = "bcs must be 1 (periodic) or 2 (reflective)"; return GPE_EINVAL; }
     if (p->eqn_to_solve < 1 || p->eqn_to_solve > 4) { why = "eqn_to_solve must be 1..4"; return GPE_EINVAL; }
-    if (p->storage != GPE_C128 && p->storage != GPE_C64) { why = "storage must be GPE_C128 or GPE_C64"; return GPE_EINVAL; }
+    if (p->storage != GPE_C128 && p->storage != GPE_C64 && p->storage != GPE_MIXED) {
+        why = "storage must be GPE_C128, GPE_C64 or GPE_MIXED";
+        return GPE_EINVAL;
+    }
     if (!(p->xr > 0) || !(p->yr > 0) || !(p->zr > 0)) { why = "xr, yr, zr must be > 0"; return GPE_EINVAL; }
     if (!(p->eps > 0) || !(p->safety > 0)) { why = "eps and safety must be > 0"; return GPE_EINVAL; }
     return 0;
@@ -133,17 +142,21 @@ static int validate(const gpe_params* p, std::string& why) {
 // Default tiling: 64x16 tiles for wide planes (one 1024-thread block per CU
 // marching long z-ranges, so neighbouring tiles advance together and share
 // halo rows through L2), 32x8 tiles for small planes; tz is chosen so the
-// launch has at least 256 blocks (one per CU).
+// launch has at least one block per CU (256), or two for complex64 storage,
+// whose kernels are small enough for two 1024-thread blocks per CU (the
+// second block's loads overlap the first one's barrier and compute).
 static int tz_for(const gpe_params& p, int tx, int ty) {
     const long long ntxy = (long long)((p.nx + tx - 1) / tx) * ((p.ny + ty - 1) / ty);
-    long long tz = (p.nz * ntxy) / 256;
+    const long long want = p.storage == GPE_C64 ? 512 : 256;
+    long long tz = (p.nz * ntxy) / want;
     if (tz < 4) tz = 4;
     if (tz > p.nz) tz = p.nz;
     return (int)tz;
 }
 
 static Tiles default_tiles(const gpe_params& p) {
-    Tiles t{64, 16, 0};
+    // mixed storage: two input streams per stencil load; 64x8 measured best
+    Tiles t{64, p.storage == GPE_MIXED ? 8 : 16, 0};
     if ((long long)p.nx * p.ny < 128LL * 128LL) t = Tiles{32, 8, 0};
     t.tz = tz_for(p, t.tx, t.ty);
     return t;
@@ -156,6 +169,7 @@ template <typename R> static Geom make_geom(const gpe_ctx* c) {
     g.nz = c->p.nz;
     g.bcs = c->p.bcs;
     g.plane = (long long)c->p.nx * c->p.ny;
+    g.general = make_rhs_p<R>(c->p, nullptr).flags != 0;
     return g;
 }
 
@@ -190,20 +204,30 @@ static int ensure_partial(gpe_ctx* c, size_t ndoubles) {
     return 0;
 }
 
-template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const void* u, const Op& op) {
-    Geom g = make_geom<R>(c);
+// Algorithmic field bytes per cell of an operator: kAccess values of the
+// storage precision, or the operator's own kBytesPerCell (mixed storage).
+template <class Op, class = void> struct OpBytes {
+    static double get(size_t esz) { return (double)Op::kAccess * (double)esz; }
+};
+template <class Op> struct OpBytes<Op, std::void_t<decltype(Op::kBytesPerCell)>> {
+    static double get(size_t) { return (double)Op::kBytesPerCell; }
+};
+
+// Launches one stencil pass (launch(g, nb) enqueues the kernel), with the
+// optional per-launch event timing and the final reduction of its partials.
+template <class Op, class Launch> static int run_launch(gpe_ctx* c, Geom g, Launch launch) {
     const int nb = tile_geom(c->tiles, g);
     if (Op::kNRed > 0) {
         int rc = ensure_partial(c, (size_t)nb * Op::kNRed);
         if (rc) return rc;
     }
-    gpe_ctx::ProfRec rec{Op::kKind, (double)Op::kAccess * (double)c->esz * (double)c->n, nullptr, nullptr};
+    gpe_ctx::ProfRec rec{Op::kKind, OpBytes<Op>::get(c->esz) * (double)c->n, nullptr, nullptr};
     if (c->prof) {
         HIPCHK(c, hipEventCreate(&rec.a));
         HIPCHK(c, hipEventCreate(&rec.b));
         HIPCHK(c, hipEventRecord(rec.a, c->stream));
     }
-    HIPCHK(c, (launch_stencil<R, ORD, Op>(c->tiles, g, make_w<R>(c->p), (const C<R>*)u, op, c->partial, c->stream)));
+    HIPCHK(c, launch(g));
     if (c->prof) {
         HIPCHK(c, hipEventRecord(rec.b, c->stream));
         c->recs.push_back(rec);
@@ -212,6 +236,21 @@ template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const voi
         HIPCHK(c, launch_reduce_final(Op::kMode, c->partial, nb, Op::kNRed, c->dres, c->stream));
     }
     return 0;
+}
+
+template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const void* u, const Op& op) {
+    return run_launch<Op>(c, make_geom<R>(c), [&](Geom g) {
+        return launch_stencil<R, ORD, Op>(c->tiles, g, make_w<R>(c->p), (const C<R>*)u, op, c->partial, c->stream);
+    });
+}
+
+// Stencil over psi0 + s d (mixed storage).
+template <int ORD, class Op>
+static int run_op_sum(gpe_ctx* c, const void* a, const void* d, double2 s, const Op& op) {
+    const SumSrc src{(const double2*)a, (const float2*)d, s};
+    return run_launch<Op>(c, make_geom<double>(c), [&](Geom g) {
+        return launch_stencil_sum<ORD, Op>(c->tiles, g, make_w<double>(c->p), src, op, c->partial, c->stream);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -254,6 +293,12 @@ template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, dou
 static int need_rk_buffers(gpe_ctx* c, int scheme) {
     int rc;
     const size_t fb = field_bytes(c);
+    if (c->p.storage == GPE_MIXED && (scheme == GPE_RK4 || scheme == GPE_RK45)) {
+        const int nmx = scheme == GPE_RK4 ? 3 : 7;
+        for (int j = 0; j < nmx; ++j)
+            if ((rc = dalloc(c, &c->mx[j], (size_t)c->n * sizeof(float2), "mixed increment"))) return rc;
+        return dalloc(c, &c->pnew, fb, "pnew");
+    }
     if ((rc = dalloc(c, &c->uA, fb, "uA"))) return rc;
     if (scheme == GPE_EULER) return 0;
     if ((rc = dalloc(c, &c->uB, fb, "uB"))) return rc;
@@ -310,6 +355,61 @@ static int ck_attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0
     return 0;
 }
 
+// Classic RK4 with complex64 increments (GPE_MIXED): K1 = dt f(psi0),
+// K2 = dt f(psi0 + K1/2), K3 = dt f(psi0 + K2/2), then
+// psi_new = psi0 + (K1 + 2 K2 + 2 K3 + dt f(psi0 + K3))/6 into pnew (swapped
+// in).  144 field bytes per cell.
+template <int ORD> static int step_rk4_mixed(gpe_ctx* c, double dre, double dim) {
+    const RhsC<double> k = make_rhs<double>(c);
+    const CD dt = cdt<double>(dre, dim, 1.0);
+    CF* K1 = (CF*)c->mx[0];
+    CF* K2 = (CF*)c->mx[1];
+    CF* K3 = (CF*)c->mx[2];
+    int rc;
+    if ((rc = run_op<double, ORD>(c, c->psi, OpMxRk4S1{k, K1, dt}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, K1, mkc<double>(0.5, 0.0), OpMxRk4S23{k, K2, dt}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, K2, mkc<double>(0.5, 0.0), OpMxRk4S23{k, K3, dt}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, K3, mkc<double>(1.0, 0.0), OpMxRk4Last{k, K1, K2, (CD*)c->pnew, dt})))
+        return rc;
+    std::swap(c->psi, c->pnew);
+    return 0;
+}
+
+// One Cash-Karp attempt with complex64 increments (GPE_MIXED); same error
+// norm and outputs as ck_attempt.  F0 = f(psi0) is kept across retries.
+template <int ORD>
+static int ck_attempt_mixed(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    const RhsC<double> k = make_rhs<double>(c);
+    const CkTab<double> t = make_tab<double>();
+    const CD dt = cdt<double>(dre, dim, 1.0);
+    CF* F0 = (CF*)c->mx[0];
+    CF* K2 = (CF*)c->mx[1];
+    CF* K3 = (CF*)c->mx[2];
+    CF* K4 = (CF*)c->mx[3];
+    CF* K5 = (CF*)c->mx[4];
+    CF* DA = (CF*)c->mx[5];
+    CF* DB = (CF*)c->mx[6];
+    const double2 one = mkc<double>(1.0, 0.0);
+    int rc;
+    if (!c->f0_valid) {
+        if ((rc = run_op<double, ORD>(c, c->psi, OpMxCk1{k, F0}))) return rc;
+        c->f0_valid = true;
+    }
+    const double2 s21 = mkc<double>(t.b21 * dt.x, t.b21 * dt.y);
+    if ((rc = run_op_sum<ORD>(c, c->psi, F0, s21, OpMxCk2{k, K2, DA, dt, t}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, DA, one, OpMxCk3{k, F0, K2, nullptr, nullptr, K3, DB, dt, t}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, DB, one, OpMxCk4{k, F0, K2, K3, nullptr, K4, DA, dt, t}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, DA, one, OpMxCk5{k, F0, K2, K3, K4, K5, DB, dt, t}))) return rc;
+    if ((rc = run_op_sum<ORD>(c, c->psi, DB, one, OpMxCkLast{k, F0, K3, K4, K5, (CD*)c->pnew, dt, d0re, d0im, t})))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = c->hres[0] / c->p.eps;
+    out[1] = c->hres[1];
+    c->cand_valid = true;
+    return 0;
+}
+
 template <typename R> static int dispatch_fixed(gpe_ctx* c, int scheme, double dre, double dim) {
     return c->p.order == 2 ? step_fixed<R, 2>(c, scheme, dre, dim) : step_fixed<R, 4>(c, scheme, dre, dim);
 }
@@ -322,6 +422,9 @@ static int dispatch_attempt(gpe_ctx* c, double dre, double dim, double d0re, dou
 static int attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
     int rc = need_rk_buffers(c, GPE_RK45);
     if (rc) return rc;
+    if (c->p.storage == GPE_MIXED)
+        return c->p.order == 2 ? ck_attempt_mixed<2>(c, dre, dim, d0re, d0im, out)
+                               : ck_attempt_mixed<4>(c, dre, dim, d0re, d0im, out);
     return c->p.storage == GPE_C128 ? dispatch_attempt<double>(c, dre, dim, d0re, d0im, out)
                                     : dispatch_attempt<float>(c, dre, dim, d0re, d0im, out);
 }
@@ -413,7 +516,7 @@ static int upload_to(gpe_ctx* c, void* dst, const void* host, int host_dtype, in
     const size_t spitch = (size_t)c->p.nx * ny_alloc * he;
     HIPCHK(c, hipMemcpy2DAsync(target, width, src, spitch, width, c->p.nz, hipMemcpyHostToDevice, c->stream));
     if (!same) {
-        if (c->p.storage == GPE_C128) {
+        if (c->p.storage != GPE_C64) {
             HIPCHK(c, launch_from_c64<double>((const float2*)c->stage, (double2*)dst, c->n, c->stream));
         } else {
             HIPCHK(c, launch_from_c128<float>((const double2*)c->stage, (float2*)dst, c->n, c->stream));
@@ -431,7 +534,7 @@ static int download_from(gpe_ctx* c, const void* src, void* host, int host_dtype
     if (he != c->esz) {
         int rc = ensure_stage(c, (size_t)c->n * he);
         if (rc) return rc;
-        if (c->p.storage == GPE_C128) {
+        if (c->p.storage != GPE_C64) {
             HIPCHK(c, launch_to_c64<double>((const double2*)src, (float2*)c->stage, c->n, c->stream));
         } else {
             HIPCHK(c, launch_to_c128<float>((const float2*)src, (double2*)c->stage, c->n, c->stream));
@@ -492,7 +595,7 @@ static int deriv_common(gpe_ctx* c, int op, const void* in, int64_t ny_alloc, in
         if ((rc = upload_to(c, c->sin, in, host_dtype, ny_alloc, y_off, z_off))) return rc;
         u = c->sin;
     }
-    rc = c->p.storage == GPE_C128 ? deriv_impl<double>(c, op, u, c->sout, rhs) : deriv_impl<float>(c, op, u, c->sout, rhs);
+    rc = c->p.storage != GPE_C64 ? deriv_impl<double>(c, op, u, c->sout, rhs) : deriv_impl<float>(c, op, u, c->sout, rhs);
     if (rc) return rc;
     return download_from(c, c->sout, out, host_dtype);
 }
@@ -509,7 +612,7 @@ static int norm_mass(gpe_ctx* c, double* v, int weighted) {
     if (!v) return fail(c, GPE_EINVAL, "norm/mass: output is NULL");
     int rc = ensure_partial(c, kNormBlocks);
     if (rc) return rc;
-    if (c->p.storage == GPE_C128) {
+    if (c->p.storage != GPE_C64) {
         HIPCHK(c, launch_norm_partial<double>((const double2*)c->psi, c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
                                               kNormBlocks, c->stream));
     } else {
@@ -547,7 +650,7 @@ int gpe_ctx_create(const gpe_params* p, gpe_ctx** out) {
     gpe_ctx* c = new gpe_ctx();
     c->p = *p;
     c->n = (long long)p->nx * p->ny * p->nz;
-    c->esz = p->storage == GPE_C128 ? 16 : 8;
+    c->esz = p->storage == GPE_C64 ? 8 : 16;
     c->tiles = default_tiles(*p);
     int rc = 0;
     hipError_t e = hipGetDevice(&c->device);
@@ -564,7 +667,7 @@ int gpe_ctx_create(const gpe_params* p, gpe_ctx** out) {
         if (e != hipSuccess) rc = fail(c, GPE_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     }
     if (!rc && p->diss_amp != 0.0 && (p->eqn_to_solve == 1 || p->eqn_to_solve == 3))
-        rc = p->storage == GPE_C128 ? build_dtab<double>(c) : build_dtab<float>(c);
+        rc = p->storage != GPE_C64 ? build_dtab<double>(c) : build_dtab<float>(c);
     if (!rc) {
         e = hipMemsetAsync(c->psi, 0, field_bytes(c), c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -583,8 +686,9 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     prof_clear(c);
-    void* bufs[] = {c->psi, c->uA, c->uB, c->acc, c->f0, c->k2, c->k3, c->k4, c->k5, c->pnew,
-                    c->sin, c->sout, c->stage, c->partial, c->dres, c->dtab};
+    void* bufs[] = {c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
+                    c->pnew,  c->sin,  c->sout, c->stage, c->partial, c->dres, c->dtab, c->mx[0], c->mx[1],
+                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6]};
     for (void* b : bufs) dfree(c, b);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -640,8 +744,12 @@ int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
     }
     const int nrhs = scheme == GPE_EULER ? 1 : scheme == GPE_RK2 ? 2 : 4;
     for (int s = 0; s < nsteps; ++s) {
-        rc = c->p.storage == GPE_C128 ? dispatch_fixed<double>(c, scheme, io->dt_re, io->dt_im)
-                                      : dispatch_fixed<float>(c, scheme, io->dt_re, io->dt_im);
+        if (c->p.storage == GPE_MIXED && scheme == GPE_RK4)
+            rc = c->p.order == 2 ? step_rk4_mixed<2>(c, io->dt_re, io->dt_im)
+                                 : step_rk4_mixed<4>(c, io->dt_re, io->dt_im);
+        else
+            rc = c->p.storage != GPE_C64 ? dispatch_fixed<double>(c, scheme, io->dt_re, io->dt_im)
+                                         : dispatch_fixed<float>(c, scheme, io->dt_re, io->dt_im);
         if (rc) return rc;
         io->t += io->dt_re;             // t = t + real(dt)
         io->im_t += fabs(io->dt_im);    // im_t = im_t + abs(aimag(dt))
@@ -697,7 +805,7 @@ int gpe_energy(gpe_ctx* c, double* energy) {
     if (!c) return fail(nullptr, GPE_EINVAL, "gpe_energy: ctx is NULL");
     if (!energy) return fail(c, GPE_EINVAL, "gpe_energy: output is NULL");
     int rc;
-    if (c->p.storage == GPE_C128) {
+    if (c->p.storage != GPE_C64) {
         OpEnergy<double> o;
         rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
     } else {
@@ -717,7 +825,7 @@ int gpe_hamiltonian(gpe_ctx* c, double* h) {
     if (!c) return fail(nullptr, GPE_EINVAL, "gpe_hamiltonian: ctx is NULL");
     if (!h) return fail(c, GPE_EINVAL, "gpe_hamiltonian: output is NULL");
     int rc;
-    if (c->p.storage == GPE_C128) {
+    if (c->p.storage != GPE_C64) {
         RhsC<double> k = make_rhs<double>(c);
         OpHamiltonian<double> o{k, k.pim >= 0 ? 1.0 : -1.0};
         rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
@@ -739,7 +847,7 @@ int gpe_renormalise(gpe_ctx* c, double norm) {
     if (!c) return fail(nullptr, GPE_EINVAL, "gpe_renormalise: ctx is NULL");
     if (!(norm > 0.0) || !isfinite(norm)) return fail(c, GPE_EINVAL, "gpe_renormalise: norm must be positive");
     const double snn = sqrt(c->p.nn), sn = sqrt(norm);
-    if (c->p.storage == GPE_C128) {
+    if (c->p.storage != GPE_C64) {
         HIPCHK(c, launch_scale<double>((double2*)c->psi, c->n, snn, sn, c->stream));
     } else {
         HIPCHK(c, launch_scale<float>((float2*)c->psi, c->n, snn, sn, c->stream));

@@ -109,6 +109,7 @@ struct Geom {
     int bcs;
     int ntx, nty, tz, nblocks;
     long long plane;  // nx*ny
+    int general;      // 1: the right-hand side has optional terms (RhsC::flags != 0)
 };
 
 // Stencil weights.  2nd order: d1 = (f+1 - f-1) w1, d2 = (f+1 - 2f + f-1) w2
@@ -140,8 +141,12 @@ template <typename R> struct RhsC {
 
 // Accessor for the stencil of one cell: LDS tile of the current plane plus
 // the register z-window.
-template <typename R, int ORD, int LW> struct Sten {
+// GEN = false compiles the right-hand side without its optional terms
+// (RhsC::flags == 0), which keeps their constants and table addresses out of
+// the registers of the common kernels.
+template <typename R, int ORD, int LW, bool GEN = true> struct Sten {
     static constexpr int H = ORD / 2;
+    static constexpr bool kGeneral = GEN;
     const C<R>* L;
     int lc;
     C<R> q[2 * H + 1];
@@ -205,25 +210,30 @@ __device__ __forceinline__ C<R> rhs_eval(const RhsC<R>& k, const S& st, int x, i
     const C<R> p = st.c();
     const C<R> L = st.lap();
     R s = k.b + k.c * (p.x * p.x + p.y * p.y);
-    if (k.flags & RHS_V) {
-        const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;
-        s += k.d * (k.vx * (xc * xc) + k.vy * (yc * yc) + k.vz * (zc * zc));
+    if constexpr (!S::kGeneral) {
+        const C<R> w = mkc<R>(k.a * L.x + s * p.x, k.a * L.y + s * p.y);
+        return mkc<R>(-k.pim * w.y, k.pim * w.x);
+    } else {
+        if (k.flags & RHS_V) {
+            const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;
+            s += k.d * (k.vx * (xc * xc) + k.vy * (yc * yc) + k.vz * (zc * zc));
+        }
+        const C<R> w = mkc<R>(k.a * L.x + s * p.x, k.a * L.y + s * p.y);
+        C<R> f = mkc<R>(-k.pim * w.y, k.pim * w.x);
+        if (k.flags & RHS_DISS) {
+            // diss = diss_amp*(tx(x) + ty(y) + tz(z)); (0.5*(i + diss)) * w adds 0.5*diss*w
+            const R diss = k.diss_amp * ((k.dtab[x] + k.dtab[k.nx + y]) + k.dtab[k.nx + k.ny + z]);
+            f = caxpy<R>(R(0.5) * diss, w, f);
+        }
+        if (k.flags & RHS_U) f = caxpy<R>(k.U, st.d1x(), f);
+        if (k.flags & RHS_MASK) {
+            // sphere() (src/ic.f90:635-657): 0.5*(1 + tanh(r - 10 - 2))
+            const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;
+            const R m = R(0.5) * (R(1) + tanh(sqrt(xc * xc + yc * yc + zc * zc) - R(12)));
+            f = cscale<R>(m, f);
+        }
+        return f;
     }
-    const C<R> w = mkc<R>(k.a * L.x + s * p.x, k.a * L.y + s * p.y);
-    C<R> f = mkc<R>(-k.pim * w.y, k.pim * w.x);
-    if (k.flags & RHS_DISS) {
-        // diss = diss_amp*(tx(x) + ty(y) + tz(z)); (0.5*(i + diss)) * w adds 0.5*diss*w
-        const R diss = k.diss_amp * ((k.dtab[x] + k.dtab[k.nx + y]) + k.dtab[k.nx + k.ny + z]);
-        f = caxpy<R>(R(0.5) * diss, w, f);
-    }
-    if (k.flags & RHS_U) f = caxpy<R>(k.U, st.d1x(), f);
-    if (k.flags & RHS_MASK) {
-        // sphere() (src/ic.f90:635-657): 0.5*(1 + tanh(r - 10 - 2))
-        const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;
-        const R m = R(0.5) * (R(1) + tanh(sqrt(xc * xc + yc * yc + zc * zc) - R(12)));
-        f = cscale<R>(m, f);
-    }
-    return f;
 }
 
 }  // namespace gpe

@@ -41,7 +41,7 @@ template <typename R> struct EnsMember {
 
 // One RK4 stage for every member.  STAGE 1..4 selects the operator and the
 // fields exactly as step_fixed does for one context.
-template <typename R, int ORD, int TX, int TY, int STAGE>
+template <typename R, int ORD, int TX, int TY, int STAGE, bool GEN>
 __global__ __launch_bounds__(TX* TY) GPE_OCC void k_rk4_batched(const EnsMember<R>* __restrict__ mem, int nmem,
                                                                  int total, C<R> a1, C<R> a2) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -55,41 +55,49 @@ __global__ __launch_bounds__(TX* TY) GPE_OCC void k_rk4_batched(const EnsMember<
         const int local = item - M.begin;
         if constexpr (STAGE == 1) {
             const OpRk4First<R> op{M.k, M.ua, a1};
-            stencil_body<R, ORD, TX, TY>(M.g, M.w, M.psi, op, nullptr, local, 0);
+            stencil_body<R, ORD, TX, TY, GEN>(M.g, M.w, M.psi, op, nullptr, local, 0);
         } else if constexpr (STAGE == 2) {
             const OpRk4Second<R> op{M.k, M.psi, M.ub, a1};
-            stencil_body<R, ORD, TX, TY>(M.g, M.w, M.ua, op, nullptr, local, 0);
+            stencil_body<R, ORD, TX, TY, GEN>(M.g, M.w, M.ua, op, nullptr, local, 0);
         } else if constexpr (STAGE == 3) {
             const OpRk4Third<R> op{M.k, M.psi, M.ua, M.acc, a1, a2};
-            stencil_body<R, ORD, TX, TY>(M.g, M.w, M.ub, op, nullptr, local, 0);
+            stencil_body<R, ORD, TX, TY, GEN>(M.g, M.w, M.ub, op, nullptr, local, 0);
         } else {
             const OpRk4Last<R> op{M.k, M.acc, M.psi, a1};
-            stencil_body<R, ORD, TX, TY>(M.g, M.w, M.ua, op, nullptr, local, 0);
+            stencil_body<R, ORD, TX, TY, GEN>(M.g, M.w, M.ua, op, nullptr, local, 0);
         }
         __syncthreads();  // the next item reuses the LDS tile
     }
 }
 
-template <typename R, int ORD, int TX, int TY>
-static hipError_t launch_batched_tile(int stage, const EnsMember<R>* mem, int nmem, int total, int grid, C<R> a1,
-                                      C<R> a2, hipStream_t s) {
+template <typename R, int ORD, int TX, int TY, bool GEN>
+static void launch_batched_stage(int stage, const EnsMember<R>* mem, int nmem, int total, int grid, C<R> a1, C<R> a2,
+                                 hipStream_t s) {
     switch (stage) {
-        case 1: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 1>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
-        case 2: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 2>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
-        case 3: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 3>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
-        default: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 4>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
+        case 1: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 1, GEN>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
+        case 2: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 2, GEN>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
+        case 3: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 3, GEN>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
+        default: hipLaunchKernelGGL((k_rk4_batched<R, ORD, TX, TY, 4, GEN>), dim3(grid), dim3(TX * TY), 0, s, mem, nmem, total, a1, a2); break;
     }
+}
+
+// general: some member's right-hand side has optional terms (RhsC::flags != 0)
+template <typename R, int ORD, int TX, int TY>
+static hipError_t launch_batched_tile(int stage, bool general, const EnsMember<R>* mem, int nmem, int total, int grid,
+                                      C<R> a1, C<R> a2, hipStream_t s) {
+    if (general) launch_batched_stage<R, ORD, TX, TY, true>(stage, mem, nmem, total, grid, a1, a2, s);
+    else launch_batched_stage<R, ORD, TX, TY, false>(stage, mem, nmem, total, grid, a1, a2, s);
     return hipGetLastError();
 }
 
 template <typename R, int ORD>
-static hipError_t launch_batched(int tx, int ty, int stage, const EnsMember<R>* mem, int nmem, int total, int grid,
-                                 C<R> a1, C<R> a2, hipStream_t s) {
+static hipError_t launch_batched(int tx, int ty, int stage, bool general, const EnsMember<R>* mem, int nmem, int total,
+                                 int grid, C<R> a1, C<R> a2, hipStream_t s) {
     switch (tx * 100 + ty) {
-        case 3208: return launch_batched_tile<R, ORD, 32, 8>(stage, mem, nmem, total, grid, a1, a2, s);
-        case 3204: return launch_batched_tile<R, ORD, 32, 4>(stage, mem, nmem, total, grid, a1, a2, s);
-        case 6404: return launch_batched_tile<R, ORD, 64, 4>(stage, mem, nmem, total, grid, a1, a2, s);
-        case 6408: return launch_batched_tile<R, ORD, 64, 8>(stage, mem, nmem, total, grid, a1, a2, s);
+        case 3208: return launch_batched_tile<R, ORD, 32, 8>(stage, general, mem, nmem, total, grid, a1, a2, s);
+        case 3204: return launch_batched_tile<R, ORD, 32, 4>(stage, general, mem, nmem, total, grid, a1, a2, s);
+        case 6404: return launch_batched_tile<R, ORD, 64, 4>(stage, general, mem, nmem, total, grid, a1, a2, s);
+        case 6408: return launch_batched_tile<R, ORD, 64, 8>(stage, general, mem, nmem, total, grid, a1, a2, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -115,6 +123,7 @@ struct gpe_ens {
     std::vector<Group> groups;
     void* gdesc = nullptr;                // descriptors in group order, begin relative to the group
     bool desc_dirty = true;
+    bool general = false;  // some member's right-hand side has optional terms
     // graph of `gsteps` batched steps with dt (gre, gim)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -161,6 +170,7 @@ int default_tz(const gpe_params& p, int tx, int ty) {
 template <typename R> int build_desc(gpe_ens* e) {
     std::vector<EnsMember<R>> h(e->n);
     int begin = 0;
+    e->general = false;
     for (int m = 0; m < e->n; ++m) {
         const gpe_params& p = e->mp[m];
         EnsMember<R>& d = h[m];
@@ -173,6 +183,8 @@ template <typename R> int build_desc(gpe_ens* e) {
         tile_geom(Tiles{e->tx, e->ty, e->tz[m]}, d.g);
         d.w = make_w<R>(p);
         d.k = make_rhs_p<R>(p, nullptr);
+        d.g.general = d.k.flags != 0;
+        e->general = e->general || d.g.general;
         d.psi = (C<R>*)e->psi[m];
         d.ua = (C<R>*)e->ua[m];
         d.ub = (C<R>*)e->ub[m];
@@ -228,8 +240,10 @@ int step_launches(gpe_ens* e, const EnsMember<R>* mem, int nmem, int total, int 
     struct St { int stage; double s1, s2; } st[4] = {{1, 0.5, 0.0}, {2, 0.5, 0.0}, {3, 1.0, 1.0 / 3.0}, {4, 1.0 / 6.0, 0.0}};
     for (auto& s : st) {
         const C<R> a1 = cdt<R>(dre, dim, s.s1), a2 = cdt<R>(dre, dim, s.s2);
-        hipError_t x = ord == 2 ? launch_batched<R, 2>(e->tx, e->ty, s.stage, mem, nmem, total, grid, a1, a2, e->stream)
-                                : launch_batched<R, 4>(e->tx, e->ty, s.stage, mem, nmem, total, grid, a1, a2, e->stream);
+        hipError_t x = ord == 2 ? launch_batched<R, 2>(e->tx, e->ty, s.stage, e->general, mem, nmem, total, grid, a1, a2,
+                                                       e->stream)
+                                : launch_batched<R, 4>(e->tx, e->ty, s.stage, e->general, mem, nmem, total, grid, a1, a2,
+                                                       e->stream);
         EHIP(e, x);
     }
     return 0;
@@ -260,6 +274,7 @@ template <typename R, int ORD> int step_member(gpe_ens* e, int m, double dre, do
     g.nz = p.nz;
     g.bcs = p.bcs;
     g.plane = (long long)p.nx * p.ny;
+    g.general = k.flags != 0;
     const Tiles t{e->tx, e->ty, e->tz[m]};
     C<R>* psi = (C<R>*)e->psi[m];
     C<R>* ua = (C<R>*)e->ua[m];
@@ -354,8 +369,9 @@ int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out) {
         if (p.eqn_to_solve < 1 || p.eqn_to_solve > 4)
             return efail(nullptr, GPE_EINVAL, "gpe_ens_create: eqn_to_solve must be 1..4");
         if (p.storage != GPE_C128 && p.storage != GPE_C64)
-            return efail(nullptr, GPE_EINVAL, "gpe_ens_create: bad storage");
+            return efail(nullptr, GPE_EINVAL, "gpe_ens_create: storage must be GPE_C128 or GPE_C64");
         if (p.nx < 6 || p.ny < 6 || p.nz < 6) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: nx, ny, nz must be >= 6");
+        if ((long long)p.nx * p.ny > (1LL << 28)) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: nx*ny must be <= 2^28");
         if (!(p.xr > 0) || !(p.yr > 0) || !(p.zr > 0)) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: xr, yr, zr must be > 0");
         if (p.diss_amp != 0.0) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: diss_amp is not supported in an ensemble");
     }

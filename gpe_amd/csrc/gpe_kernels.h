@@ -1,6 +1,8 @@
 // The z-marching stencil kernel and the fused stage operators.
 #pragma once
 
+#include <type_traits>
+
 #include "gpe_device.h"
 
 namespace gpe {
@@ -31,8 +33,8 @@ template <int NT, int MODE> __device__ __forceinline__ double block_reduce(doubl
 // ---------------------------------------------------------------------------
 // The stencil kernel.  Op supplies:
 //   kNRed (0..2 reduction values), kMode (RED_SUM / RED_MAX),
-//   struct Pre; Pre load(long long i) const   -- pointwise inputs of cell i
-//   void apply(long long i, int x, int y, int z, const Sten& st, const Pre& p, double* red) const
+//   struct Pre; Pre load(Cell i) const   -- pointwise inputs of cell i
+//   void apply(Cell i, int x, int y, int z, const Sten& st, const Pre& p, double* red) const
 // GPE_WAVES_PER_EU > 0 asks the compiler for at least that many waves per
 // SIMD (caps VGPRs); GPE_PW_PREFETCH = 1 prefetches the pointwise inputs of
 // the next plane (double-buffered registers), 0 loads them at the top of the
@@ -49,11 +51,78 @@ template <int NT, int MODE> __device__ __forceinline__ double block_reduce(doubl
 #define GPE_OCC
 #endif
 
+// Index of a cell as a plane base (uniform across the wavefront, kept in
+// scalar registers) plus the 32-bit offset within the plane.  `ptr + cell`
+// forms the address as (ptr + zb) + own * sizeof(T), a 64-bit scalar base and
+// a 32-bit per-lane byte offset, so a kernel keeps one VGPR per distinct
+// element size instead of a 64-bit address per field.  Requires
+// nx * ny * 16 < 2^32 (checked at context creation).
+struct Cell {
+    long long zb;
+    unsigned own;
+};
+// readfirstlane pins the (uniform) plane offset in scalar registers and
+// hides it from loop strength reduction, which would otherwise turn every
+// field into a 64-bit per-lane pointer stepped by one plane per iteration.
+// The pointer itself is only offset (never rebuilt from an integer), so the
+// compiler keeps the global address space: global_load with a scalar base
+// and a 32-bit vector offset.
+__device__ __forceinline__ long long uniform_i64(long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long long)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T> __device__ __forceinline__ T* operator+(T* p, Cell c) {
+    return (T*)((char*)(p + uniform_i64(c.zb)) + c.own * (unsigned)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ const T* operator+(const T* p, Cell c) {
+    return (const T*)((const char*)(p + uniform_i64(c.zb)) + c.own * (unsigned)sizeof(T));
+}
+
+// Sources of the stencil input.  PlainSrc reads one field.  SumSrc forms
+// u = a + s d from an FP64 field a and a complex64 increment d scaled by the
+// FP64 complex s, so a stage can stencil psi0 + (increment) without the sum
+// ever being stored; it also hands the operator the raw d of the thread's own
+// cell (Aux), which the increments-only stages need.
+struct NoAux {};
+template <typename R> struct PlainSrc {
+    using Aux = NoAux;
+    const C<R>* __restrict__ u;
+    __device__ __forceinline__ C<R> operator()(Cell i) const { return *(u + i); }
+    __device__ __forceinline__ C<R> operator()(Cell i, NoAux&) const { return *(u + i); }
+};
+struct SumSrc {
+    using Aux = float2;
+    const double2* __restrict__ a;
+    const float2* __restrict__ d;
+    double2 s;
+    __device__ __forceinline__ double2 comb(double2 p, float2 v) const {
+        const double vx = v.x, vy = v.y;
+        return mkc<double>(p.x + (s.x * vx - s.y * vy), p.y + (s.x * vy + s.y * vx));
+    }
+    __device__ __forceinline__ double2 operator()(Cell i) const { return comb(*(a + i), *(d + i)); }
+    __device__ __forceinline__ double2 operator()(Cell i, float2& dv) const {
+        dv = *(d + i);
+        return comb(*(a + i), dv);
+    }
+};
+
+// An operator may set kPrefetch = false: its pointwise inputs are then
+// loaded at the top of each plane instead of one plane ahead, which frees the
+// second register set where that set would spill (spill reloads wait for
+// every outstanding load, the plane prefetch included).
+template <class Op, class = void> struct OpPrefetch : std::true_type {};
+template <class Op> struct OpPrefetch<Op, std::void_t<decltype(Op::kPrefetch)>> : std::bool_constant<Op::kPrefetch> {};
+template <class Op> constexpr bool op_prefetch() { return OpPrefetch<Op>::value; }
+
 // Body of the stencil kernel for the tile `bid` of grid g; the block's
 // reductions (if any) go to partial[r * g.nblocks + pidx].
-template <typename R, int ORD, int TX, int TY, class Op>
-__device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, const C<R>* __restrict__ u, const Op& op,
-                                             double* __restrict__ partial, int bid, int pidx) {
+template <typename R, int ORD, int TX, int TY, class Op, class Src, bool GEN>
+__device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w, const Src& src, const Op& op,
+                                                 double* __restrict__ partial, int bid, int pidx) {
+    using Aux = typename Src::Aux;
+    constexpr bool kAux = !std::is_same<Aux, NoAux>::value;
+    constexpr bool PF = GPE_PW_PREFETCH && op_prefetch<Op>();
     constexpr int H = ORD / 2, NT = TX * TY, LW = TX + 2 * H, LH = TY + 2 * H;
     constexpr int NHALO = 2 * H * TX + 2 * H * TY;
     constexpr int NSLOT = (NHALO + NT - 1) / NT;
@@ -69,13 +138,13 @@ __device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, con
     const int lx = tid % TX, ly = tid / TX;
     const int x = x0 + lx, y = y0 + ly;
     const bool valid = (x < g.nx) && (y < g.ny);
-    const long long col = (long long)bcmap(x, g.nx, g.bcs) + (long long)g.nx * bcmap(y, g.ny, g.bcs);
-    const long long own = (long long)x + (long long)g.nx * y;  // cell offset within a plane (valid threads)
+    const unsigned col = (unsigned)bcmap(x, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(y, g.ny, g.bcs);
+    const unsigned own = (unsigned)x + (unsigned)g.nx * (unsigned)y;  // cell offset within a plane (valid threads)
     const int lc = (ly + H) * LW + lx + H;
 
     // Halo slots: first the 2H rows above/below the tile (coalesced rows of
     // TX), then the 2H columns left/right of each tile row.
-    long long hoff[NSLOT];
+    unsigned hoff[NSLOT];
     int hl[NSLOT];
     bool hv[NSLOT];
 #pragma unroll
@@ -98,28 +167,33 @@ __device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, con
             gy = y0 + r;
         }
         hl[s] = lyy * LW + lxx;
-        hoff[s] = (long long)bcmap(gx, g.nx, g.bcs) + (long long)g.nx * bcmap(gy, g.ny, g.bcs);
+        hoff[s] = (unsigned)bcmap(gx, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(gy, g.ny, g.bcs);
     }
 
-    // Register z-window q[0..2H] = planes z-H..z+H, qn = plane z+H+1 (prefetch).
+    // Register z-window q[0..2H] = planes z-H..z+H, qn = plane z+H+1
+    // (prefetch); qa/qan the matching Aux of the own column (only qa[H] is
+    // read, the rest is dead and removed when Aux is empty).
     C<R> q[2 * H + 1];
+    Aux qa[2 * H + 1];
 #pragma unroll
-    for (int d = 0; d < 2 * H; ++d) q[d] = u[g.plane * bcmap(z0 - H + d, g.nz, g.bcs) + col];
+    for (int d = 0; d < 2 * H; ++d) q[d] = src(Cell{g.plane * bcmap(z0 - H + d, g.nz, g.bcs), col}, qa[d]);
     C<R> qn = mkc<R>(R(0), R(0));
+    Aux qan{};
     C<R> hr[NSLOT];
     typename Op::Pre pc{};
     if (z0 < z1) {
-        qn = u[g.plane * bcmap(z0 + H, g.nz, g.bcs) + col];
+        qn = src(Cell{g.plane * bcmap(z0 + H, g.nz, g.bcs), col}, qan);
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (hv[s]) hr[s] = u[g.plane * z0 + hoff[s]];
-        if (GPE_PW_PREFETCH && valid) pc = op.load(g.plane * z0 + own);
+            if (hv[s]) hr[s] = src(Cell{g.plane * z0, hoff[s]});
+        if (PF && valid) pc = op.load(Cell{g.plane * z0, own});
     }
 
     double red[2] = {0.0, 0.0};
     for (int z = z0; z < z1; ++z) {
         q[2 * H] = qn;
-        if (!GPE_PW_PREFETCH && valid) pc = op.load(g.plane * z + own);
+        qa[2 * H] = qan;
+        if (!PF && valid) pc = op.load(Cell{g.plane * z, own});
         C<R>* L = lds[(z - z0) & 1];
         L[lc] = q[H];
 #pragma unroll
@@ -128,24 +202,28 @@ __device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, con
         __syncthreads();
         typename Op::Pre pn{};
         if (z + 1 < z1) {
-            qn = u[g.plane * bcmap(z + 1 + H, g.nz, g.bcs) + col];
+            qn = src(Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), col}, qan);
 #pragma unroll
             for (int s = 0; s < NSLOT; ++s)
-                if (hv[s]) hr[s] = u[g.plane * (z + 1) + hoff[s]];
-            if (GPE_PW_PREFETCH && valid) pn = op.load(g.plane * (z + 1) + own);
+                if (hv[s]) hr[s] = src(Cell{g.plane * (z + 1), hoff[s]});
+            if (PF && valid) pn = op.load(Cell{g.plane * (z + 1), own});
         }
         if (valid) {
-            Sten<R, ORD, LW> st;
+            Sten<R, ORD, LW, GEN> st;
             st.L = L;
             st.lc = lc;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) st.q[d] = q[d];
             st.w = w;
-            op.apply(g.plane * z + own, x, y, z, st, pc, red);
+            if constexpr (kAux) op.apply(Cell{g.plane * z, own}, x, y, z, st, pc, qa[H], red);
+            else op.apply(Cell{g.plane * z, own}, x, y, z, st, pc, red);
         }
-        if (GPE_PW_PREFETCH) pc = pn;
+        if (PF) pc = pn;
 #pragma unroll
-        for (int d = 0; d < 2 * H; ++d) q[d] = q[d + 1];
+        for (int d = 0; d < 2 * H; ++d) {
+            q[d] = q[d + 1];
+            qa[d] = qa[d + 1];
+        }
     }
 
     if constexpr (Op::kNRed > 0) {
@@ -157,10 +235,25 @@ __device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, con
     }
 }
 
-template <typename R, int ORD, int TX, int TY, class Op>
+template <typename R, int ORD, int TX, int TY, bool GEN, class Op>
+__device__ __forceinline__ void stencil_body(const Geom& g, const StW<R>& w, const C<R>* __restrict__ u, const Op& op,
+                                             double* __restrict__ partial, int bid, int pidx) {
+    stencil_body_src<R, ORD, TX, TY, Op, PlainSrc<R>, GEN>(g, w, PlainSrc<R>{u}, op, partial, bid, pidx);
+}
+
+template <typename R, int ORD, int TX, int TY, class Op, bool GEN>
 __global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil(Geom g, StW<R> w, const C<R>* __restrict__ u, Op op,
                                                              double* __restrict__ partial) {
-    stencil_body<R, ORD, TX, TY, Op>(g, w, u, op, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
+    stencil_body<R, ORD, TX, TY, GEN, Op>(g, w, u, op, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
+}
+
+// Stencil of a + s d (SumSrc): FP64 arithmetic over an FP64 state and a
+// complex64 increment.
+template <int ORD, int TX, int TY, class Op, bool GEN>
+__global__ __launch_bounds__(TX* TY) GPE_OCC void k_stencil_sum(Geom g, StW<double> w, SumSrc src, Op op,
+                                                                 double* __restrict__ partial) {
+    stencil_body_src<double, ORD, TX, TY, Op, SumSrc, GEN>(g, w, src, op, partial, xcd_remap(blockIdx.x, g.nblocks),
+                                                            blockIdx.x);
 }
 
 // Final reduction of per-block partials (one block, fixed order).
@@ -189,8 +282,8 @@ template <typename R> struct OpAxpySelf {
     RhsC<R> k;
     C<R>* __restrict__ out;
     C<R> a;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre&, double*) const {
         st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), st.c()));
     }
 };
@@ -204,8 +297,8 @@ template <typename R> struct OpAxpyP0 {
     const C<R>* __restrict__ p0;
     C<R>* __restrict__ out;
     C<R> a;
-    __device__ Pre load(long long i) const { return {ld_stream<R>(p0 + i)}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+    __device__ Pre load(Cell i) const { return {ld_stream<R>(p0 + i)}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.p0));
     }
 };
@@ -229,8 +322,8 @@ template <typename R> struct OpRk4First {
     RhsC<R> k;
     C<R>* __restrict__ unext;
     C<R> a1;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre&, double*) const {
         st_stream<R>(unext + i, cfma<R>(a1, rhs_eval<R>(k, st, x, y, z), st.c()));
     }
 };
@@ -244,8 +337,8 @@ template <typename R> struct OpRk4Second {
     const C<R>* __restrict__ p0;
     C<R>* __restrict__ unext;
     C<R> a1;
-    __device__ Pre load(long long i) const { return {ld_stream<R>(p0 + i)}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+    __device__ Pre load(Cell i) const { return {ld_stream<R>(p0 + i)}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         st_stream<R>(unext + i, cfma<R>(a1, rhs_eval<R>(k, st, x, y, z), p.p0));
     }
 };
@@ -261,8 +354,8 @@ template <typename R> struct OpRk4Third {
     C<R>* u13;
     C<R>* __restrict__ acc;
     C<R> a1, a3;  // dt, dt/3
-    __device__ Pre load(long long i) const { return {ld_stream<R>(p0 + i), ld_stream<R>(u13 + i)}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+    __device__ Pre load(Cell i) const { return {ld_stream<R>(p0 + i), ld_stream<R>(u13 + i)}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         const C<R> f = rhs_eval<R>(k, st, x, y, z);
         const C<R> u2 = st.c();
         const R third = R(1) / R(3);
@@ -281,8 +374,8 @@ template <typename R> struct OpRk4Last {
     const C<R>* __restrict__ acc;
     C<R>* __restrict__ out;
     C<R> a;
-    __device__ Pre load(long long i) const { return {ld_stream<R>(acc + i)}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+    __device__ Pre load(Cell i) const { return {ld_stream<R>(acc + i)}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         st_stream<R>(out + i, cfma<R>(a, rhs_eval<R>(k, st, x, y, z), p.acc));
     }
 };
@@ -304,8 +397,8 @@ template <typename R> struct OpCk1 {
     C<R>* __restrict__ unext;
     C<R> dt;
     R b21;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre&, double*) const {
         const C<R> f = rhs_eval<R>(k, st, x, y, z);
         st_stream<R>(f0 + i, f);
         const C<R> k1 = cmul<R>(dt, f);
@@ -329,7 +422,7 @@ template <typename R, int S_> struct OpCkMid {
     C<R>* __restrict__ unext;
     C<R> dt;
     CkTab<R> t;
-    __device__ Pre load(long long i) const {
+    __device__ Pre load(Cell i) const {
         Pre p;
         p.p0 = ld_stream<R>(p0 + i);
         p.f0 = ld_stream<R>(f0 + i);
@@ -338,7 +431,7 @@ template <typename R, int S_> struct OpCkMid {
         if constexpr (S_ >= 5) p.k4 = ld_stream<R>(k4 + i);
         return p;
     }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double*) const {
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double*) const {
         const C<R> ks = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
         const C<R> k1 = cmul<R>(dt, p.f0);
         st_stream<R>(kout + i, ks);
@@ -375,11 +468,11 @@ template <typename R> struct OpCkLast {
     C<R> dt;
     double dt0_re, dt0_im;
     CkTab<R> t;
-    __device__ Pre load(long long i) const {
+    __device__ Pre load(Cell i) const {
         return {ld_stream<R>(p0 + i), ld_stream<R>(f0 + i), ld_stream<R>(k3 + i), ld_stream<R>(k4 + i),
                 ld_stream<R>(k5 + i)};
     }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre& p, double* red) const {
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre& p, double* red) const {
         const C<R> k6 = cmul<R>(dt, rhs_eval<R>(k, st, x, y, z));
         const C<R> k1 = cmul<R>(dt, p.f0);
         const C<R> nw = caxpy<R>(t.c6, k6, caxpy<R>(t.c4, p.k4, caxpy<R>(t.c3, p.k3, caxpy<R>(t.c1, k1, p.p0))));
@@ -405,8 +498,8 @@ template <typename R> struct OpDeriv {
     using Pre = NoPre;
     C<R>* __restrict__ out;
     int op;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long i, int, int, int, const S& st, const Pre&, double*) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int, int, int, const S& st, const Pre&, double*) const {
         C<R> v;
         switch (op) {
             case 0: v = st.d1x(); break;
@@ -428,8 +521,8 @@ template <typename R> struct OpRhs {
     using Pre = NoPre;
     RhsC<R> k;
     C<R>* __restrict__ out;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long i, int x, int y, int z, const S& st, const Pre&, double*) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int x, int y, int z, const S& st, const Pre&, double*) const {
         st_stream<R>(out + i, rhs_eval<R>(k, st, x, y, z));
     }
 };
@@ -440,8 +533,8 @@ template <typename R> struct OpEnergy {
     static constexpr int kKind = 13, kAccess = 1;
     static constexpr int kNRed = 1, kMode = RED_SUM;
     using Pre = NoPre;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long, int, int, int, const S& st, const Pre&, double* red) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell, int, int, int, const S& st, const Pre&, double* red) const {
         const C<R> L = st.lap();
         const C<R> p = st.c();
         const double pr = p.x, pi = p.y;
@@ -462,8 +555,8 @@ template <typename R> struct OpHamiltonian {
     using Pre = NoPre;
     RhsC<R> k;
     double sgn;
-    __device__ Pre load(long long) const { return {}; }
-    template <class S> __device__ void apply(long long, int x, int y, int z, const S& st, const Pre&, double* red) const {
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell, int x, int y, int z, const S& st, const Pre&, double* red) const {
         const C<R> L = st.lap();
         const C<R> p = st.c();
         const double pr = p.x, pi = p.y;

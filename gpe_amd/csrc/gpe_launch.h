@@ -4,7 +4,7 @@
 // compile in parallel.
 #pragma once
 
-#include "gpe_kernels.h"
+#include "gpe_mixed.h"
 
 namespace gpe {
 
@@ -54,12 +54,29 @@ hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u
     X(R, OpEnergy<R>)                                                                                              \
     X(R, OpHamiltonian<R>)
 
+// Mixed-storage operators: those over a plain complex128 stencil use
+// launch_stencil<double>, the others launch_stencil_sum.
+template <int ORD, class Op>
+hipError_t launch_stencil_sum(const Tiles& t, Geom g, const StW<double>& w, const SumSrc& src, const Op& op,
+                              double* partial, hipStream_t s);
+using OpMxCk3 = OpMxCkMid<3>;
+using OpMxCk4 = OpMxCkMid<4>;
+using OpMxCk5 = OpMxCkMid<5>;
+#define GPE_FOR_EACH_MIXED_PLAIN(X) X(double, OpMxRk4S1) X(double, OpMxCk1)
+#define GPE_FOR_EACH_MIXED_SUM(X) X(OpMxRk4S23) X(OpMxRk4Last) X(OpMxCk2) X(OpMxCk3) X(OpMxCk4) X(OpMxCk5) X(OpMxCkLast)
+
 #define GPE_DECLARE_EXTERN_ORD(R, OP, ORD) \
     extern template hipError_t launch_stencil<R, ORD, OP>(const Tiles&, Geom, const StW<R>&, const C<R>*, const OP&, \
                                                           double*, hipStream_t);
 #define GPE_DECLARE_EXTERN(R, OP) GPE_DECLARE_EXTERN_ORD(R, OP, 2) GPE_DECLARE_EXTERN_ORD(R, OP, 4)
 GPE_FOR_EACH_OP(GPE_DECLARE_EXTERN, double)
 GPE_FOR_EACH_OP(GPE_DECLARE_EXTERN, float)
+GPE_FOR_EACH_MIXED_PLAIN(GPE_DECLARE_EXTERN)
+#define GPE_DECLARE_EXTERN_SUM_ORD(OP, ORD)                                                                        \
+    extern template hipError_t launch_stencil_sum<ORD, OP>(const Tiles&, Geom, const StW<double>&, const SumSrc&, \
+                                                           const OP&, double*, hipStream_t);
+#define GPE_DECLARE_EXTERN_SUM(OP) GPE_DECLARE_EXTERN_SUM_ORD(OP, 2) GPE_DECLARE_EXTERN_SUM_ORD(OP, 4)
+GPE_FOR_EACH_MIXED_SUM(GPE_DECLARE_EXTERN_SUM)
 
 // Pointwise kernels (gpe_pointwise.hip).
 template <typename R>

@@ -13,7 +13,7 @@ module gpe_context
   type(c_ptr), save :: ctx = c_null_ptr
   ! device options with no counterpart in the reference's module parameters;
   ! set them before the first call that creates the context
-  character(4), save :: gpe_storage = 'c128'   ! 'c128' or 'c64' device fields
+  character(5), save :: gpe_storage = 'c128'   ! 'c128', 'c64' or 'mixed' device fields
   real(c_double), save :: gpe_vscale = 0.0_c_double ! V term in eqn 1 (0 = reference)
 
 contains
@@ -25,7 +25,14 @@ contains
       gp%nx = nx; gp%ny = ny; gp%nz = nz
       gp%order = order; gp%bcs = bcs; gp%eqn_to_solve = eqn_to_solve
       gp%real_time = merge(1, 0, real_time)
-      gp%storage = merge(GPE_C64, GPE_C128, gpe_storage == 'c64')
+      select case (trim(gpe_storage))
+      case ('c64')
+        gp%storage = GPE_C64
+      case ('mixed')
+        gp%storage = GPE_MIXED
+      case default
+        gp%storage = GPE_C128
+      end select
       gp%xr = xr; gp%yr = yr; gp%zr = zr
       gp%Urhs = Urhs; gp%diss_amp = diss_amp; gp%g = g; gp%mu = mu; gp%nn = nn
       gp%omx = omx; gp%omy = omy; gp%omz = omz; gp%vscale = gpe_vscale

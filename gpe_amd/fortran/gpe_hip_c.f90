@@ -6,7 +6,7 @@ module gpe_hip_c
   implicit none
 
   integer(c_int), parameter :: GPE_OK = 0
-  integer(c_int), parameter :: GPE_C128 = 0, GPE_C64 = 1
+  integer(c_int), parameter :: GPE_C128 = 0, GPE_C64 = 1, GPE_MIXED = 2
   integer(c_int), parameter :: GPE_HOST_C128 = 0, GPE_HOST_C64 = 1
   integer(c_int), parameter :: GPE_EULER = 0, GPE_RK2 = 1, GPE_RK4 = 2, GPE_RK45 = 3
   integer(c_int), parameter :: GPE_D_X = 0, GPE_D_Y = 1, GPE_D_Z = 2, GPE_D_XX = 3, &

@@ -58,7 +58,7 @@ module parameters
   ! grid_params: the compile-time values of the reference as run-time
   ! input, plus the device options of this build
   integer      :: nx = 64, ny = 64, nz = 64
-  character(4) :: storage = 'c128'   ! 'c128' or 'c64' device fields
+  character(5) :: storage = 'c128'   ! 'c128', 'c64' or 'mixed' device fields
   real (pr)    :: vscale = 0.0_pr     ! V term in eqn 1 (0 = reference)
   character(8) :: host_mode = 'resident' ! 'resident' or 'parity' (copy per step)
 

@@ -13,6 +13,8 @@ from dataclasses import dataclass, field
 
 from . import _lib
 
+STORAGE = {"c128": _lib.GPE_C128, "c64": _lib.GPE_C64, "mixed": _lib.GPE_MIXED}
+
 
 @dataclass
 class RunParams:
@@ -38,7 +40,7 @@ class RunParams:
     omy: float = 0.0
     omz: float = 0.0
     vscale: float = 0.0     # coefficient of V in eqn 1 (0 = the reference's eqn 1)
-    storage: str = "c128"   # "c128" or "c64"
+    storage: str = "c128"   # "c128", "c64" or "mixed" (c128 state, c64 RK increments)
     # adaptive time stepping (src/parameters.f90:89-92)
     eps: float = 1e-6
     safety: float = 0.9
@@ -80,11 +82,11 @@ class RunParams:
         return complex(self.tau, 0.0) if self.real_time else complex(0.0, -self.tau)
 
     def to_c(self) -> _lib.gpe_params:
-        if self.storage not in ("c128", "c64"):
-            raise ValueError("storage must be 'c128' or 'c64'")
+        if self.storage not in STORAGE:
+            raise ValueError("storage must be 'c128', 'c64' or 'mixed'")
         return _lib.gpe_params(
             self.nx, self.ny, self.nz, self.order, self.bcs, self.eqn_to_solve, int(bool(self.real_time)),
-            _lib.GPE_C128 if self.storage == "c128" else _lib.GPE_C64,
+            STORAGE[self.storage],
             self.xr, self.yr, self.zr, self.Urhs, self.diss_amp, self.g, self.mu, self.nn,
             self.omx, self.omy, self.omz, self.vscale, self.eps, self.safety, self.dt_decrease, self.dt_increase)
 

@@ -1,4 +1,4 @@
-"""Studies of the north-star additions, run on one GPU:
+"""Accuracy and speed studies of the storage and step-control variants, on one GPU:
 
   drift      complex64 storage vs complex128 on the trapped-vortex workload
              (256^3 reflective, dx = 0.2, 4th order, eqn 4, RK4 dt = 0.002,
@@ -6,9 +6,11 @@
              (gpe_hamiltonian) for each, and the c64 field's distance from the
              FP64 (c128) run.
   time_to_t  RKF45 to a fixed final time on the random-phase workload:
-             an FP64 reference at a tight tolerance, plain c128 at 1e-6,
-             c64 at 1e-6 (and the learned controller when a model is given);
-             wall time, steps, rejections and relative L2 error at T.
+             an FP64 reference at a tight tolerance, then plain c128, c64 and
+             mixed storage (c128 state, c64 increments) at 1e-6, and the
+             learned controller on mixed storage when a model is given;
+             wall time, steps, rejections, relative L2 error at T and the
+             speed-up over plain c128.
 
 Each prints one JSON object per line.  Usage:
     python -m gpe_amd.studies drift [--n 256] [--steps 2000]
@@ -82,7 +84,8 @@ def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=
     return st, final
 
 
-def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None):
+def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None,
+              storages=("c128", "c64", "mixed")):
     p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=dt0)
     ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
     emit(dict(study="time_to_t", n=n, T=T, method=ref.method, wall_s=ref.wall_s, steps=ref.n_accept,
@@ -90,11 +93,11 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
     rows = []
     runs = []
     for eps in eps_list:
-        runs.append(("c128", eps, None, False))
-        runs.append(("c64", eps, None, False))
+        for storage in storages:
+            runs.append((storage, eps, None, False))
         if learned_model is not None:
             from .learned import LearnedController
-            runs.append(("c64", eps, LearnedController(learned_model), True))
+            runs.append(("mixed", eps, LearnedController(learned_model, p.replace(eps=eps)), True))
     for storage, eps, ctrl, energy in runs:
         st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy)
         err = rel_l2(fin.astype(np.complex128), ref_final)
@@ -102,6 +105,15 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
                    steps=st.n_accept, rejects=st.n_reject, rhs=st.n_rhs, rel_l2_vs_oracle=err)
         rows.append(row)
         emit(row)
+    for eps in eps_list:
+        base = next((r for r in rows if r["eps"] == eps and r["storage"] == "c128" and r["method"].startswith("plain")),
+                    None)
+        if base is None:
+            continue
+        emit(dict(study="time_to_t_speedup", n=n, T=T, eps=eps, baseline=base["method"],
+                  **{r["method"]: dict(speedup=base["wall_s"] / r["wall_s"], rel_l2_vs_oracle=r["rel_l2_vs_oracle"],
+                                       within_1e6=r["rel_l2_vs_oracle"] <= 1e-6)
+                     for r in rows if r["eps"] == eps}))
     return rows
 
 
@@ -116,6 +128,7 @@ def main(argv=None):
     ap.add_argument("--oracle-eps", type=float, default=1e-10)
     ap.add_argument("--eps", default="1e-6")
     ap.add_argument("--model", default=None, help="learned controller weights (.npz)")
+    ap.add_argument("--storages", default="c128,c64,mixed")
     a = ap.parse_args(argv)
     if a.study == "drift":
         drift(a.n or 256, a.steps, a.every)
@@ -124,7 +137,8 @@ def main(argv=None):
         if a.model:
             from .learned import load_model
             model = load_model(a.model)
-        time_to_t(a.n or 512, a.T, a.dt0, a.oracle_eps, [float(v) for v in a.eps.split(",")], model)
+        time_to_t(a.n or 512, a.T, a.dt0, a.oracle_eps, [float(v) for v in a.eps.split(",")], model,
+                  tuple(a.storages.split(",")))
     return 0
 
 

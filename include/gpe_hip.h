@@ -44,6 +44,8 @@ extern "C" {
 /* storage precision of device fields */
 #define GPE_C128 0        /* complex128 fields, FP64 stencil arithmetic */
 #define GPE_C64 1         /* complex64 fields, FP32 stencil, FP64 reductions and time */
+#define GPE_MIXED 2       /* complex128 state and FP64 stencil; the RK4/RKF45 stage increments
+                             (k_j = dt f(u_j) and partial Butcher sums) stored as complex64 */
 
 /* host array element type */
 #define GPE_HOST_C128 0
@@ -73,7 +75,7 @@ typedef struct gpe_params {
     int32_t bcs;            /* 1 periodic, 2 reflective */
     int32_t eqn_to_solve;   /* 1..4 (src/solve.f90:414-437) */
     int32_t real_time;      /* 1: dt real, 0: imaginary time */
-    int32_t storage;        /* GPE_C128 or GPE_C64 */
+    int32_t storage;        /* GPE_C128, GPE_C64 or GPE_MIXED */
     double xr, yr, zr;      /* half widths: dx = 2 xr/(nx-1), x(i) = -xr + i dx (src/ic.f90:43-56) */
     double Urhs, diss_amp, g, mu, nn, omx, omy, omz;
     double vscale;          /* coefficient of the trap V in eqn 1: i/2(lap + (1-|psi|^2 - vscale V)psi);
@@ -172,7 +174,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 16
+#define GPE_NKINDS 25
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

@@ -39,6 +39,16 @@ def test_kernel_kind_names_match_header_count():
     assert lib.gpe_kernel_kind_name(_lib.GPE_NKINDS) == b""
 
 
+def test_storage_modes_match_header():
+    text = open(os.path.join(ROOT, "include", "gpe_hip.h")).read()
+    for name in ("GPE_C128", "GPE_C64", "GPE_MIXED"):
+        assert int(re.search(rf"#define {name} (\d+)", text).group(1)) == getattr(_lib, name)
+    p = RunParams(nx=8, ny=8, nz=8, xr=2.0, yr=2.0, zr=2.0)
+    assert [p.replace(storage=s).to_c().storage for s in ("c128", "c64", "mixed")] == [0, 1, 2]
+    with pytest.raises(ValueError, match="storage"):
+        p.replace(storage="c32").to_c()
+
+
 def test_struct_layouts_match_header():
     # gpe_params: 8 int32 then 16 doubles; gpe_step_info: 6 doubles + 3 int64
     assert ct.sizeof(_lib.gpe_params) == 8 * 4 + 16 * 8

@@ -92,7 +92,9 @@ def test_gpu_matches_reference_build(path):
     err = rel(got, z["out"])
     print(f"{os.path.basename(path)}: GPU vs reference build rel L2 = {err:.3e}")
     assert err <= 1e-12
-    assert abs(info.t - float(z["t"])) <= 1e-12 and abs(info.im_t - float(z["im_t"])) <= 1e-12
+    # rk45 dts come from a cancellation-limited error norm (see the dt check below)
+    t_tol = 1e-10 * max(abs(float(z["t"])), 1.0) if p.scheme == "rk45" else 1e-12
+    assert abs(info.t - float(z["t"])) <= t_tol and abs(info.im_t - float(z["im_t"])) <= t_tol
     assert abs(info.dt - complex(z["dt"])) <= 1e-9 * abs(info.dt)
     assert abs(nrm - float(z["norm"])) <= 1e-12 * float(z["norm"])
     assert abs(e - float(z["energy"])) <= 1e-11 * abs(float(z["energy"]))
