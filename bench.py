@@ -18,7 +18,6 @@ workload: --cpu-steps RK4 steps of the same grid and IC).
 from __future__ import annotations
 
 import argparse
-import ctypes as ct
 import json
 import os
 import sys
@@ -28,51 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 DOMINANT = "rk4_stage3"  # the RK4 kernel with the most traffic per launch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
-
-class Hip:
-    """HIP events through the runtime's C API (the kernels run on the
-    context's own stream, so torch's current-stream events would not see them)."""
-
-    def __init__(self):
-        self.h = ct.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
-
-    def event(self):
-        e = ct.c_void_p()
-        assert self.h.hipEventCreate(ct.byref(e)) == 0
-        return e
-
-    def record(self, e, stream):
-        assert self.h.hipEventRecord(e, ct.c_void_p(stream)) == 0
-
-    def elapsed_ms(self, a, b) -> float:
-        assert self.h.hipEventSynchronize(b) == 0
-        f = ct.c_float()
-        assert self.h.hipEventElapsedTime(ct.byref(f), a, b) == 0
-        return f.value
-
-    def device_sync(self):
-        assert self.h.hipDeviceSynchronize() == 0
-
-    def malloc(self, nbytes):
-        p = ct.c_void_p()
-        assert self.h.hipMalloc(ct.byref(p), ct.c_size_t(nbytes)) == 0
-        return p
-
-    def free(self, p):
-        assert self.h.hipFree(p) == 0
-
-
-def copy_probe(hip, nbytes):
-    """One plain copy kernel of known size (read nbytes + write nbytes): the
-    calibration point for the FETCH_SIZE / WRITE_SIZE counters."""
-    from gpe_amd import _lib
-
-    lib = _lib.load()
-    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
-    assert lib.gpe_bench_copy(b, a, nbytes, None) == 0
-    hip.device_sync()
-    hip.free(a)
-    hip.free(b)
+from gpe_amd.hiprt import Hip, copy_probe  # noqa: E402
 
 
 def timed_steps(ctx, hip, scheme, steps, info):
@@ -89,20 +44,7 @@ def timed_steps(ctx, hip, scheme, steps, info):
 
 def kernel_profile(ctx, scheme, steps, info):
     """Per-kind launch count, average duration (ms) and algorithmic bytes per launch."""
-    from gpe_amd import _lib
-
-    lib = ctx.lib
-    assert lib.gpe_prof_enable(ctx.h, 1) == 0
-    ctx.step(scheme, steps, info)
-    out = {}
-    for k in range(_lib.GPE_NKINDS):
-        ms, n, b = ct.c_double(), ct.c_int64(), ct.c_double()
-        assert lib.gpe_prof_read(ctx.h, k, ct.byref(ms), ct.byref(n), ct.byref(b)) == 0
-        if n.value:
-            out[lib.gpe_kernel_kind_name(k).decode()] = {"launches": n.value, "avg_ms": ms.value / n.value,
-                                                         "bytes": b.value}
-    assert lib.gpe_prof_enable(ctx.h, 0) == 0
-    return out
+    return ctx.kernel_profile(lambda: ctx.step(scheme, steps, info))
 
 
 def load_traffic(name):

@@ -185,3 +185,21 @@ class Context:
 
     def sync(self) -> None:
         self._chk(self.lib.gpe_sync(self.h), "gpe_sync")
+
+    def kernel_profile(self, fn) -> dict:
+        """Run fn() with the library's per-launch event timing on and return
+        {kind: {"launches", "avg_ms", "bytes"}} (bytes = algorithmic field
+        bytes of one launch)."""
+        self._chk(self.lib.gpe_prof_enable(self.h, 1), "gpe_prof_enable")
+        try:
+            fn()
+            out = {}
+            for k in range(_lib.GPE_NKINDS):
+                ms, n, b = ct.c_double(), ct.c_int64(), ct.c_double()
+                self._chk(self.lib.gpe_prof_read(self.h, k, ct.byref(ms), ct.byref(n), ct.byref(b)), "gpe_prof_read")
+                if n.value:
+                    out[self.lib.gpe_kernel_kind_name(k).decode()] = {"launches": n.value, "avg_ms": ms.value / n.value,
+                                                                      "bytes": b.value}
+        finally:
+            self.lib.gpe_prof_enable(self.h, 0)
+        return out

@@ -19,13 +19,33 @@ import json
 import os
 import re
 
-KINDS = {
-    "OpRk4First": "rk4_stage1", "OpRk4Second": "rk4_stage2", "OpRk4Third": "rk4_stage3", "OpRk4Last": "rk4_stage4",
-    "OpAxpySelf": "euler_or_rk2_stage1", "OpAxpyP0": "rk2_stage2", "OpCk1": "rkck_stage1",
-    "OpCkLast": "rkck_stage6_error", "OpCkMid<double, 2>": "rkck_stage2", "OpCkMid<float, 2>": "rkck_stage2",
-}
-ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3}
-DOMINANT = "rk4_stage3"
+# (pattern in the kernel name, kind) in match order
+KINDS = [
+    ("OpRk4First", "rk4_stage1"), ("OpRk4Second", "rk4_stage2"), ("OpRk4Third", "rk4_stage3"),
+    ("OpRk4Last", "rk4_stage4"), ("OpAxpySelf", "euler_or_rk2_stage1"), ("OpAxpyP0", "rk2_stage2"),
+    ("OpCk1", "rkck_stage1"), ("OpCkLast", "rkck_stage6_error"),
+    ("OpCkMid<double, 2>", "rkck_stage2"), ("OpCkMid<float, 2>", "rkck_stage2"),
+    ("OpCkMid<double, 3>", "rkck_stage3"), ("OpCkMid<float, 3>", "rkck_stage3"),
+    ("OpCkMid<double, 4>", "rkck_stage4"), ("OpCkMid<float, 4>", "rkck_stage4"),
+    ("OpCkMid<double, 5>", "rkck_stage5"), ("OpCkMid<float, 5>", "rkck_stage5"),
+    ("OpMxRk4K<16, 24>", "mx_rk4_stage1"), ("OpMxRk4K<17, 32>", "mx_rk4_stage2_3"), ("OpMxRk4Last", "mx_rk4_stage4"),
+    ("OpMxCk1", "mx_rkck_stage1"), ("OpMxCk2", "mx_rkck_stage2"), ("OpMxCkMid<3>", "mx_rkck_stage3"),
+    ("OpMxCkMid<4>", "mx_rkck_stage4"), ("OpMxCkMid<5>", "mx_rkck_stage5"), ("OpMxCkLast", "mx_rkck_stage6_error"),
+]
+# algorithmic field bytes per cell: accesses x element size, or fixed bytes (mixed storage)
+ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3,
+          "rkck_stage1": 3, "rkck_stage2": 5, "rkck_stage3": 6, "rkck_stage4": 7, "rkck_stage5": 8,
+          "rkck_stage6_error": 7}
+MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
+               "mx_rkck_stage1": 24, "mx_rkck_stage2": 40, "mx_rkck_stage3": 56, "mx_rkck_stage4": 64,
+               "mx_rkck_stage5": 72, "mx_rkck_stage6_error": 72}
+DOMINANT = {"c128": "rk4_stage3", "c64": "rk4_stage3", "mixed": "mx_rk4_stage4"}
+
+
+def cell_bytes(kind: str, prec: str) -> int:
+    if prec == "mixed":
+        return MIXED_BYTES.get(kind, 0)
+    return ACCESS.get(kind, 0) * (16 if prec == "c128" else 8)
 
 
 def _one(pattern):
@@ -37,11 +57,17 @@ def _one(pattern):
 
 def kernel_label(name: str) -> str:
     m = re.search(r"k_stencil<(\w+), (\d), (\d+), (\d+), gpe::(\w+)", name)
-    if m:
-        prec = "c128" if m.group(1) == "double" else "c64"
-        op = m.group(5)
-        kind = next((v for k, v in KINDS.items() if name.find(k) >= 0), op)
-        return f"{kind} {prec} order{m.group(2)} tile{m.group(3)}x{m.group(4)}"
+    ms = re.search(r"k_stencil_sum<(\d), (\d+), (\d+), gpe::(\w+)", name)
+    if m or ms:
+        if ms:
+            prec, order, tx, ty, op = "mixed", ms.group(1), ms.group(2), ms.group(3), ms.group(4)
+        else:
+            prec = "c128" if m.group(1) == "double" else "c64"
+            order, tx, ty, op = m.group(2), m.group(3), m.group(4), m.group(5)
+        kind = next((v for k, v in KINDS if name.find(k) >= 0), op)
+        if kind.startswith("mx_"):
+            prec = "mixed"
+        return f"{kind} {prec} order{order} tile{tx}x{ty}"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
@@ -61,7 +87,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--out", default="profiles")
-    ap.add_argument("--command", default="")
+    ap.add_argument("--workload", choices=["rk4", "rkck"], default="rk4",
+                    help="rk4: bench.py runs; rkck: gpe_amd.studies attempts runs")
     a = ap.parse_args(argv)
 
     stats = list(csv.DictReader(open(_one(os.path.join(a.kt, "**", "*kernel_stats.csv")))))
@@ -73,15 +100,24 @@ def main(argv=None):
     w_scale = copy_bytes / write[kcopy]
     cells = a.n ** 3
     os.makedirs(a.out, exist_ok=True)
-    lines = [f"# RK4 step kernels, {a.n}^3 random-phase workload (bench.py)", "",
+    if a.workload == "rk4":
+        title = f"# RK4 step kernels, {a.n}^3 random-phase workload (bench.py)"
+        run_kt = "python3 bench.py --steps 20 --warmup 5 --cpu-steps 0"
+        run_pmc = f"python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}"
+    else:
+        title = f"# RKF45 (Cash-Karp) stage kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
+        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128,mixed,c64"
+        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128,mixed,c64 "
+                   f"--copy-probe-mib {a.copy_mib}")
+    lines = [title, "",
              "Commands (run from the repository root; TMPDIR=/tmp):", "",
              "    rocprofv3 --kernel-trace --stats -d <KT_DIR> -o kt --output-format csv -- \\",
-             "        python3 bench.py --steps 20 --warmup 5 --cpu-steps 0",
+             f"        {run_kt}",
              "    rocprofv3 --pmc FETCH_SIZE -d <FETCH_DIR> -o fetch --output-format csv -- \\",
-             f"        python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}",
+             f"        {run_pmc}",
              "    rocprofv3 --pmc WRITE_SIZE -d <WRITE_DIR> -o write --output-format csv -- \\",
-             f"        python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}",
-             f"    python3 -m gpe_amd.profsum <KT_DIR> <FETCH_DIR> <WRITE_DIR> --n {a.n}", "",
+             f"        {run_pmc}",
+             f"    python3 -m gpe_amd.profsum <KT_DIR> <FETCH_DIR> <WRITE_DIR> --n {a.n} --workload {a.workload}", "",
              f"Counter calibration with a plain copy of {a.copy_mib} MiB (k_copy16): FETCH_SIZE read "
              f"{fetch[kcopy] / 2**20:.1f} MiB -> scale {f_scale:.4f}; WRITE_SIZE read {write[kcopy] / 2**20:.1f} MiB "
              f"-> scale {w_scale:.4f}.", "",
@@ -96,24 +132,23 @@ def main(argv=None):
         label = kernel_label(name)
         kind = label.split()[0]
         prec = label.split()[1]
-        esz = 16 if prec == "c128" else 8
-        alg = ACCESS.get(kind, 0) * esz * cells
+        alg = cell_bytes(kind, prec) * cells
         avg_ms = float(r["AverageNs"]) / 1e6
         fk = next((v for k, v in fetch.items() if k.startswith(name[:120])), None)
         wk = next((v for k, v in write.items() if k.startswith(name[:120])), None)
         rd = fk * f_scale if fk else None
         wr = wk * w_scale if wk else None
         t = (rd + wr) if (rd and wr) else None
-        if t and kind == DOMINANT:
+        if t and a.workload == "rk4" and kind == DOMINANT.get(prec):
             traffic[prec] = t
             json.dump({"traffic_bytes_per_launch": t, "read_bytes": rd, "write_bytes": wr,
                        "algorithmic_bytes_per_launch": alg, "fetch_scale": f_scale, "write_scale": w_scale,
                        "kernel": label},
-                      open(os.path.join(a.out, f"traffic_{DOMINANT}_{prec}_{a.n}.json"), "w"), indent=1)
+                      open(os.path.join(a.out, f"traffic_{kind}_{prec}_{a.n}.json"), "w"), indent=1)
         lines.append(f"| {label} | {r['Calls']} | {avg_ms:.4f} | {alg / 1e9:.3f} | "
                      f"{alg / (avg_ms * 1e-3) / 1e9:.0f} | {100 * alg / (avg_ms * 1e-3) / 8e12:.1f} | "
                      f"{(rd or 0) / 1e9:.3f} | {(wr or 0) / 1e9:.3f} | {(t / alg) if (t and alg) else float('nan'):.3f} |")
-    out = os.path.join(a.out, f"rk4_stencil_{a.n}.md")
+    out = os.path.join(a.out, f"{a.workload}_stencil_{a.n}.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(open(out).read())
     return 0

@@ -12,9 +12,13 @@
              wall time, steps, rejections, relative L2 error at T and the
              speed-up over plain c128.
 
+  attempts   RKF45 attempts per storage with per-kernel times (the RKCK
+             profiling target).
+
 Each prints one JSON object per line.  Usage:
     python -m gpe_amd.studies drift [--n 256] [--steps 2000]
     python -m gpe_amd.studies time_to_t [--n 512] [--T 20]
+    python -m gpe_amd.studies attempts [--n 512] [--attempts 6] [--storages c128,mixed]
 """
 from __future__ import annotations
 
@@ -117,9 +121,37 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
     return rows
 
 
+def attempts(n: int, count: int, storages, copy_mib: int = 0):
+    """`count` RKF45 steps (rkqs: attempts until one is accepted) of the
+    random-phase workload at tolerance 1e-6 for each storage, with the
+    library's per-kernel timing: time per attempt and per RKCK stage kernel.
+    Also the target of the RKCK rocprofv3 profiles (copy_mib launches the
+    counter-calibration copy first)."""
+    from .hiprt import Hip, copy_probe
+
+    if copy_mib:
+        copy_probe(Hip(), copy_mib << 20)
+    p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=0.01)
+    for storage in storages:
+        with Context(p.replace(storage=storage, eps=1e-6)) as ctx:
+            ctx.upload(psi)
+            info = StepInfo(p.dt0())
+            ctx.step("rk45", 2, info)
+            a0 = info.n_accept + info.n_reject
+            t0 = time.perf_counter()
+            prof = ctx.kernel_profile(lambda: ctx.step("rk45", count, info))
+            wall = time.perf_counter() - t0
+            na = info.n_accept + info.n_reject - a0
+            ker = {k: dict(avg_ms=v["avg_ms"], launches=v["launches"],
+                           GB_s=v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9) for k, v in prof.items()}
+            emit(dict(study="attempts", n=n, storage=storage, steps=count, attempts=na,
+                      ms_per_attempt=sum(v["avg_ms"] * v["launches"] for v in prof.values()) / max(na, 1),
+                      wall_ms_per_attempt=1e3 * wall / max(na, 1), kernels=ker))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["drift", "time_to_t"])
+    ap.add_argument("study", choices=["drift", "time_to_t", "attempts"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=100)
@@ -129,9 +161,13 @@ def main(argv=None):
     ap.add_argument("--eps", default="1e-6")
     ap.add_argument("--model", default=None, help="learned controller weights (.npz)")
     ap.add_argument("--storages", default="c128,c64,mixed")
+    ap.add_argument("--attempts", type=int, default=6)
+    ap.add_argument("--copy-probe-mib", type=int, default=0)
     a = ap.parse_args(argv)
     if a.study == "drift":
         drift(a.n or 256, a.steps, a.every)
+    elif a.study == "attempts":
+        attempts(a.n or 512, a.attempts, tuple(a.storages.split(",")), a.copy_probe_mib)
     else:
         model = None
         if a.model:
