@@ -36,14 +36,18 @@ template <int NT, int MODE> __device__ __forceinline__ double block_reduce(doubl
 //   struct Pre; Pre load(Cell i) const   -- pointwise inputs of cell i
 //   void apply(Cell i, int x, int y, int z, const Sten& st, const Pre& p, double* red) const
 // GPE_WAVES_PER_EU > 0 asks the compiler for at least that many waves per
-// SIMD (caps VGPRs); GPE_PW_PREFETCH = 1 prefetches the pointwise inputs of
-// the next plane (double-buffered registers), 0 loads them at the top of the
-// plane's iteration.
+// SIMD (caps VGPRs); GPE_PW_PREFETCH = 1 prefetches the pointwise inputs
+// together with the stencil planes, 0 loads them at the top of the plane's
+// iteration.
 #ifndef GPE_WAVES_PER_EU
 #define GPE_WAVES_PER_EU 0
 #endif
 #ifndef GPE_PW_PREFETCH
 #define GPE_PW_PREFETCH 1
+#endif
+// Planes whose loads are in flight while a plane is evaluated (1 or 2).
+#ifndef GPE_PF_DEPTH
+#define GPE_PF_DEPTH 2
 #endif
 #if GPE_WAVES_PER_EU > 0
 #define GPE_OCC __attribute__((amdgpu_waves_per_eu(GPE_WAVES_PER_EU)))
@@ -114,6 +118,12 @@ struct SumSrc {
 template <class Op, class = void> struct OpPrefetch : std::true_type {};
 template <class Op> struct OpPrefetch<Op, std::void_t<decltype(Op::kPrefetch)>> : std::bool_constant<Op::kPrefetch> {};
 template <class Op> constexpr bool op_prefetch() { return OpPrefetch<Op>::value; }
+// An operator may also cap the prefetch depth (kDepth = 1) where two sets
+// of in-flight registers would spill.
+template <class Op, class = void> struct OpDepth : std::integral_constant<int, GPE_PF_DEPTH> {};
+template <class Op>
+struct OpDepth<Op, std::void_t<decltype(Op::kDepth)>>
+    : std::integral_constant<int, (Op::kDepth < GPE_PF_DEPTH ? Op::kDepth : GPE_PF_DEPTH)> {};
 
 // Body of the stencil kernel for the tile `bid` of grid g; the block's
 // reductions (if any) go to partial[r * g.nblocks + pidx].
@@ -170,44 +180,45 @@ __device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w,
         hoff[s] = (unsigned)bcmap(gx, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(gy, g.ny, g.bcs);
     }
 
-    // Register z-window q[0..2H] = planes z-H..z+H, qn = plane z+H+1
-    // (prefetch); qa/qan the matching Aux of the own column (only qa[H] is
-    // read, the rest is dead and removed when Aux is empty).
+    // Register z-window q[0..2H] = planes z-H..z+H; qa the matching Aux of
+    // the own column (only qa[H] is read, the rest is dead and removed when
+    // Aux is empty).
     C<R> q[2 * H + 1];
     Aux qa[2 * H + 1];
 #pragma unroll
     for (int d = 0; d < 2 * H; ++d) q[d] = src(Cell{g.plane * bcmap(z0 - H + d, g.nz, g.bcs), col}, qa[d]);
-    C<R> qn = mkc<R>(R(0), R(0));
-    Aux qan{};
-    C<R> hr[NSLOT];
-    typename Op::Pre pc{};
-    if (z0 < z1) {
-        qn = src(Cell{g.plane * bcmap(z0 + H, g.nz, g.bcs), col}, qan);
+    double red[2] = {0.0, 0.0};
+
+    // The loads for plane z: centre of plane z+H (the window's new entry),
+    // the halo of plane z and the operator's pointwise inputs.
+    struct Pf {
+        C<R> qn;
+        Aux qan;
+        C<R> hr[NSLOT];
+        typename Op::Pre pw;
+    };
+    auto fetch = [&](Pf& S, int zz) {
+        S.qn = src(Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), col}, S.qan);
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (hv[s]) hr[s] = src(Cell{g.plane * z0, hoff[s]});
-        if (PF && valid) pc = op.load(Cell{g.plane * z0, own});
-    }
-
-    double red[2] = {0.0, 0.0};
-    for (int z = z0; z < z1; ++z) {
-        q[2 * H] = qn;
-        qa[2 * H] = qan;
+            if (hv[s]) S.hr[s] = src(Cell{g.plane * zz, hoff[s]});
+        if (PF && valid) S.pw = op.load(Cell{g.plane * zz, own});
+    };
+    // One plane: stage plane z in LDS (double buffered, one barrier), start
+    // the loads of plane z + DEPTH into S, evaluate and store plane z.
+    constexpr int DEPTH = OpDepth<Op>::value;
+    auto step = [&](int z, Pf& S) {
+        q[2 * H] = S.qn;
+        qa[2 * H] = S.qan;
+        typename Op::Pre pc = S.pw;
         if (!PF && valid) pc = op.load(Cell{g.plane * z, own});
         C<R>* L = lds[(z - z0) & 1];
         L[lc] = q[H];
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (hv[s]) L[hl[s]] = hr[s];
+            if (hv[s]) L[hl[s]] = S.hr[s];
         __syncthreads();
-        typename Op::Pre pn{};
-        if (z + 1 < z1) {
-            qn = src(Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), col}, qan);
-#pragma unroll
-            for (int s = 0; s < NSLOT; ++s)
-                if (hv[s]) hr[s] = src(Cell{g.plane * (z + 1), hoff[s]});
-            if (PF && valid) pn = op.load(Cell{g.plane * (z + 1), own});
-        }
+        if (z + DEPTH < z1) fetch(S, z + DEPTH);
         if (valid) {
             Sten<R, ORD, LW, GEN> st;
             st.L = L;
@@ -218,11 +229,24 @@ __device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w,
             if constexpr (kAux) op.apply(Cell{g.plane * z, own}, x, y, z, st, pc, qa[H], red);
             else op.apply(Cell{g.plane * z, own}, x, y, z, st, pc, red);
         }
-        if (PF) pc = pn;
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) {
             q[d] = q[d + 1];
             qa[d] = qa[d + 1];
+        }
+    };
+    if constexpr (DEPTH == 1) {
+        Pf S0{};
+        if (z0 < z1) fetch(S0, z0);
+        for (int z = z0; z < z1; ++z) step(z, S0);
+    } else {
+        // two planes in flight: S0 carries planes z0, z0+2, ..., S1 the others
+        Pf S0{}, S1{};
+        if (z0 < z1) fetch(S0, z0);
+        if (z0 + 1 < z1) fetch(S1, z0 + 1);
+        for (int z = z0; z < z1; z += 2) {
+            step(z, S0);
+            if (z + 1 < z1) step(z + 1, S1);
         }
     }
 
@@ -410,6 +434,7 @@ template <typename R> struct OpCk1 {
 template <typename R, int S_> struct OpCkMid {
     // kernel s reads u_s, psi0, f0, k2..k_{s-1} and writes k_s, u_{s+1}
     static constexpr int kKind = 4 + S_, kAccess = S_ + 3;
+    static constexpr int kDepth = S_ >= 4 ? 1 : 2;
     static constexpr int kNRed = 0, kMode = RED_NONE;
     struct Pre { C<R> p0, f0, k2, k3, k4; };
     RhsC<R> k;
@@ -456,6 +481,7 @@ template <typename R, int S_> struct OpCkMid {
 // The error combination and scale are evaluated in FP64.
 template <typename R> struct OpCkLast {
     static constexpr int kKind = 10, kAccess = 7;
+    static constexpr int kDepth = 1;
     static constexpr int kNRed = 2, kMode = RED_MAX;
     struct Pre { C<R> p0, f0, k3, k4, k5; };
     RhsC<R> k;

@@ -151,6 +151,7 @@ template <int S_> struct OpMxCkMid {
 struct OpMxCkLast {
     static constexpr int kKind = 24, kBytesPerCell = 72;
     static constexpr bool kPrefetch = false;
+    static constexpr int kDepth = 1;
     static constexpr int kNRed = 2, kMode = RED_MAX;
     struct Pre { CF f0, k3, k4, k5; };
     RhsC<double> k;
