@@ -5,13 +5,14 @@ Workload (the headline metric's configuration): 512^3 periodic grid,
 dx = 0.3, 4th-order centred FD, eqn 1 (i/2(lap + (1-|psi|^2)psi)), classic
 RK4 with dt = 0.005, random-phase turbulence IC (Gaussian shell spectrum,
 k0 = 1, sigma = 0.2, seed 0, mean |psi|^2 = 1), complex128 fields.  A step
-is one RK4 step of the whole grid (4 fused stencil+stage kernels); inputs
-are resident in HBM when the timed region starts.
+is one RK4 step of the whole grid (two fused kernels of two stages each;
+--unfused: the four stage kernels); inputs are resident in HBM when the
+timed region starts.
 
 Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
-dominant kernel (RK4 stage 3, 5 field accesses per cell of the step's 13,
-timed live with HIP events around each
-launch on the context's stream), the complex64-storage run, and a CPU
+dominant kernel (the one with the largest share of the step's time), timed
+live with HIP events around each launch on the context's stream, the
+complex64- and mixed-storage runs, and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC).
 """
@@ -24,7 +25,6 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
-DOMINANT = "rk4_stage3"  # the RK4 kernel with the most traffic per launch
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 from gpe_amd.hiprt import Hip, copy_probe  # noqa: E402
@@ -61,6 +61,7 @@ def run_storage(p, psi, args, hip):
     from gpe_amd import Context, StepInfo
 
     with Context(p) as ctx:
+        ctx.set_fused(not args.unfused)
         if args.tiles:
             ctx.set_tiles(*[int(v) for v in args.tiles.split(",")])
         ctx.upload(psi)
@@ -70,6 +71,11 @@ def run_storage(p, psi, args, hip):
         prof = kernel_profile(ctx, "rk4", args.steps, info)
         tiles = ctx.tiles()
     return ms, prof, tiles
+
+
+def dominant(prof):
+    """The kernel kind with the largest share of the step's time."""
+    return max(prof, key=lambda k: prof[k]["avg_ms"] * prof[k]["launches"])
 
 
 def roofline(prof, kind, traffic=None):
@@ -108,6 +114,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU baseline sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
+    ap.add_argument("--unfused", action="store_true", help="four stage kernels per RK4 step instead of two fused")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (counter calibration)")
     args = ap.parse_args(argv)
@@ -130,7 +137,8 @@ def main(argv=None) -> int:
     print(f"[bench] c128: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     value = p.cells * args.steps / (ms * 1e-3) / 1e9
     step_bytes = sum(v["bytes"] for k, v in prof.items() if k.startswith("rk4"))
-    rf = roofline(prof, DOMINANT, load_traffic(f"traffic_{DOMINANT}_c128_{args.n}.json"))
+    dom = dominant(prof)
+    rf = roofline(prof, dom, load_traffic(f"traffic_{dom}_c128_{args.n}.json"))
     result = {
         "metric": "RK4 Gcell-steps/s + %HBM peak at 512³; time-to-T, learned vs plain RKF45",
         "value": round(value, 4),
@@ -147,7 +155,8 @@ def main(argv=None) -> int:
         "config": {"workload": f"random-phase turbulence {args.n}^3 periodic, dx=0.3, 4th-order FD, eqn 1, "
                                "RK4 dt=0.005, complex128 fields",
                    "grid": [args.n] * 3, "order": 4, "bcs": "periodic", "scheme": "rk4", "dt": 0.005,
-                   "storage": "c128", "tiles": list(tiles)},
+                   "storage": "c128", "kernels_per_step": "4 stage kernels" if args.unfused else "2 fused (stages 1+2, 3+4)",
+                   "tiles": list(tiles)},
         "roofline": rf,
         "step_hbm_frac": round(step_bytes * args.steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernel_table(prof),
@@ -158,8 +167,8 @@ def main(argv=None) -> int:
         print(f"[bench] c64: {ms64 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
         result["c64"] = {"value": round(p.cells * args.steps / (ms64 * 1e-3) / 1e9, 4),
                          "ms_per_step": round(ms64 / args.steps, 4), "dtype": "f32 stencil, f64 reductions",
-                         "roofline": roofline(prof64, DOMINANT,
-                                              load_traffic(f"traffic_{DOMINANT}_c64_{args.n}.json")),
+                         "roofline": roofline(prof64, dominant(prof64),
+                                              load_traffic(f"traffic_{dominant(prof64)}_c64_{args.n}.json")),
                          "kernels": kernel_table(prof64)}
     if not args.no_mixed:
         pm = p.replace(storage="mixed")
@@ -168,8 +177,8 @@ def main(argv=None) -> int:
         result["mixed"] = {"value": round(p.cells * args.steps / (msm * 1e-3) / 1e9, 4),
                            "ms_per_step": round(msm / args.steps, 4),
                            "dtype": "f64 state and stencil, complex64 RK increments",
-                           "roofline": roofline(profm, "mx_rk4_stage4",
-                                                load_traffic(f"traffic_mx_rk4_stage4_mixed_{args.n}.json")),
+                           "roofline": roofline(profm, dominant(profm),
+                                                load_traffic(f"traffic_{dominant(profm)}_mixed_{args.n}.json")),
                            "kernels": kernel_table(profm)}
     if args.cpu_steps > 0:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libgpehip.so")
 GPE_OK = 0
 GPE_C128, GPE_C64, GPE_MIXED = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
-GPE_NKINDS = 25  # kernel kinds counted by gpe_prof_read
+GPE_NKINDS = 27  # kernel kinds counted by gpe_prof_read
 SCHEMES = {"euler": 0, "rk2": 1, "rk4": 2, "rk45": 3}
 DERIV_OPS = {"x": 0, "y": 1, "z": 2, "xx": 3, "yy": 4, "zz": 5, "lap": 6}
 
@@ -52,6 +52,7 @@ PROTOTYPES = [
     ("gpe_ctx_create", _I, [ct.POINTER(gpe_params), ct.POINTER(_VP)]),
     ("gpe_ctx_destroy", None, [_VP]),
     ("gpe_set_tiles", _I, [_VP, _I, _I, _I]),
+    ("gpe_set_fused", _I, [_VP, _I]),
     ("gpe_get_tiles", _I, [_VP, ct.POINTER(_I), ct.POINTER(_I), ct.POINTER(_I)]),
     ("gpe_upload", _I, [_VP, _VP, _I, _I64, _I64, _I64]),
     ("gpe_download", _I, [_VP, _VP, _I]),

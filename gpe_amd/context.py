@@ -86,6 +86,10 @@ class Context:
     def set_tiles(self, tx: int, ty: int, tz: int = 0) -> None:
         self._chk(self.lib.gpe_set_tiles(self.h, tx, ty, tz), "gpe_set_tiles")
 
+    def set_fused(self, on: bool) -> None:
+        """Fused RK4 (two kernels per step) where it applies; on by default."""
+        self._chk(self.lib.gpe_set_fused(self.h, int(bool(on))), "gpe_set_fused")
+
     def tiles(self):
         a, b, c = ct.c_int(), ct.c_int(), ct.c_int()
         self._chk(self.lib.gpe_get_tiles(self.h, ct.byref(a), ct.byref(b), ct.byref(c)), "gpe_get_tiles")

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/gpe_hip.h"
+#include "gpe_fused.h"
 #include "gpe_host.h"
 
 using namespace gpe;
@@ -25,6 +26,8 @@ thread_local std::string g_thread_error;
 struct gpe_ctx {
     gpe_params p;
     int device = 0;
+    int ncu = 256;        // compute units of the device (fused RK4 wave sizing)
+    bool fused = true;    // fused RK4 where it applies (gpe_set_fused)
     hipStream_t stream = nullptr;
     long long n = 0;   // cells
     size_t esz = 0;    // bytes per complex value
@@ -75,7 +78,8 @@ static const char* const kKindNames[GPE_NKINDS] = {
     "euler_or_rk2_stage1", "rk2_stage2", "rk4_stage1", "rk4_stage2", "rk4_stage4", "rkck_stage1",
     "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
     "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
-    "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error"};
+    "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error",
+    "rk4_fused12", "rk4_fused34"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -213,10 +217,11 @@ template <class Op> struct OpBytes<Op, std::void_t<decltype(Op::kBytesPerCell)>>
     static double get(size_t) { return (double)Op::kBytesPerCell; }
 };
 
-// Launches one stencil pass (launch(g, nb) enqueues the kernel), with the
-// optional per-launch event timing and the final reduction of its partials.
+// Launches one stencil pass (launch(g) enqueues the kernel) on the tiled
+// geometry g, with the optional per-launch event timing and the final
+// reduction of its partials.
 template <class Op, class Launch> static int run_launch(gpe_ctx* c, Geom g, Launch launch) {
-    const int nb = tile_geom(c->tiles, g);
+    const int nb = g.nblocks;
     if (Op::kNRed > 0) {
         int rc = ensure_partial(c, (size_t)nb * Op::kNRed);
         if (rc) return rc;
@@ -239,7 +244,9 @@ template <class Op, class Launch> static int run_launch(gpe_ctx* c, Geom g, Laun
 }
 
 template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const void* u, const Op& op) {
-    return run_launch<Op>(c, make_geom<R>(c), [&](Geom g) {
+    Geom g0 = make_geom<R>(c);
+    tile_geom(c->tiles, g0);
+    return run_launch<Op>(c, g0, [&](Geom g) {
         return launch_stencil<R, ORD, Op>(c->tiles, g, make_w<R>(c->p), (const C<R>*)u, op, c->partial, c->stream);
     });
 }
@@ -248,7 +255,9 @@ template <typename R, int ORD, class Op> static int run_op(gpe_ctx* c, const voi
 template <int ORD, class Op>
 static int run_op_sum(gpe_ctx* c, const void* a, const void* d, double2 s, const Op& op) {
     const SumSrc src{(const double2*)a, (const float2*)d, s};
-    return run_launch<Op>(c, make_geom<double>(c), [&](Geom g) {
+    Geom g0 = make_geom<double>(c);
+    tile_geom(c->tiles, g0);
+    return run_launch<Op>(c, g0, [&](Geom g) {
         return launch_stencil_sum<ORD, Op>(c->tiles, g, make_w<double>(c->p), src, op, c->partial, c->stream);
     });
 }
@@ -290,9 +299,18 @@ template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, dou
     return 0;
 }
 
+// Fused RK4 applies to complex128 storage with a plain right-hand side.
+static bool fused_rk4(const gpe_ctx* c, int scheme) {
+    return c->fused && scheme == GPE_RK4 && c->p.storage == GPE_C128 && make_rhs_p<double>(c->p, nullptr).flags == 0;
+}
+
 static int need_rk_buffers(gpe_ctx* c, int scheme) {
     int rc;
     const size_t fb = field_bytes(c);
+    if (fused_rk4(c, scheme)) {
+        if ((rc = dalloc(c, &c->uA, fb, "uA"))) return rc;
+        return dalloc(c, &c->pnew, fb, "pnew");
+    }
     if (c->p.storage == GPE_MIXED && (scheme == GPE_RK4 || scheme == GPE_RK45)) {
         const int nmx = scheme == GPE_RK4 ? 3 : 7;
         for (int j = 0; j < nmx; ++j)
@@ -407,6 +425,38 @@ static int ck_attempt_mixed(gpe_ctx* c, double dre, double dim, double d0re, dou
     out[0] = c->hres[0] / c->p.eps;
     out[1] = c->hres[1];
     c->cand_valid = true;
+    return 0;
+}
+
+// Launch-accounting tags of the fused kernels (run_launch reads these).
+struct FusedTag12 {
+    static constexpr int kKind = 25, kBytesPerCell = 32, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedTag34 {
+    static constexpr int kKind = 26, kBytesPerCell = 48, kNRed = 0, kMode = RED_NONE;
+};
+
+// One classic RK4 step as two fused kernels (gpe_fused.h): psi -> u2 (uA),
+// then (u2, psi) -> psi_new (pnew, swapped in).
+template <int ORD> static int step_rk4_fused(gpe_ctx* c, double dre, double dim) {
+    const RhsC<double> k = make_rhs<double>(c);
+    const StW<double> w = make_w<double>(c->p);
+    const double2 dt = cdt<double>(dre, dim, 1.0);
+    Geom g = make_geom<double>(c);
+    fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
+    const double2* psi = (const double2*)c->psi;
+    double2* u2 = (double2*)c->uA;
+    double2* out = (double2*)c->pnew;
+    int rc;
+    if ((rc = run_launch<FusedTag12>(c, g, [&](Geom gg) {
+             return launch_rk4_fused<ORD>(gg, w, k, psi, u2, nullptr, dt, 12, c->stream);
+         })))
+        return rc;
+    if ((rc = run_launch<FusedTag34>(c, g, [&](Geom gg) {
+             return launch_rk4_fused<ORD>(gg, w, k, psi, u2, out, dt, 34, c->stream);
+         })))
+        return rc;
+    std::swap(c->psi, c->pnew);
     return 0;
 }
 
@@ -654,6 +704,7 @@ int gpe_ctx_create(const gpe_params* p, gpe_ctx** out) {
     c->tiles = default_tiles(*p);
     int rc = 0;
     hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         rc = fail(nullptr, GPE_EHIP, std::string("gpe_ctx_create: ") + hipGetErrorString(e));
@@ -706,6 +757,12 @@ int gpe_set_tiles(gpe_ctx* c, int tx, int ty, int tz) {
     return 0;
 }
 
+int gpe_set_fused(gpe_ctx* c, int on) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_set_fused: ctx is NULL");
+    c->fused = on != 0;
+    return 0;
+}
+
 int gpe_get_tiles(const gpe_ctx* c, int* tx, int* ty, int* tz) {
     if (!c || !tx || !ty || !tz) return fail(nullptr, GPE_EINVAL, "gpe_get_tiles: NULL argument");
     *tx = c->tiles.tx;
@@ -744,7 +801,10 @@ int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
     }
     const int nrhs = scheme == GPE_EULER ? 1 : scheme == GPE_RK2 ? 2 : 4;
     for (int s = 0; s < nsteps; ++s) {
-        if (c->p.storage == GPE_MIXED && scheme == GPE_RK4)
+        if (fused_rk4(c, scheme))
+            rc = c->p.order == 2 ? step_rk4_fused<2>(c, io->dt_re, io->dt_im)
+                                 : step_rk4_fused<4>(c, io->dt_re, io->dt_im);
+        else if (c->p.storage == GPE_MIXED && scheme == GPE_RK4)
             rc = c->p.order == 2 ? step_rk4_mixed<2>(c, io->dt_re, io->dt_im)
                                  : step_rk4_mixed<4>(c, io->dt_re, io->dt_im);
         else

@@ -1,0 +1,297 @@
+// Fused RK4: the four stages of a classic RK4 step in two kernels that each
+// evaluate two stages (temporal blocking in the z-march), for complex128
+// fields and a right-hand side without optional terms (RhsC::flags == 0).
+//
+//   F12: u1 = psi0 + dt/2 f(psi0) on the whole block, then
+//        u2 = psi0 + dt/2 f(u1) on the block's inner tile      reads psi0        writes u2
+//   F34: u3 = psi0 + dt f(u2) on the whole block, and on the inner tile
+//        acc = psi0 + dt f(psi0)/6 + 2(u2 - psi0)/3 + dt f(u2)/3
+//        psi_new = acc + dt/6 f(u3)                           reads u2, psi0    writes psi_new
+//
+// (dt k2 = 2(u2 - psi0) recovers k2 from u2, k1 = f(psi0) is re-evaluated.)
+// The step moves 5 fields per cell instead of 13 and evaluates the
+// right-hand side about 5.9 times per cell instead of 4.
+//
+// Geometry: a block of TXB x TYB = (64 + 2H) x 15 threads owns one column
+// of cells each; its inner (64 x (15 - 2H)) tile is the output, the H-wide
+// rim is recomputed by the neighbouring blocks (overlapped tiling).  A block
+// marches its z-range [z0, z1): at iteration z it evaluates the first stage
+// at plane p = z + H over the whole block and the second stage at plane z
+// over the inner tile.  2H warm-up iterations before z0 fill the queues.
+//
+// Per thread (own column): register queues of the inputs for planes z..z+2H,
+// of the first stage's result for planes z-H..z+H, and (F34) of acc for the
+// inner planes z..z+H.  LDS: the first stage's input plane p with an H halo
+// (double buffered), F34's psi0 plane p (double buffered), and the first
+// stage's results for the planes z..z+H (a ring of H+1 planes, written when
+// computed and read H iterations later), so one barrier per iteration
+// orders every LDS reuse.
+//
+// Cells outside the domain are evaluated at their unfolded positions with
+// bcmap-mapped inputs.  With periodic boundaries that is the wrapped value;
+// with even reflection the axis stencils of the Laplacian are symmetric, so
+// the unfolded value equals the value at the mirrored index bit for bit.
+// This is why the optional terms (position-dependent V, d/dx, masks) are
+// excluded: the unfused kernels handle them.
+#pragma once
+
+#include "gpe_kernels.h"
+
+namespace gpe {
+
+constexpr int kFusedTYB = 15;
+
+template <int ORD> struct FusedShape {
+    static constexpr int H = ORD / 2;
+    static constexpr int TXB = 64 + 2 * H, TYB = kFusedTYB, NTB = TXB * TYB;
+    static constexpr int OX = 64, OY = TYB - 2 * H;       // output tile
+    static constexpr int LWA = TXB + 2 * H, LHA = TYB + 2 * H;  // LDS plane with halo
+    static constexpr int NHALO = 2 * H * TXB + 2 * H * TYB;
+    static constexpr int NSLOT = (NHALO + 1023) / 1024;
+};
+
+// Per-thread geometry of a fused block.
+template <int ORD> struct FusedCtx {
+    using S = FusedShape<ORD>;
+    int z0, z1;
+    bool active, inner, store;
+    int lx, ly, gx, gy;
+    unsigned col, own;
+    int la, lb;
+    unsigned hoff[S::NSLOT];
+    int hl[S::NSLOT];
+    bool hv[S::NSLOT];
+
+    __device__ __forceinline__ FusedCtx(const Geom& g, int bid) {
+        const int tid = threadIdx.x;
+        const int ntxy = g.ntx * g.nty;
+        const int txy = bid % ntxy, zc = bid / ntxy;
+        const int ox0 = (txy % g.ntx) * S::OX, oy0 = (txy / g.ntx) * S::OY;
+        z0 = zc * g.tz;
+        z1 = min(z0 + g.tz, g.nz);
+        active = tid < S::NTB;
+        lx = tid % S::TXB;
+        ly = tid / S::TXB;
+        if (!active) ly = 0;
+        gx = ox0 - S::H + lx;
+        gy = oy0 - S::H + ly;
+        inner = active && lx >= S::H && lx < S::TXB - S::H && ly >= S::H && ly < S::TYB - S::H;
+        store = inner && gx < g.nx && gy < g.ny;
+        col = (unsigned)bcmap(gx, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(gy, g.ny, g.bcs);
+        own = store ? (unsigned)gx + (unsigned)g.nx * (unsigned)gy : 0u;
+        la = (ly + S::H) * S::LWA + lx + S::H;
+        lb = ly * S::TXB + lx;
+        // halo of the block plane: 2H rows above/below, then 2H columns left/right
+#pragma unroll
+        for (int s = 0; s < S::NSLOT; ++s) {
+            const int e = tid + s * 1024;
+            hv[s] = e < S::NHALO;
+            int hx = ox0 - S::H, hy = oy0 - S::H, lxx = S::H, lyy = S::H;
+            if (e < 2 * S::H * S::TXB) {
+                const int r = e / S::TXB, c = e % S::TXB;
+                lxx = c + S::H;
+                lyy = r < S::H ? r : S::TYB + r;
+                hx = ox0 - S::H + c;
+                hy = oy0 - 2 * S::H + lyy;
+            } else if (e < S::NHALO) {
+                const int e2 = e - 2 * S::H * S::TXB;
+                const int r = e2 / (2 * S::H), c = e2 % (2 * S::H);
+                lyy = r + S::H;
+                lxx = c < S::H ? c : S::TXB + c;
+                hx = ox0 - 2 * S::H + lxx;
+                hy = oy0 - S::H + r;
+            }
+            hl[s] = lyy * S::LWA + lxx;
+            hoff[s] = (unsigned)bcmap(hx, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(hy, g.ny, g.bcs);
+        }
+    }
+};
+
+template <int ORD>
+__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
+                                                  double2* __restrict__ u2out, double2 hdt) {
+    using S = FusedShape<ORD>;
+    constexpr int H = S::H;
+    __shared__ double2 ldsA[2][S::LHA * S::LWA];
+    __shared__ double2 ldsB[H + 1][S::NTB];
+    const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
+    if (c.z0 >= c.z1) return;  // uniform per block
+    const int start = c.z0 - 2 * H;
+
+    double2 qp[2 * H + 1];  // psi0, planes z..z+2H
+    double2 qv[2 * H + 1];  // u1, planes z-H..z+H
+    double2 qn = mkc<double>(0.0, 0.0), hr[S::NSLOT];
+#pragma unroll
+    for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
+    if (c.active) {
+#pragma unroll
+        for (int d = 0; d < 2 * H; ++d) qp[d] = *(psi + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
+        qn = *(psi + Cell{g.plane * bcmap(start + 2 * H, g.nz, g.bcs), c.col});
+    }
+#pragma unroll
+    for (int s = 0; s < S::NSLOT; ++s)
+        if (c.hv[s]) hr[s] = *(psi + Cell{g.plane * bcmap(start + H, g.nz, g.bcs), c.hoff[s]});
+
+    for (int z = start; z < c.z1; ++z) {
+        const int p = z + H;
+        qp[2 * H] = qn;
+        double2* A = ldsA[(z - start) & 1];
+        if (c.active) A[c.la] = qp[H];
+#pragma unroll
+        for (int s = 0; s < S::NSLOT; ++s)
+            if (c.hv[s]) A[c.hl[s]] = hr[s];
+        __syncthreads();
+        if (z + 1 < c.z1) {
+            if (c.active) qn = *(psi + Cell{g.plane * bcmap(z + 1 + 2 * H, g.nz, g.bcs), c.col});
+#pragma unroll
+            for (int s = 0; s < S::NSLOT; ++s)
+                if (c.hv[s]) hr[s] = *(psi + Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), c.hoff[s]});
+        }
+        if (c.active) {
+            // first stage at plane p, whole block
+            Sten<double, ORD, S::LWA, false> sa;
+            sa.L = A;
+            sa.lc = c.la;
+#pragma unroll
+            for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qp[d];
+            sa.w = w;
+            const double2 u1 = cfma<double>(hdt, rhs_eval<double>(k, sa, 0, 0, 0), qp[H]);
+            qv[2 * H] = u1;
+            ldsB[(p - start) % (H + 1)][c.lb] = u1;
+        }
+        if (z >= c.z0 && c.inner) {
+            // second stage at plane z, inner tile (ldsB[z] was written H iterations ago)
+            Sten<double, ORD, S::TXB, false> sb;
+            sb.L = ldsB[(z - start) % (H + 1)];
+            sb.lc = c.lb;
+#pragma unroll
+            for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
+            sb.w = w;
+            const double2 u2 = cfma<double>(hdt, rhs_eval<double>(k, sb, 0, 0, 0), qp[0]);
+            if (c.store) st_stream<double>(u2out + Cell{g.plane * z, c.own}, u2);
+        }
+#pragma unroll
+        for (int d = 0; d < 2 * H; ++d) {
+            qp[d] = qp[d + 1];
+            qv[d] = qv[d + 1];
+        }
+    }
+}
+
+template <int ORD>
+__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ u2,
+                                                  const double2* __restrict__ psi, double2* __restrict__ out,
+                                                  double2 dt, double2 dt3, double2 dt6) {
+    using S = FusedShape<ORD>;
+    constexpr int H = S::H;
+    __shared__ double2 ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
+    __shared__ double2 ldsP[2][S::NTB];           // psi0 plane p
+    __shared__ double2 ldsB[H + 1][S::NTB];       // u3 planes
+    const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
+    if (c.z0 >= c.z1) return;
+    const int start = c.z0 - 2 * H;
+
+    double2 qu[2 * H + 1];  // u2, planes z..z+2H
+    double2 qp[2 * H + 1];  // psi0, planes z..z+2H
+    double2 qv[2 * H + 1];  // u3, planes z-H..z+H
+    double2 qa[H + 1];      // acc, planes z..z+H (inner tile)
+    double2 qnu = mkc<double>(0.0, 0.0), qnp = mkc<double>(0.0, 0.0), hr[S::NSLOT];
+#pragma unroll
+    for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d <= H; ++d) qa[d] = mkc<double>(0.0, 0.0);
+    if (c.active) {
+#pragma unroll
+        for (int d = 0; d < 2 * H; ++d) {
+            const long long zb = g.plane * bcmap(start + d, g.nz, g.bcs);
+            qu[d] = *(u2 + Cell{zb, c.col});
+            qp[d] = *(psi + Cell{zb, c.col});
+        }
+        const long long zb = g.plane * bcmap(start + 2 * H, g.nz, g.bcs);
+        qnu = *(u2 + Cell{zb, c.col});
+        qnp = *(psi + Cell{zb, c.col});
+    }
+#pragma unroll
+    for (int s = 0; s < S::NSLOT; ++s)
+        if (c.hv[s]) hr[s] = *(u2 + Cell{g.plane * bcmap(start + H, g.nz, g.bcs), c.hoff[s]});
+    constexpr double two3 = 2.0 / 3.0;
+
+    for (int z = start; z < c.z1; ++z) {
+        const int p = z + H;
+        qu[2 * H] = qnu;
+        qp[2 * H] = qnp;
+        double2* A = ldsA[(z - start) & 1];
+        double2* P = ldsP[(z - start) & 1];
+        if (c.active) {
+            A[c.la] = qu[H];
+            P[c.lb] = qp[H];
+        }
+#pragma unroll
+        for (int s = 0; s < S::NSLOT; ++s)
+            if (c.hv[s]) A[c.hl[s]] = hr[s];
+        __syncthreads();
+        if (z + 1 < c.z1) {
+            if (c.active) {
+                const long long zb = g.plane * bcmap(z + 1 + 2 * H, g.nz, g.bcs);
+                qnu = *(u2 + Cell{zb, c.col});
+                qnp = *(psi + Cell{zb, c.col});
+            }
+#pragma unroll
+            for (int s = 0; s < S::NSLOT; ++s)
+                if (c.hv[s]) hr[s] = *(u2 + Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), c.hoff[s]});
+        }
+        if (c.active) {
+            // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
+            Sten<double, ORD, S::LWA, false> sa;
+            sa.L = A;
+            sa.lc = c.la;
+#pragma unroll
+            for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[d];
+            sa.w = w;
+            const double2 f2 = rhs_eval<double>(k, sa, 0, 0, 0);
+            const double2 u3 = cfma<double>(dt, f2, qp[H]);
+            qv[2 * H] = u3;
+            ldsB[(p - start) % (H + 1)][c.lb] = u3;
+            if (c.inner) {
+                // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
+                Sten<double, ORD, S::TXB, false> sp;
+                sp.L = P;
+                sp.lc = c.lb;
+#pragma unroll
+                for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[d];
+                sp.w = w;
+                const double2 f1 = rhs_eval<double>(k, sp, 0, 0, 0);
+                const double2 p0 = qp[H], uu = qu[H];
+                const double2 base = mkc<double>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
+                qa[H] = cfma<double>(dt3, f2, cfma<double>(dt6, f1, base));
+            }
+        }
+        if (z >= c.z0 && c.inner) {
+            // fourth stage at plane z, inner tile
+            Sten<double, ORD, S::TXB, false> sb;
+            sb.L = ldsB[(z - start) % (H + 1)];
+            sb.lc = c.lb;
+#pragma unroll
+            for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
+            sb.w = w;
+            const double2 nw = cfma<double>(dt6, rhs_eval<double>(k, sb, 0, 0, 0), qa[0]);
+            if (c.store) st_stream<double>(out + Cell{g.plane * z, c.own}, nw);
+        }
+#pragma unroll
+        for (int d = 0; d < 2 * H; ++d) {
+            qu[d] = qu[d + 1];
+            qp[d] = qp[d + 1];
+            qv[d] = qv[d + 1];
+        }
+#pragma unroll
+        for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
+    }
+}
+
+// Host side (gpe_k_fused.hip).
+template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);
+template <int ORD>
+hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
+                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s);
+
+}  // namespace gpe

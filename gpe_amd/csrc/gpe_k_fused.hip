@@ -1,0 +1,55 @@
+// Fused RK4 stage pairs (complex128, plain right-hand side): launchers and
+// the z-range choice.
+#include "gpe_fused.h"
+
+namespace gpe {
+
+// Output tiles of 64 x (15 - 2H) cells; the z-range is split into C chunks
+// so that the block count fills whole waves of one block per CU, each chunk
+// paying 2H warm-up planes.  Returns the block count.
+template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
+    using S = FusedShape<ORD>;
+    g.ntx = (nx + S::OX - 1) / S::OX;
+    g.nty = (ny + S::OY - 1) / S::OY;
+    const long long tiles = (long long)g.ntx * g.nty;
+    double best = -1.0;
+    int best_tz = nz;
+    for (int cch = 1; cch <= 16 && cch <= nz; ++cch) {
+        const int tz = (nz + cch - 1) / cch;
+        const int nzc = (nz + tz - 1) / tz;
+        const long long b = tiles * nzc;
+        const long long waves = (b + ncu - 1) / ncu;
+        const double eff = (double)b / (double)(waves * ncu) * (double)tz / (double)(tz + 2 * S::H);
+        if (eff > best + 1e-9) {
+            best = eff;
+            best_tz = tz;
+        }
+    }
+    g.tz = best_tz;
+    g.nblocks = (int)(tiles * ((nz + best_tz - 1) / best_tz));
+    return g.nblocks;
+}
+
+template <int ORD>
+hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
+                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s) {
+    if (g.nblocks <= 0) return hipSuccess;
+    if (stage_pair == 12) {
+        const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
+        hipLaunchKernelGGL((k_rk4_f12<ORD>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+    } else {
+        const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
+        hipLaunchKernelGGL((k_rk4_f34<ORD>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const double2*)u2, psi, out,
+                           dt, dt3, dt6);
+    }
+    return hipGetLastError();
+}
+
+template int fused_tiles<2>(int, int, int, int, Geom&);
+template int fused_tiles<4>(int, int, int, int, Geom&);
+template hipError_t launch_rk4_fused<2>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
+                                        double2*, double2, int, hipStream_t);
+template hipError_t launch_rk4_fused<4>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
+                                        double2*, double2, int, hipStream_t);
+
+}  // namespace gpe

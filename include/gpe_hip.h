@@ -114,6 +114,14 @@ void gpe_ctx_destroy(gpe_ctx* ctx);
 int gpe_set_tiles(gpe_ctx* ctx, int tx, int ty, int tz);
 int gpe_get_tiles(const gpe_ctx* ctx, int* tx, int* ty, int* tz);
 
+/* Fused RK4 (on by default): for complex128 storage and a right-hand side
+ * without optional terms (no V, Urhs, dissipation or eqn-3 mask), gpe_step
+ * with GPE_RK4 runs the four stages as two kernels that each evaluate two
+ * stages (5 field accesses per cell instead of 13).  on = 0 selects the
+ * four stage kernels.  No counterpart in the reference: a performance switch
+ * of the same rk4 (src/solve.f90:109-154). */
+int gpe_set_fused(gpe_ctx* ctx, int on);
+
 /* Copy a host field into the device state.  The host array has extent
  * nx * ny_alloc * (any) and the interior starts at (0, y_off, z_off): pass
  * (ny, 0, 0) for an interior array and (ny+4, 2, 2) for the reference's
@@ -174,7 +182,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 25
+#define GPE_NKINDS 27
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

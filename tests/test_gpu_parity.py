@@ -258,3 +258,50 @@ def test_hamiltonian_closed_form_and_conservation():
             ctx.step("rk4", int(round(0.2 / dt)), StepInfo(complex(dt)))
             drifts.append(abs(ctx.hamiltonian() - h0) / abs(h0))
     assert drifts[0] < 1e-6 and drifts[1] < drifts[0] / 8
+
+
+@pytest.mark.parametrize("shape", [(24, 20, 18), (37, 21, 19), (70, 33, 12), (6, 6, 6)])
+@pytest.mark.parametrize("order,bcs,eqn", [(2, 1, 1), (4, 2, 1), (4, 1, 2), (2, 2, 2)])
+def test_fused_rk4_matches_oracle_and_stage_kernels(orc, shape, order, bcs, eqn):
+    """Fused RK4 (two kernels per step, overlapped tiles, unfolded boundary
+    cells) against the FP64 oracle (1e-12) and the four stage kernels (1e-13)."""
+    nx, ny, nz = shape
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=0.35 * nx, yr=0.35 * ny, zr=0.35 * nz, order=order, bcs=bcs,
+                  eqn_to_solve=eqn)
+    psi = smooth_field(p, seed=7)
+    dt = 0.01 + 0j
+    exp, t = orc.rk4_n(p, psi, dt, 6)
+    outs = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            info = StepInfo(dt)
+            ctx.step("rk4", 6, info)
+            outs[fused] = ctx.download()
+            assert abs(info.t - t[0]) < 1e-15
+    assert rel(outs[True], exp) < 1e-12
+    assert rel(outs[True], outs[False]) < 1e-13
+
+
+def test_fused_rk4_imaginary_time_and_fallback(orc):
+    """Complex dt through the fused kernels (eqn 2, imaginary time), and a
+    right-hand side with a trap (eqn 4), which falls back to the stage kernels."""
+    p = RunParams(nx=30, ny=26, nz=22, xr=8.0, yr=7.0, zr=6.0, order=4, bcs=2, eqn_to_solve=2, real_time=False,
+                  tau=0.004)
+    psi = smooth_field(p, seed=8)
+    exp, t = orc.rk4_n(p, psi, p.dt0(), 5)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        info = StepInfo(p.dt0())
+        ctx.step("rk4", 5, info)
+        got = ctx.download()
+        prof = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(p.dt0())))
+    assert rel(got, exp) < 1e-12
+    assert abs(info.im_t - t[1]) < 1e-15
+    assert set(prof) == {"rk4_fused12", "rk4_fused34"}
+    p4 = p.replace(eqn_to_solve=4, g=1.0, mu=1.0, omx=0.1, omy=0.1, omz=0.1, real_time=True)
+    with Context(p4) as ctx:
+        ctx.upload(psi)
+        prof4 = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(0.01 + 0j)))
+    assert "rk4_fused12" not in prof4 and "rk4_stage3" in prof4
