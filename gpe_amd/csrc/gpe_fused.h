@@ -21,11 +21,12 @@
 //
 // Per thread (own column): register queues of the inputs for planes z..z+2H,
 // of the first stage's result for planes z-H..z+H, and (F34) of acc for the
-// inner planes z..z+H.  LDS: the first stage's input plane p with an H halo
-// (double buffered), F34's psi0 plane p (double buffered), and the first
-// stage's results for the planes z..z+H (a ring of H+1 planes, written when
-// computed and read H iterations later), so one barrier per iteration
-// orders every LDS reuse.
+// inner planes z..z+H.  LDS: the first stage's input plane p with an H halo (double
+// buffered), F34's psi0 plane p (double buffered), and the first stage's
+// results in a ring of planes z..z+H, written when computed and read H
+// iterations later, so one barrier per iteration orders every LDS reuse.
+// F12 keeps the loads of the next GPE_FUSED_DEPTH planes in flight while a
+// plane is evaluated; F34 one plane.
 //
 // Cells outside the domain are evaluated at their unfolded positions with
 // bcmap-mapped inputs.  With periodic boundaries that is the wrapped value;
@@ -107,11 +108,36 @@ template <int ORD> struct FusedCtx {
     }
 };
 
+// Planes of loads in flight (1 or 2) in the fused kernels.
+#ifndef GPE_FUSED_DEPTH
+#define GPE_FUSED_DEPTH 2
+#endif
+
+// Runs step(z, F) for z = start .. z1-1 with DEPTH sets of in-flight loads:
+// fetch(F, z) issues the loads iteration z consumes, step(z, F) consumes
+// them and refills F for iteration z + DEPTH.
+template <int DEPTH, class Pf, class Fetch, class Step>
+__device__ __forceinline__ void fused_march(int start, int z1, Fetch fetch, Step step) {
+    if constexpr (DEPTH == 1) {
+        Pf F0{};
+        fetch(F0, start);
+        for (int z = start; z < z1; ++z) step(z, F0);
+    } else {
+        Pf F0{}, F1{};
+        fetch(F0, start);
+        if (start + 1 < z1) fetch(F1, start + 1);
+        for (int z = start; z < z1; z += 2) {
+            step(z, F0);
+            if (z + 1 < z1) step(z + 1, F1);
+        }
+    }
+}
+
 template <int ORD>
 __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
                                                   double2* __restrict__ u2out, double2 hdt) {
     using S = FusedShape<ORD>;
-    constexpr int H = S::H;
+    constexpr int H = S::H, D = GPE_FUSED_DEPTH;
     __shared__ double2 ldsA[2][S::LHA * S::LWA];
     __shared__ double2 ldsB[H + 1][S::NTB];
     const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
@@ -120,33 +146,33 @@ __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<do
 
     double2 qp[2 * H + 1];  // psi0, planes z..z+2H
     double2 qv[2 * H + 1];  // u1, planes z-H..z+H
-    double2 qn = mkc<double>(0.0, 0.0), hr[S::NSLOT];
 #pragma unroll
     for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
     if (c.active) {
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) qp[d] = *(psi + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
-        qn = *(psi + Cell{g.plane * bcmap(start + 2 * H, g.nz, g.bcs), c.col});
     }
+    // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H
+    struct Pf {
+        double2 qn;
+        double2 hr[S::NSLOT];
+    };
+    auto fetch = [&](Pf& F, int zz) {
+        if (c.active) F.qn = *(psi + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
 #pragma unroll
-    for (int s = 0; s < S::NSLOT; ++s)
-        if (c.hv[s]) hr[s] = *(psi + Cell{g.plane * bcmap(start + H, g.nz, g.bcs), c.hoff[s]});
-
-    for (int z = start; z < c.z1; ++z) {
+        for (int s = 0; s < S::NSLOT; ++s)
+            if (c.hv[s]) F.hr[s] = *(psi + Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), c.hoff[s]});
+    };
+    auto step = [&](int z, Pf& F) {
         const int p = z + H;
-        qp[2 * H] = qn;
+        qp[2 * H] = F.qn;
         double2* A = ldsA[(z - start) & 1];
         if (c.active) A[c.la] = qp[H];
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s)
-            if (c.hv[s]) A[c.hl[s]] = hr[s];
+            if (c.hv[s]) A[c.hl[s]] = F.hr[s];
         __syncthreads();
-        if (z + 1 < c.z1) {
-            if (c.active) qn = *(psi + Cell{g.plane * bcmap(z + 1 + 2 * H, g.nz, g.bcs), c.col});
-#pragma unroll
-            for (int s = 0; s < S::NSLOT; ++s)
-                if (c.hv[s]) hr[s] = *(psi + Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), c.hoff[s]});
-        }
+        if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // first stage at plane p, whole block
             Sten<double, ORD, S::LWA, false> sa;
@@ -160,7 +186,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<do
             ldsB[(p - start) % (H + 1)][c.lb] = u1;
         }
         if (z >= c.z0 && c.inner) {
-            // second stage at plane z, inner tile (ldsB[z] was written H iterations ago)
+            // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
             Sten<double, ORD, S::TXB, false> sb;
             sb.L = ldsB[(z - start) % (H + 1)];
             sb.lc = c.lb;
@@ -175,7 +201,8 @@ __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<do
             qp[d] = qp[d + 1];
             qv[d] = qv[d + 1];
         }
-    }
+    };
+    fused_march<D, Pf>(start, c.z1, fetch, step);
 }
 
 template <int ORD>
@@ -183,10 +210,12 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
                                                   const double2* __restrict__ psi, double2* __restrict__ out,
                                                   double2 dt, double2 dt3, double2 dt6) {
     using S = FusedShape<ORD>;
-    constexpr int H = S::H;
+    // one plane of loads in flight: a second set would not fit beside the
+    // three register queues without spilling
+    constexpr int H = S::H, D = 1, NB = H + 1;
     __shared__ double2 ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ double2 ldsP[2][S::NTB];           // psi0 plane p
-    __shared__ double2 ldsB[H + 1][S::NTB];       // u3 planes
+    __shared__ double2 ldsB[NB][S::NTB];          // u3 planes z..z+H
     const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
@@ -195,7 +224,6 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
     double2 qp[2 * H + 1];  // psi0, planes z..z+2H
     double2 qv[2 * H + 1];  // u3, planes z-H..z+H
     double2 qa[H + 1];      // acc, planes z..z+H (inner tile)
-    double2 qnu = mkc<double>(0.0, 0.0), qnp = mkc<double>(0.0, 0.0), hr[S::NSLOT];
 #pragma unroll
     for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
 #pragma unroll
@@ -207,19 +235,26 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
             qu[d] = *(u2 + Cell{zb, c.col});
             qp[d] = *(psi + Cell{zb, c.col});
         }
-        const long long zb = g.plane * bcmap(start + 2 * H, g.nz, g.bcs);
-        qnu = *(u2 + Cell{zb, c.col});
-        qnp = *(psi + Cell{zb, c.col});
     }
-#pragma unroll
-    for (int s = 0; s < S::NSLOT; ++s)
-        if (c.hv[s]) hr[s] = *(u2 + Cell{g.plane * bcmap(start + H, g.nz, g.bcs), c.hoff[s]});
     constexpr double two3 = 2.0 / 3.0;
-
-    for (int z = start; z < c.z1; ++z) {
+    struct Pf {
+        double2 qnu, qnp;
+        double2 hr[S::NSLOT];
+    };
+    auto fetch = [&](Pf& F, int zz) {
+        if (c.active) {
+            const long long zb = g.plane * bcmap(zz + 2 * H, g.nz, g.bcs);
+            F.qnu = *(u2 + Cell{zb, c.col});
+            F.qnp = *(psi + Cell{zb, c.col});
+        }
+#pragma unroll
+        for (int s = 0; s < S::NSLOT; ++s)
+            if (c.hv[s]) F.hr[s] = *(u2 + Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), c.hoff[s]});
+    };
+    auto step = [&](int z, Pf& F) {
         const int p = z + H;
-        qu[2 * H] = qnu;
-        qp[2 * H] = qnp;
+        qu[2 * H] = F.qnu;
+        qp[2 * H] = F.qnp;
         double2* A = ldsA[(z - start) & 1];
         double2* P = ldsP[(z - start) & 1];
         if (c.active) {
@@ -228,18 +263,9 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
         }
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s)
-            if (c.hv[s]) A[c.hl[s]] = hr[s];
+            if (c.hv[s]) A[c.hl[s]] = F.hr[s];
         __syncthreads();
-        if (z + 1 < c.z1) {
-            if (c.active) {
-                const long long zb = g.plane * bcmap(z + 1 + 2 * H, g.nz, g.bcs);
-                qnu = *(u2 + Cell{zb, c.col});
-                qnp = *(psi + Cell{zb, c.col});
-            }
-#pragma unroll
-            for (int s = 0; s < S::NSLOT; ++s)
-                if (c.hv[s]) hr[s] = *(u2 + Cell{g.plane * bcmap(z + 1 + H, g.nz, g.bcs), c.hoff[s]});
-        }
+        if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
             Sten<double, ORD, S::LWA, false> sa;
@@ -251,7 +277,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
             const double2 f2 = rhs_eval<double>(k, sa, 0, 0, 0);
             const double2 u3 = cfma<double>(dt, f2, qp[H]);
             qv[2 * H] = u3;
-            ldsB[(p - start) % (H + 1)][c.lb] = u3;
+            ldsB[(p - start) % NB][c.lb] = u3;
             if (c.inner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
                 Sten<double, ORD, S::TXB, false> sp;
@@ -267,9 +293,10 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
             }
         }
         if (z >= c.z0 && c.inner) {
-            // fourth stage at plane z, inner tile
+            // fourth stage at plane z, inner tile (ldsB of plane z was written
+            // H iterations ago)
             Sten<double, ORD, S::TXB, false> sb;
-            sb.L = ldsB[(z - start) % (H + 1)];
+            sb.L = ldsB[(z - start) % NB];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
@@ -285,7 +312,8 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
         }
 #pragma unroll
         for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
-    }
+    };
+    fused_march<D, Pf>(start, c.z1, fetch, step);
 }
 
 // Host side (gpe_k_fused.hip).
