@@ -12,7 +12,8 @@ timed region starts.
 Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
 dominant kernel (the one with the largest share of the step's time), timed
 live with HIP events around each launch on the context's stream, the
-complex64- and mixed-storage runs, and a CPU
+four-stage-kernel run (13 field accesses per cell, the bandwidth-bound
+formulation), the complex64- and mixed-storage runs, and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC).
 """
@@ -57,11 +58,11 @@ def load_traffic(name):
         return json.load(f)["traffic_bytes_per_launch"]
 
 
-def run_storage(p, psi, args, hip):
+def run_storage(p, psi, args, hip, fused=True):
     from gpe_amd import Context, StepInfo
 
     with Context(p) as ctx:
-        ctx.set_fused(not args.unfused)
+        ctx.set_fused(fused)
         if args.tiles:
             ctx.set_tiles(*[int(v) for v in args.tiles.split(",")])
         ctx.upload(psi)
@@ -115,6 +116,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
     ap.add_argument("--unfused", action="store_true", help="four stage kernels per RK4 step instead of two fused")
+    ap.add_argument("--no-unfused", action="store_true", help="skip the four-stage-kernel comparison run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one plain copy kernel of this size first (counter calibration)")
     args = ap.parse_args(argv)
@@ -133,7 +135,7 @@ def main(argv=None) -> int:
     if args.copy_probe_mib:
         copy_probe(hip, args.copy_probe_mib << 20)
 
-    ms, prof, tiles = run_storage(p, psi, args, hip)
+    ms, prof, tiles = run_storage(p, psi, args, hip, fused=not args.unfused)
     print(f"[bench] c128: {ms / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
     value = p.cells * args.steps / (ms * 1e-3) / 1e9
     step_bytes = sum(v["bytes"] for k, v in prof.items() if k.startswith("rk4"))
@@ -159,8 +161,23 @@ def main(argv=None) -> int:
                    "tiles": list(tiles)},
         "roofline": rf,
         "step_hbm_frac": round(step_bytes * args.steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        # the bytes a stage-by-stage RK4 must stream (13 field accesses per
+        # cell) over this step time, as a fraction of peak: above 1 means no
+        # stage-by-stage implementation could match the step at any bandwidth
+        "step_equivalent_hbm_frac": round(13 * 16 * p.cells * args.steps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels": kernel_table(prof),
     }
+    if not args.unfused and not args.no_unfused:
+        msu, profu, _ = run_storage(p, psi, args, hip, fused=False)
+        print(f"[bench] c128 four stage kernels: {msu / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+        bytes_u = sum(v["bytes"] for k, v in profu.items() if k.startswith("rk4"))
+        domu = dominant(profu)
+        result["unfused"] = {"value": round(p.cells * args.steps / (msu * 1e-3) / 1e9, 4),
+                             "ms_per_step": round(msu / args.steps, 4),
+                             "kernels_per_step": "4 stage kernels (13 field accesses per cell)",
+                             "roofline": roofline(profu, domu, load_traffic(f"traffic_{domu}_c128_{args.n}.json")),
+                             "step_hbm_frac": round(bytes_u * args.steps / (msu * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "kernels": kernel_table(profu)}
     if not args.no_c64:
         p64 = p.replace(storage="c64")
         ms64, prof64, _ = run_storage(p64, psi, args, hip)

@@ -39,10 +39,14 @@ ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3,
 MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
                "mx_rkck_stage1": 24, "mx_rkck_stage2": 40, "mx_rkck_stage3": 56, "mx_rkck_stage4": 64,
                "mx_rkck_stage5": 72, "mx_rkck_stage6_error": 72}
-DOMINANT = {"c128": "rk4_stage3", "c64": "rk4_stage3", "mixed": "mx_rk4_stage4"}
+FUSED_BYTES = {"rk4_fused12": 32, "rk4_fused34": 48}  # complex128 only
+# kernels whose calibrated traffic bench.py reads back (traffic_<kind>_<prec>_<n>.json)
+TRAFFIC_KINDS = {"rk4_fused12", "rk4_fused34", "rk4_stage3", "mx_rk4_stage4"}
 
 
 def cell_bytes(kind: str, prec: str) -> int:
+    if kind in FUSED_BYTES:
+        return FUSED_BYTES[kind]
     if prec == "mixed":
         return MIXED_BYTES.get(kind, 0)
     return ACCESS.get(kind, 0) * (16 if prec == "c128" else 8)
@@ -68,6 +72,10 @@ def kernel_label(name: str) -> str:
         if kind.startswith("mx_"):
             prec = "mixed"
         return f"{kind} {prec} order{order} tile{tx}x{ty}"
+    mf = re.search(r"k_rk4_f(12|34)<(\d)>", name)
+    if mf:
+        h = int(mf.group(2)) // 2
+        return f"rk4_fused{mf.group(1)} c128 order{mf.group(2)} tile{64 + 2 * h}x15"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
@@ -127,7 +135,7 @@ def main(argv=None):
     traffic = {}
     for r in stats:
         name = r["Name"]
-        if "k_stencil" not in name:
+        if "k_stencil" not in name and "k_rk4_f" not in name:
             continue
         label = kernel_label(name)
         kind = label.split()[0]
@@ -139,7 +147,7 @@ def main(argv=None):
         rd = fk * f_scale if fk else None
         wr = wk * w_scale if wk else None
         t = (rd + wr) if (rd and wr) else None
-        if t and a.workload == "rk4" and kind == DOMINANT.get(prec):
+        if t and a.workload == "rk4" and kind in TRAFFIC_KINDS:
             traffic[prec] = t
             json.dump({"traffic_bytes_per_launch": t, "read_bytes": rd, "write_bytes": wr,
                        "algorithmic_bytes_per_launch": alg, "fetch_scale": f_scale, "write_scale": w_scale,
