@@ -170,6 +170,31 @@ class Context:
         self._chk(self.lib.gpe_hamiltonian(self.h, ct.byref(v)), "gpe_hamiltonian")
         return v.value
 
+    def momentum(self) -> np.ndarray:
+        """variables::momentum (src/variables.f90:615-694): the 3 components."""
+        m = (ct.c_double * 3)()
+        self._chk(self.lib.gpe_momentum(self.h, m), "gpe_momentum")
+        return np.array(m[:])
+
+    def velocity_pdf(self, nbins: int = 256) -> dict:
+        """get_pdf_velocity + get_pdf (src/variables.f90:377-463): per component
+        the valid count, mean, stdev, vmax and the PDF over nbins bins (bin i
+        holds (2(i-1) vmax/nbins, 2 i vmax/nbins], i = -nbins/2+1 .. nbins/2).
+        The velocity is the reference's, -Im(psi) Re(d psi)/|psi|^2."""
+        cnt = (ct.c_int64 * 3)()
+        mean, sd, vmax = (ct.c_double * 3)(), (ct.c_double * 3)(), (ct.c_double * 3)()
+        pdf = np.zeros((3, nbins), dtype=np.float64)
+        self._chk(self.lib.gpe_velocity_pdf(self.h, nbins, cnt, mean, sd, vmax,
+                                            pdf.ctypes.data_as(ct.POINTER(ct.c_double))), "gpe_velocity_pdf")
+        return dict(counts=np.array(cnt[:]), vmean=np.array(mean[:]), vstdev=np.array(sd[:]),
+                    vmax=np.array(vmax[:]), pdf=pdf)
+
+    def vcf(self) -> np.ndarray:
+        """get_vcf (src/variables.f90:467-507): f(r), r = 0 .. nx-1."""
+        f = np.zeros(self.params.nx, dtype=np.float64)
+        self._chk(self.lib.gpe_vcf(self.h, f.ctypes.data_as(ct.POINTER(ct.c_double))), "gpe_vcf")
+        return f
+
     def mass(self) -> float:
         v = ct.c_double()
         self._chk(self.lib.gpe_mass(self.h, ct.byref(v)), "gpe_mass")

@@ -46,6 +46,10 @@ struct gpe_ctx {
     // mixed storage (GPE_MIXED): complex64 increments K1..K3 (RK4) or
     // F0, K2..K5, D_A, D_B (RKCK)
     void* mx[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // diagnostics: velocity components (3 n doubles), histogram counts
+    void* vel = nullptr;
+    void* hist = nullptr;
+    size_t hist_cap = 0;
     bool f0_valid = false;
     bool cand_valid = false;
     // drop-in scratch
@@ -79,7 +83,7 @@ static const char* const kKindNames[GPE_NKINDS] = {
     "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
     "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
     "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error",
-    "rk4_fused12", "rk4_fused34"};
+    "rk4_fused12", "rk4_fused34", "momentum", "velocity"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -739,7 +743,7 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     prof_clear(c);
     void* bufs[] = {c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
                     c->pnew,  c->sin,  c->sout, c->stage, c->partial, c->dres, c->dtab, c->mx[0], c->mx[1],
-                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6]};
+                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist};
     for (void* b : bufs) dfree(c, b);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -878,6 +882,118 @@ int gpe_energy(gpe_ctx* c, double* energy) {
     double d[3];
     spacing(c->p, d);
     *energy = s * d[0] * d[1] * d[2];  // E = E*dx*dy*dz (src/variables.f90:584)
+    return 0;
+}
+
+int gpe_momentum(gpe_ctx* c, double mom[3]) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_momentum: ctx is NULL");
+    if (!mom) return fail(c, GPE_EINVAL, "gpe_momentum: output is NULL");
+    int rc;
+    if (c->p.storage != GPE_C64) {
+        OpMomentum<double> o{c->p.nx, c->p.ny, c->p.nz};
+        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+    } else {
+        OpMomentum<float> o{c->p.nx, c->p.ny, c->p.nz};
+        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+    }
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double d[3];
+    spacing(c->p, d);
+    for (int a = 0; a < 3; ++a) mom[a] = c->hres[a] * d[0] * d[1] * d[2];  // integrate_x/y/z multiply by dx, dy, dz
+    return 0;
+}
+
+// Velocity field into c->vel (OpVelocity), masked as get_pdf_velocity or not.
+static int velocity_field(gpe_ctx* c, int masked, int ncomp) {
+    int rc;
+    if ((rc = dalloc(c, &c->vel, (size_t)c->n * 3 * sizeof(double), "velocity"))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    const double f = c->p.order == 2 ? 2.0 : 12.0;  // (2.0_pr*dx) or (12.0_pr*dx), src/derivs.f90:41-56
+    if (c->p.storage != GPE_C64) {
+        OpVelocity<double> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
+        return c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+    }
+    OpVelocity<float> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
+    return c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+}
+
+int gpe_velocity_pdf(gpe_ctx* c, int nbins, int64_t count[3], double mean[3], double stdev[3], double vmax[3],
+                     double* pdf) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_velocity_pdf: ctx is NULL");
+    if (!count || !mean || !stdev || !vmax || !pdf) return fail(c, GPE_EINVAL, "gpe_velocity_pdf: output is NULL");
+    if (nbins < 2 || nbins > 4096 || (nbins & 1)) return fail(c, GPE_EINVAL, "gpe_velocity_pdf: nbins must be even, 2..4096");
+    int rc;
+    if ((rc = velocity_field(c, 1, 3))) return rc;
+    const int nb = kNormBlocks;
+    if ((rc = ensure_partial(c, (size_t)15 * nb))) return rc;
+    HIPCHK(c, launch_vel_stats((const double*)c->vel, c->n, c->partial, nb, c->stream));
+    // c->dres holds 8 values: the first 8 sums now, the ninth with the maxima below
+    HIPCHK(c, launch_reduce_final(RED_SUM, c->partial, nb, 8, c->dres, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double sums[9];
+    for (int j = 0; j < 8; ++j) sums[j] = c->hres[j];
+    HIPCHK(c, launch_reduce_final(RED_SUM, c->partial + (size_t)8 * nb, nb, 1, c->dres, c->stream));
+    HIPCHK(c, launch_reduce_final(RED_MAX, c->partial + (size_t)9 * nb, nb, 6, c->dres + 1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    sums[8] = c->hres[0];
+    double vm[3];
+    for (int a = 0; a < 3; ++a) {
+        const double n = sums[3 * a], s = sums[3 * a + 1], s2 = sums[3 * a + 2];
+        count[a] = (int64_t)n;
+        mean[a] = s / n;                                  // mean (src/variables.f90:827-848)
+        stdev[a] = sqrt(s2 / n - mean[a] * mean[a]);      // stdev (src/variables.f90:852-875)
+        const double mx = c->hres[1 + 2 * a], mn = -c->hres[2 + 2 * a];
+        vm[a] = fabs(mx) > fabs(mn) ? fabs(mx) : fabs(mn);  // max_vel (src/variables.f90:441-442)
+        vmax[a] = vm[a];
+    }
+    // histogram with the reference's bin edges
+    const size_t hb = (size_t)3 * nbins * sizeof(unsigned long long);
+    if (c->hist_cap < hb) {
+        dfree(c, c->hist);
+        c->hist = nullptr;
+        if ((rc = dalloc(c, &c->hist, hb, "histogram"))) return rc;
+        c->hist_cap = hb;
+    }
+    for (int a = 0; a < 3; ++a) c->hres[a] = vm[a];  // pinned staging
+    HIPCHK(c, hipMemcpyAsync(c->dres, c->hres, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->hist, 0, hb, c->stream));
+    HIPCHK(c, launch_vel_hist((const double*)c->vel, c->n, nbins, c->dres, (unsigned long long*)c->hist, nb, c->stream));
+    std::vector<unsigned long long> h((size_t)3 * nbins);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->hist, hb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+        unsigned long long total = 0;
+        for (int j = 0; j < nbins; ++j) total += h[(size_t)a * nbins + j];
+        for (int j = 0; j < nbins; ++j) pdf[(size_t)a * nbins + j] = (double)h[(size_t)a * nbins + j] / (double)total;
+    }
+    return 0;
+}
+
+int gpe_vcf(gpe_ctx* c, double* f) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_vcf: ctx is NULL");
+    if (!f) return fail(c, GPE_EINVAL, "gpe_vcf: output is NULL");
+    const int nx = c->p.nx;
+    if (nx > 8192) return fail(c, GPE_EINVAL, "gpe_vcf: nx must be <= 8192");
+    int rc;
+    if ((rc = velocity_field(c, 0, 1))) return rc;
+    const long long nlines = (long long)c->p.ny * c->p.nz;
+    const int nb = (int)(nlines < 1024 ? nlines : 1024);
+    if ((rc = ensure_partial(c, (size_t)nb * nx + nx))) return rc;
+    double* out = c->partial + (size_t)nb * nx;
+    HIPCHK(c, launch_vcf((const double*)c->vel, nx, nlines, c->partial, nb, out, c->stream));
+    std::vector<double> q(nx);
+    HIPCHK(c, hipMemcpyAsync(q.data(), out, nx * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // Qxx(r) / (num(r) ny nz), num(r) = nx - r, then f = Qxx / Qxx(0) (src/variables.f90:501-502)
+    const double yz = (double)c->p.ny * (double)c->p.nz;
+    std::vector<double> qn(nx);
+    for (int r = 0; r < nx; ++r) qn[r] = q[r] / ((double)(nx - r) * yz);
+    for (int r = 0; r < nx; ++r) f[r] = qn[r] / qn[0];
     return 0;
 }
 

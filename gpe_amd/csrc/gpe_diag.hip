@@ -1,0 +1,150 @@
+// Diagnostics of the state: velocity statistics and PDF (get_pdf_velocity +
+// get_pdf, src/variables.f90:377-463) and the velocity correlation function
+// along x (get_vcf, src/variables.f90:467-507), from a velocity field that
+// OpVelocity wrote.  All sums are FP64 in a fixed order (per-block partials,
+// then one block); histogram counts are integers.
+#include <hip/hip_runtime.h>
+
+#include "gpe_launch.h"
+
+namespace gpe {
+
+constexpr int kDiagThreads = 256;
+constexpr int kVelValid = 500;  // the reference keeps values < 0.5e3 (marker 1e3)
+
+// Per block: for each component a, count/sum/sum of squares of the valid
+// values (partial rows 3a+0..2) and max / -min (rows 9+2a, 10+2a).
+__global__ __launch_bounds__(kDiagThreads) void k_vel_stats(const double* __restrict__ vel, long long n,
+                                                           double* __restrict__ partial) {
+    __shared__ double sh[kDiagThreads / 64];
+    const int nb = gridDim.x;
+    for (int a = 0; a < 3; ++a) {
+        const double* v = vel + a * n;
+        double cnt = 0.0, s = 0.0, s2 = 0.0, mx = -1e300, mn = -1e300;
+        for (long long i = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i < n; i += (long long)nb * kDiagThreads) {
+            const double x = v[i];
+            if (x < kVelValid) {
+                cnt += 1.0;
+                s += x;
+                s2 += x * x;
+                mx = x > mx ? x : mx;
+                mn = -x > mn ? -x : mn;
+            }
+        }
+        const double r0 = block_reduce<kDiagThreads, RED_SUM>(cnt, sh);
+        const double r1 = block_reduce<kDiagThreads, RED_SUM>(s, sh);
+        const double r2 = block_reduce<kDiagThreads, RED_SUM>(s2, sh);
+        const double r3 = block_reduce<kDiagThreads, RED_MAX>(mx, sh);
+        const double r4 = block_reduce<kDiagThreads, RED_MAX>(mn, sh);
+        if (threadIdx.x == 0) {
+            partial[(long long)(3 * a + 0) * nb + blockIdx.x] = r0;
+            partial[(long long)(3 * a + 1) * nb + blockIdx.x] = r1;
+            partial[(long long)(3 * a + 2) * nb + blockIdx.x] = r2;
+            partial[(long long)(9 + 2 * a) * nb + blockIdx.x] = r3;
+            partial[(long long)(10 + 2 * a) * nb + blockIdx.x] = r4;
+        }
+    }
+}
+
+// Bin edges of get_pdf (src/variables.f90:448-451): bin i in
+// (-nbins/2, nbins/2] holds (thr(i-1), thr(i)], thr(i) = ((2 i) vmax) / nbins,
+// evaluated in that order so the edges equal the reference's.
+__device__ __forceinline__ double pdf_edge(int i, double vmax, int nbins) {
+    return (2.0 * (double)i) * vmax / (double)nbins;
+}
+
+// Histogram of the valid values per component, counted in LDS per block and
+// added into hist[a * nbins + (i + nbins/2 - 1)].
+__global__ __launch_bounds__(kDiagThreads) void k_vel_hist(const double* __restrict__ vel, long long n, int nbins,
+                                                          const double* __restrict__ vmax3,
+                                                          unsigned long long* __restrict__ hist) {
+    extern __shared__ unsigned int hsh[];  // 3 * nbins
+    for (int j = threadIdx.x; j < 3 * nbins; j += kDiagThreads) hsh[j] = 0u;
+    __syncthreads();
+    const int lo = -nbins / 2 + 1, hi = nbins / 2;
+    for (int a = 0; a < 3; ++a) {
+        const double* v = vel + a * n;
+        const double vmax = vmax3[a];
+        for (long long i = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i < n;
+             i += (long long)gridDim.x * kDiagThreads) {
+            const double x = v[i];
+            if (!(x < kVelValid) || !(vmax > 0.0)) continue;
+            double q = ceil(x * (double)nbins / (2.0 * vmax));
+            q = q < lo - 1 ? lo - 1 : (q > hi + 1 ? hi + 1 : q);
+            int b = (int)q;
+            // settle on the bin whose edges (as the reference forms them) hold x
+            while (b <= hi && x > pdf_edge(b, vmax, nbins)) ++b;
+            while (b >= lo && x <= pdf_edge(b - 1, vmax, nbins)) --b;
+            if (b >= lo && b <= hi) atomicAdd(&hsh[a * nbins + (b - lo)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < 3 * nbins; j += kDiagThreads)
+        if (hsh[j]) atomicAdd(&hist[j], (unsigned long long)hsh[j]);
+}
+
+// Velocity correlation along x: block b sums, over its lines (y, z), the
+// products v(i) v(i+r) for every r into partial[b * nx + r].
+__global__ __launch_bounds__(kDiagThreads) void k_vcf_partial(const double* __restrict__ vx, int nx, long long nlines,
+                                                             double* __restrict__ partial) {
+    extern __shared__ double line[];  // nx
+    const long long per = (nlines + gridDim.x - 1) / gridDim.x;
+    const long long l0 = (long long)blockIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
+    // each thread owns the lags r = tid, tid + blockDim, ...; at most 32 of them
+    constexpr int kMaxLags = 32;
+    double acc[kMaxLags];
+#pragma unroll
+    for (int q = 0; q < kMaxLags; ++q) acc[q] = 0.0;
+    for (long long l = l0; l < l1; ++l) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < nx; i += kDiagThreads) line[i] = vx[l * nx + i];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kMaxLags; ++q) {
+            const int r = threadIdx.x + q * kDiagThreads;
+            if (r < nx) {
+                double s = 0.0;
+                for (int i = 0; i + r < nx; ++i) s += line[i] * line[i + r];
+                acc[q] += s;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxLags; ++q) {
+        const int r = threadIdx.x + q * kDiagThreads;
+        if (r < nx) partial[(long long)blockIdx.x * nx + r] = acc[q];
+    }
+}
+
+// out[r] = sum over blocks (fixed order) of partial[b * nx + r].
+__global__ __launch_bounds__(kDiagThreads) void k_vcf_final(const double* __restrict__ partial, int nx, int nb,
+                                                           double* __restrict__ out) {
+    const int r = blockIdx.x * kDiagThreads + threadIdx.x;
+    if (r >= nx) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[(long long)b * nx + r];
+    out[r] = s;
+}
+
+hipError_t launch_vel_stats(const double* vel, long long n, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_vel_stats, dim3(nblocks), dim3(kDiagThreads), 0, s, vel, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,
+                           int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_vel_hist, dim3(nblocks), dim3(kDiagThreads), (size_t)3 * nbins * sizeof(unsigned int), s, vel,
+                       n, nbins, vmax3, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_vcf_partial, dim3(nblocks), dim3(kDiagThreads), (size_t)nx * sizeof(double), s, vx, nx, nlines,
+                       partial);
+    hipLaunchKernelGGL(k_vcf_final, dim3((nx + kDiagThreads - 1) / kDiagThreads), dim3(kDiagThreads), 0, s, partial, nx,
+                       nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace gpe

@@ -187,7 +187,7 @@ __device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w,
     Aux qa[2 * H + 1];
 #pragma unroll
     for (int d = 0; d < 2 * H; ++d) q[d] = src(Cell{g.plane * bcmap(z0 - H + d, g.nz, g.bcs), col}, qa[d]);
-    double red[2] = {0.0, 0.0};
+    double red[Op::kNRed > 2 ? Op::kNRed : 2] = {};
 
     // The loads for plane z: centre of plane z+H (the window's new entry),
     // the halo of plane z and the operator's pointwise inputs.
@@ -595,6 +595,89 @@ template <typename R> struct OpHamiltonian {
         }
         const double re_pl = (double)L.x * pr + (double)L.y * pi;
         red[0] += -sgn * ((double)k.a * re_pl + ((double)k.b + v) * a2 + 0.5 * (double)k.c * a2 * a2);
+    }
+};
+
+// End-corrected quadrature weight of index i on an axis of n points
+// (src/variables.f90:49-51, 713-823): 3/8, 7/6, 23/24 at both ends.
+__device__ __forceinline__ double qweight_axis(int i, int n) {
+    const int e = i < n - 1 - i ? i : n - 1 - i;
+    return e == 0 ? 3.0 / 8.0 : e == 1 ? 7.0 / 6.0 : e == 2 ? 23.0 / 24.0 : 1.0;
+}
+
+// Momentum (src/variables.f90:615-694): per axis a, the end-corrected
+// quadrature of (Re psi - 1) Im(d_a psi*) - Im psi Re(d_a psi), with
+// d_a psi* = conj(d_a psi) (the derivative weights are real).  FP64 sums.
+template <typename R> struct OpMomentum {
+    static constexpr int kKind = 27, kAccess = 1;
+    static constexpr int kNRed = 3, kMode = RED_SUM;
+    using Pre = NoPre;
+    int nx, ny, nz;
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell, int x, int y, int z, const S& st, const Pre&, double* red) const {
+        const C<R> p = st.c();
+        const double w = qweight_axis(x, nx) * qweight_axis(y, ny) * qweight_axis(z, nz);
+        const double pr = (double)p.x - 1.0, pi = p.y;
+        const C<R> d[3] = {st.d1x(), st.d1y(), st.d1z()};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) red[a] += w * (pr * (-(double)d[a].y) - pi * (double)d[a].x);
+    }
+};
+
+// First derivative in the reference's operation order (src/derivs.f90:25-142):
+// order 2: (f+1 - f-1) / (2 d); order 4: (((-f+2 + 8 f+1) - 8 f-1) + f-2) / (12 d),
+// in FP64 without contraction, so that equal neighbours cancel exactly where
+// they cancel in the reference (the sign of a near-zero velocity decides its
+// PDF bin).  axis 0..2; den = 2 d or 12 d.
+template <typename R, int H, class S>
+__device__ __forceinline__ double2 d1_ref(const S& st, int axis, double den) {
+#pragma clang fp contract(off)
+    auto nb = [&](int o) -> double2 {
+        const C<R> v = axis == 0 ? st.ax(o) : axis == 1 ? st.ay(o) : st.q[H + o];
+        return mkc<double>(v.x, v.y);
+    };
+    const double2 p1 = nb(1), m1 = nb(-1);
+    if constexpr (H == 1) {
+        return mkc<double>((p1.x - m1.x) / den, (p1.y - m1.y) / den);
+    } else {
+        const double2 p2 = nb(2), m2 = nb(-2);
+        return mkc<double>((((-p2.x + 8.0 * p1.x) - 8.0 * m1.x) + m2.x) / den,
+                           (((-p2.y + 8.0 * p1.y) - 8.0 * m1.y) + m2.y) / den);
+    }
+}
+
+// The velocity of get_pdf_velocity and get_vcf (src/variables.f90:393-400,
+// 478-483): -i/2 (psi* g - psi g*) / |psi|^2 where the reference's gradient
+// arrays g, g* are declared real (lines 387, 474) and so hold Re(d_a psi);
+// the expression reduces exactly to -Im(psi) Re(d_a psi) / |psi|^2.
+// masked: cells with |psi|^2 <= 0.01 get the reference's marker 1e3 (as
+// get_pdf_velocity); otherwise every cell is divided (get_vcf).  ncomp = 3
+// writes v_x, v_y, v_z, 1 only v_x.
+template <typename R> struct OpVelocity {
+    static constexpr int kKind = 28, kBytesPerCell = (int)sizeof(C<R>) + 24;
+    static constexpr int kNRed = 0, kMode = RED_NONE;
+    using Pre = NoPre;
+    double* __restrict__ vel;  // component a at vel + a * n
+    long long n;
+    int masked, ncomp;
+    double den[3];             // 2 d or 12 d per axis
+    __device__ Pre load(Cell) const { return {}; }
+    template <class S> __device__ void apply(Cell i, int, int, int, const S& st, const Pre&, double*) const {
+#pragma clang fp contract(off)
+        const C<R> p = st.c();
+        const double pr = p.x, pi = p.y;
+        const double ab = hypot(pr, pi);  // abs(in_var)**2 as the reference forms it
+        const double a2 = ab * ab;
+        const bool ok = !masked || a2 > 0.01;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (a < ncomp) {
+                const double g = d1_ref<R, S::H>(st, a, den[a]).x;  // the real gradient array
+                const double v = ok ? -(pi * g) / a2 : 1e3;
+                double* dst = vel + a * n;
+                dst[i.zb + i.own] = v;
+            }
+        }
     }
 };
 

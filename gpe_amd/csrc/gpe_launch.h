@@ -52,7 +52,9 @@ hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u
     X(R, OpDeriv<R>)                                                                                               \
     X(R, OpRhs<R>)                                                                                                 \
     X(R, OpEnergy<R>)                                                                                              \
-    X(R, OpHamiltonian<R>)
+    X(R, OpHamiltonian<R>)                                                                                         \
+    X(R, OpMomentum<R>)                                                                                            \
+    X(R, OpVelocity<R>)
 
 // Mixed-storage operators: those over a plain complex128 stencil use
 // launch_stencil<double>, the others launch_stencil_sum.
@@ -92,6 +94,13 @@ template <typename R> hipError_t launch_to_c64(const C<R>* src, float2* dst, lon
 hipError_t launch_copy16(void* dst, const void* src, long long n16, hipStream_t s);
 hipError_t launch_stream5(void* c, void* d, const void* a, const void* b, const void* e, long long n16, hipStream_t s);
 hipError_t launch_reduce_final(int mode, const double* partial, int n, int nval, double* out, hipStream_t s);
+
+// Diagnostics (gpe_diag.hip).
+hipError_t launch_vel_stats(const double* vel, long long n, double* partial, int nblocks, hipStream_t s);
+hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,
+                           int nblocks, hipStream_t s);
+hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,
+                      hipStream_t s);
 
 constexpr int kNormBlocks = 2048;
 

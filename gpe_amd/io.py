@@ -31,7 +31,7 @@ def read_end_state(path: str, n_extra: int = 0):
     return out
 
 
-def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10) -> str:
+def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10, nbins: int = 256) -> str:
     """run.in with the reference's four namelists (src/io.f90:78-114) plus
     grid_params for the Fortran host of this package."""
     def b(v):
@@ -87,7 +87,7 @@ def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10) -
   save_zeros = .false.
 /
 &misc_params
-  nbins = 256
+  nbins = {nbins}
   diagnostic = .false.
   pp_filtered_surface = .false.
   nlines = 3

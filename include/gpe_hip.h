@@ -167,6 +167,29 @@ int gpe_rhs(gpe_ctx* ctx, const void* in, int64_t ny_alloc, int64_t y_off, int64
 int gpe_norm(gpe_ctx* ctx, double* norm);
 int gpe_energy(gpe_ctx* ctx, double* energy);
 int gpe_mass(gpe_ctx* ctx, double* mass);
+/* Momentum (variables::momentum, src/variables.f90:615-694): mom[a] is the
+ * end-corrected quadrature (weights as get_norm) of
+ * (Re psi - 1) Im(d_a psi*) - Im psi Re(d_a psi), a = x, y, z. */
+int gpe_momentum(gpe_ctx* ctx, double mom[3]);
+
+/* Velocity statistics and PDF (get_pdf_velocity + get_pdf,
+ * src/variables.f90:377-463, as save_velocity_pdf uses them,
+ * src/io.f90:222-285): the reference's velocity -i/2 (psi* g - psi g*)/|psi|^2
+ * with real gradient arrays g = g* = Re(d_a psi), i.e. -Im(psi) Re(d_a psi)/|psi|^2,
+ * where |psi|^2 > 0.01.
+ * Per component a: count[a] valid cells, mean[a], stdev[a] =
+ * sqrt(<v^2> - mean^2), vmax[a] = max |v|, and pdf[a*nbins + j], the fraction
+ * of the binned values in bin i = j - nbins/2 + 1, (2(i-1) vmax/nbins,
+ * 2 i vmax/nbins].  nbins even, 2..4096.  Uses 24 bytes per cell of device
+ * scratch. */
+int gpe_velocity_pdf(gpe_ctx* ctx, int nbins, int64_t count[3], double mean[3], double stdev[3], double vmax[3],
+                     double* pdf);
+
+/* Velocity correlation function along x (get_vcf, src/variables.f90:467-507):
+ * f[r] = Q(r)/Q(0), Q(r) = mean over (i, y, z), i + r < nx, of v_x(i) v_x(i+r),
+ * with the same velocity, unmasked.  nx <= 8192. */
+int gpe_vcf(gpe_ctx* ctx, double* f);
+
 /* Conserved energy of the equation solved (eqn 1: sum |grad psi|^2 - |psi|^2
  * + |psi|^4/2; eqn 4: sum |grad psi|^2/2 + V|psi|^2 + g|psi|^4/2 - mu|psi|^2,
  * times dx dy dz), for drift reports.  No reference counterpart (the
@@ -182,7 +205,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 27
+#define GPE_NKINDS 29
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

@@ -6,10 +6,11 @@
 # holds no parameters.in, so the compile-time configuration written here is
 # found on the include path, as the reference's own setup script intends).
 # mpif.h and the MPI library are the image's MPICH (/opt/conda).  The
-# driver is oracle/ref/ref_driver.f90 (this project's harness).  Unused
+# drivers are oracle/ref/ref_driver.f90 and oracle/ref/ref_diag.f90 (this
+# project's harnesses).  Unused
 # code (FFT diagnostics, IC generators) is dropped at link time with
 # --gc-sections, so the absent FFTW2 library is never referenced.
-# Outputs go to oracle/_ref/<nx>x<ny>x<nz>/ref_driver only.
+# Outputs go to oracle/_ref/<nx>x<ny>x<nz>/{ref_driver,ref_diag} only.
 set -euo pipefail
 REF=${GPE_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -57,6 +58,11 @@ PIN
   $FC -O2 -flto -o "$dir/ref_driver" "$dir"/{parameters,constants,error,derivs,ic,variables,solve,ref_driver}.o \
       -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -flto -fuse-ld=lld
   echo "built $dir/ref_driver"
+  # diagnostics harness (momentum, velocity PDF, velocity correlation)
+  $FC $flags -c "$HERE/ref/ref_diag.f90" -o "$dir/ref_diag.o"
+  $FC -O2 -flto -o "$dir/ref_diag" "$dir"/{parameters,constants,error,derivs,ic,variables,ref_diag}.o \
+      -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -flto -fuse-ld=lld
+  echo "built $dir/ref_diag"
 
   # Drop-in check: the same harness and the reference's own parameters,
   # constants, error, ic and variables modules, with solve and derivs

@@ -1,8 +1,10 @@
 """Writes tests/golden/ref_*.npz: outputs of the reference Fortran
 time-stepping path (compiled unmodified by oracle/build_ref.sh into
-oracle/_ref) on seeded inputs.  Each fixture holds the input field, the run
-parameters, the reference's final field, t, im_t, dt, p, its norm and energy
-(get_norm, energy) and the per-step (t, im_t, dt) trace.
+oracle/_ref) on seeded inputs.  Each stepping fixture holds the input field,
+the run parameters, the reference's final field, t, im_t, dt, p, its norm
+and energy (get_norm, energy) and the per-step (t, im_t, dt) trace.  The
+ref_diag_* fixtures hold an input field and the reference's momentum,
+velocity statistics and PDF (nbins = 64) and velocity correlation function.
 
 Run in the build container (the reference is not present on the GPU box):
     python tests/golden/make_ref_fixtures.py
@@ -102,6 +104,53 @@ def small_cases():
     save("eq4_ref_o2_rk45", p, psi, run_reference("eq4_ref_o2_rk45", p, psi, end_time=0.5))
 
 
+def run_reference_diag(name, p, psi, nbins):
+    """The reference's momentum, velocity PDF and velocity correlation
+    function of psi (oracle/ref/ref_diag.f90)."""
+    exe = os.path.join(REF, f"{p.nx}x{p.ny}x{p.nz}", "ref_diag")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} missing: run oracle/build_ref.sh first")
+    run = os.path.join(REF, "run_" + name)
+    shutil.rmtree(run, ignore_errors=True)
+    os.makedirs(run)
+    with open(os.path.join(run, "run.in"), "w") as f:
+        f.write(run_in_text(p, 0.0, nbins=nbins))
+    write_end_state(os.path.join(run, "in.dat"), psi, 0, 0.0, 0.0, p.dt0())
+    subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {exe} in.dat out.dat"], cwd=run, check=True,
+                   env=dict(os.environ, OMP_NUM_THREADS="1"))
+    raw = open(os.path.join(run, "out.dat"), "rb").read()
+    shutil.rmtree(run)
+    o = 0
+    head = np.frombuffer(raw, "<f8", 12, o); o += 96
+    counts = np.frombuffer(raw, "<i4", 3, o); o += 12
+    pdf = np.frombuffer(raw, "<f8", 3 * nbins, o).reshape(3, nbins); o += 24 * nbins
+    vcf = np.frombuffer(raw, "<f8", p.nx, o); o += 8 * p.nx
+    assert o == len(raw), (o, len(raw))
+    print(f"{name}: momentum {head[:3]}, valid {counts}", flush=True)
+    return dict(mom=head[0:3].copy(), vmean=head[3:6].copy(), vstdev=head[6:9].copy(), vmax=head[9:12].copy(),
+                counts=counts.copy(), pdf=pdf.copy(), vcf=vcf.copy())
+
+
+def diag_cases():
+    base = dict(nx=24, ny=20, nz=18, xr=8.0, yr=7.0, zr=6.5)
+    nbins = 64
+    cases = []
+    p = RunParams(**base, order=2, bcs=1, eqn_to_solve=1)
+    cases.append(("diag_per_o2_smooth", p, smooth_field(p, 11)))
+    p = RunParams(**base, order=4, bcs=2, eqn_to_solve=4, g=1.0, mu=2.0, omx=0.3, omy=0.25, omz=0.2)
+    cases.append(("diag_ref_o4_trap_line", p, np.ascontiguousarray(ics.fermi(p) * ics.vortex_line_z(p) + 0.01)))
+    p = RunParams(nx=24, ny=20, nz=18, xr=0.15 * 23, yr=0.15 * 19, zr=0.15 * 17, order=4, bcs=1, eqn_to_solve=1)
+    cases.append(("diag_per_o4_turb", p, ics.random_phase(p, seed=3)))
+    p, psi = ics.config_ring64(seed=0)
+    cases.append(("diag_ring64", p, psi))
+    for name, p, psi in cases:
+        res = run_reference_diag(name, p, psi, nbins)
+        # the 64^3 ring input is regenerated by ics.config_ring64(seed=0)
+        extra = {} if name == "diag_ring64" else dict(inp=np.ascontiguousarray(psi))
+        np.savez_compressed(os.path.join(OUT, f"ref_{name}.npz"), params=json.dumps(dataclasses.asdict(p)),
+                            nbins=nbins, **extra, **res)
+
+
 def ring64():
     p, psi = ics.config_ring64(seed=0)
     res = run_reference("ring64_rk4_200", p, psi, nsteps=200)
@@ -109,8 +158,10 @@ def ring64():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "ring64"]
+    which = sys.argv[1:] or ["small", "ring64", "diag"]
     if "small" in which:
         small_cases()
     if "ring64" in which:
         ring64()
+    if "diag" in which:
+        diag_cases()
