@@ -114,6 +114,28 @@ module gpe_hip_c
       real(c_double), intent(out) :: v
     end function gpe_energy
 
+    integer(c_int) function gpe_momentum(ctx, mom) bind(C, name='gpe_momentum')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: mom(3)
+    end function gpe_momentum
+
+    integer(c_int) function gpe_velocity_pdf(ctx, nbins, cnt, vmean, vstdev, vmax, pdf) &
+        bind(C, name='gpe_velocity_pdf')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbins
+      integer(c_int64_t), intent(out) :: cnt(3)
+      real(c_double), intent(out) :: vmean(3), vstdev(3), vmax(3)
+      real(c_double), intent(out) :: pdf(*)
+    end function gpe_velocity_pdf
+
+    integer(c_int) function gpe_vcf(ctx, f) bind(C, name='gpe_vcf')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: f(*)
+    end function gpe_vcf
+
     integer(c_int) function gpe_mass(ctx, v) bind(C, name='gpe_mass')
       import
       type(c_ptr), value :: ctx

@@ -4,16 +4,22 @@
 ! state read from end_state00.dat in the reference's restart format
 ! (src/ic.f90:117-186), the main loop `do while (t+im_t <= end_time)` with
 ! the RUNNING sentinel (134-159), the eqn-4 norm / imaginary-time logic
-! (241-271) and the final end_state.dat (src/io.f90:840-891).
+! (241-271) and the final end_state.dat (src/io.f90:840-891).  With
+! save_pdf / save_vcf (io_params) or save_mom (grid_params) it writes the
+! final state's velocity PDF, velocity correlation function and momentum in
+! the formats of save_velocity_pdf, save_vel_corr and save_momentum
+! (src/io.f90:222-311, 426-445) to pdf_vel<p>.dat, vcf<p>.dat (p as seven
+! digits, as itos forms it) and momentum.dat, from the device diagnostics.
 !
 ! host_mode = 'parity' copies psi%new -> psi%old and calls
 ! solver(psi%old, psi%new) every step exactly as the reference does;
 ! host_mode = 'resident' keeps psi on the GPU for the whole run.
 program gpe_hip_main
-  use iso_c_binding, only : c_double
+  use iso_c_binding, only : c_double, c_int, c_int64_t
   use parameters
   use error
-  use gpe_context, only : close_ctx, gpe_storage, gpe_vscale
+  use gpe_context, only : close_ctx, gpe_storage, gpe_vscale, the_ctx, check
+  use gpe_hip_c, only : gpe_momentum, gpe_velocity_pdf, gpe_vcf
   use solve
   use variables
   implicit none
@@ -35,6 +41,8 @@ program gpe_hip_main
   if (ios /= 0) call emergency_stop('ERROR: cannot read namelist eqn_params')
   rewind (u)
   read (u, nml=io_params, iostat=ios)
+  rewind (u)
+  read (u, nml=misc_params, iostat=ios)
   rewind (u)
   read (u, nml=grid_params, iostat=ios)
   close (u)
@@ -163,6 +171,8 @@ program gpe_hip_main
   write (u) psi%new
   close (u)
 
+  if (save_pdf .or. save_vcf .or. save_mom) call final_diagnostics()
+
   inquire (file='RUNNING', exist=running)
   if (running) then
     open (newunit=u, file='RUNNING')
@@ -171,4 +181,52 @@ program gpe_hip_main
   write (*, '(a,i0,a,es12.5,a,es12.5,a,f10.4,a)') 'steps=', nsteps, ' t=', t, ' im_t=', im_t, ' wall=', secs, ' s'
   if (secs > 0) write (*, '(a,f12.4)') 'Gcell-steps/s=', real(nx,pr)*ny*nz*nsteps/secs/1e9_pr
   call close_ctx()
+
+contains
+
+  ! The device diagnostics of the final state (state_put makes it the
+  ! context's state in either host mode).
+  subroutine final_diagnostics()
+    real(c_double) :: mom(3), vmean(3), vstdev(3), vmax(3), f(0:nx-1)
+    real(c_double), allocatable :: pdf(:,:)
+    integer(c_int64_t) :: cnt(3)
+    real (pr) :: bins(3), g(3)
+    integer :: i, a
+    character(7) :: tag
+    write (tag, '(i7.7)') p
+    call state_put(psi%new)
+    if (save_mom) then
+      call check(gpe_momentum(the_ctx(), mom), 'momentum')
+      open (newunit=u, file='momentum.dat', status='unknown', position='append')
+      write (u, '(4e17.9)') t, mom(1), mom(2), mom(3)
+      close (u)
+    end if
+    if (save_pdf) then
+      allocate(pdf(nbins, 3))
+      call check(gpe_velocity_pdf(the_ctx(), int(nbins, c_int), cnt, vmean, vstdev, vmax, pdf), 'velocity_pdf')
+      open (newunit=u, file='pdf_vel'//tag//'.dat', status='unknown')
+      write (u, '(a10,e17.9)') '# t:', t
+      write (u, '(a10,3e17.9)') '# Mean:', vmean(1), vmean(2), vmean(3)
+      write (u, '(a10,3e17.9)') '# StDev:', vstdev(1), vstdev(2), vstdev(3)
+      write (u, '(a10,3e17.9)') '# Sum:', sum(pdf(:, 1)), sum(pdf(:, 2)), sum(pdf(:, 3))
+      do i = -nbins/2+1, nbins/2
+        do a = 1, 3
+          bins(a) = 2.0_pr*real(i, pr)*vmax(a)/real(nbins, pr)
+          g(a) = 1.0_pr/(vstdev(a)*sqrt(2.0_pr*pi)) * exp((-0.5_pr*(bins(a)-vmean(a))**2) / vstdev(a)**2)
+        end do
+        write (u, '(9e17.9)') (bins(a), pdf(i+nbins/2, a), g(a), a = 1, 3)
+      end do
+      close (u)
+    end if
+    if (save_vcf) then
+      call check(gpe_vcf(the_ctx(), f), 'get_vcf')
+      open (newunit=u, file='vcf'//tag//'.dat', status='unknown')
+      write (u, '(a4,e17.9)') '# t:', t
+      do i = 0, nx-1
+        write (u, '(i5,e17.9)') i, f(i)
+      end do
+      close (u)
+    end if
+  end subroutine final_diagnostics
+
 end program gpe_hip_main

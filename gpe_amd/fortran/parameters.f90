@@ -55,14 +55,24 @@ module parameters
     save_contour, save_3d, save_filter, filter_kc, save_average, &
     save_spectrum, save_pdf, save_vcf, save_ll, save_zeros
 
+  ! misc_params subset: the velocity-PDF bin count (src/parameters.f90:77-86)
+  integer   :: nbins = 256
+  logical   :: diagnostic = .false., pp_filtered_surface = .false.
+  integer   :: nlines = 3, nfilter = 1
+  real (pr) :: fscale = 1.0_pr
+
+  namelist /misc_params/ nbins, diagnostic, pp_filtered_surface, nlines, &
+    nfilter, fscale
+
   ! grid_params: the compile-time values of the reference as run-time
   ! input, plus the device options of this build
   integer      :: nx = 64, ny = 64, nz = 64
   character(5) :: storage = 'c128'   ! 'c128', 'c64' or 'mixed' device fields
   real (pr)    :: vscale = 0.0_pr     ! V term in eqn 1 (0 = reference)
   character(8) :: host_mode = 'resident' ! 'resident' or 'parity' (copy per step)
+  logical      :: save_mom = .false.  ! momentum.dat of the final state
 
-  namelist /grid_params/ nx, ny, nz, storage, vscale, host_mode
+  namelist /grid_params/ nx, ny, nz, storage, vscale, host_mode, save_mom
 
   ! Parameters for adaptive time stepping (src/parameters.f90:89-93)
   real (pr), parameter :: eps         = 1e-6_pr

@@ -1,6 +1,10 @@
 ! Drop-in for the hot-path routines of the reference's module variables:
 ! laplacian (src/variables.f90:55-71), get_norm (537-559), energy (563-587),
-! mass (591-611), renormalise (698-709), with the reference's interfaces.
+! mass (591-611), renormalise (698-709), with the reference's interfaces,
+! and for its diagnostics momentum (615-694) and get_vcf (467-507).
+! velocity_pdf replaces the sequence get_pdf_velocity + get_pdf x 3 of
+! save_velocity_pdf (src/io.f90:236-239) with one call that returns the
+! same PDFs, maxima, means and deviations.
 ! The work runs on the GPU; reductions accumulate in FP64 in a fixed order.
 module variables
   use iso_c_binding
@@ -11,7 +15,8 @@ module variables
   implicit none
 
   private
-  public :: laplacian, get_norm, energy, mass, renormalise, state_norm, state_renormalise
+  public :: laplacian, get_norm, energy, mass, renormalise, state_norm, state_renormalise, momentum, get_vcf, &
+    velocity_pdf
 
   type, public :: var
     complex (pr), allocatable, dimension(:,:,:) :: new
@@ -52,6 +57,48 @@ contains
     call check(gpe_energy(the_ctx(), v), 'energy')
     E = v
   end subroutine energy
+
+  subroutine put_halo(in_var)
+    complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
+    integer(c_int64_t) :: nya, yo, zo
+    call halo_args(nya, yo, zo)
+    call check(gpe_upload(the_ctx(), in_var, host_kind(), nya, yo, zo), 'upload')
+  end subroutine put_halo
+
+  subroutine momentum(in_var, mom)
+    complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
+    real (pr), dimension(3), intent(out) :: mom
+    real(c_double) :: m(3)
+    call put_halo(in_var)
+    call check(gpe_momentum(the_ctx(), m), 'momentum')
+    mom = m
+  end subroutine momentum
+
+  subroutine get_vcf(in_var, f)
+    complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
+    real (pr), dimension(0:nx1), intent(out) :: f
+    real(c_double) :: fd(0:nx1)
+    call put_halo(in_var)
+    call check(gpe_vcf(the_ctx(), fd), 'get_vcf')
+    f = fd
+  end subroutine get_vcf
+
+  subroutine velocity_pdf(in_var, nb, pdf_vx, pdf_vy, pdf_vz, vmax, vmean, vstdev)
+    complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
+    integer, intent(in) :: nb
+    real (pr), dimension(-nb/2+1:nb/2), intent(out) :: pdf_vx, pdf_vy, pdf_vz
+    real (pr), dimension(3), intent(out) :: vmax, vmean, vstdev
+    integer(c_int64_t) :: cnt(3)
+    real(c_double) :: mx(3), mn(3), sd(3), pdf(nb, 3)
+    call put_halo(in_var)
+    call check(gpe_velocity_pdf(the_ctx(), int(nb, c_int), cnt, mn, sd, mx, pdf), 'velocity_pdf')
+    pdf_vx = pdf(:, 1)
+    pdf_vy = pdf(:, 2)
+    pdf_vz = pdf(:, 3)
+    vmax = mx
+    vmean = mn
+    vstdev = sd
+  end subroutine velocity_pdf
 
   subroutine mass(in_var, M)
     complex (pr), dimension(0:nx1,js:je,ks:ke), intent(in) :: in_var

@@ -35,7 +35,7 @@ def load(path):
 
 
 def ids(paths):
-    return [os.path.basename(x)[8:-4] for x in paths]
+    return [os.path.basename(x)[9:-4] for x in paths]
 
 
 def momentum_scale(p, psi):

@@ -108,3 +108,65 @@ def test_reference_program_with_gpu_dropin(tmp_path, name):
     t_tol = 1e-10 * abs(float(z["t"])) if p.scheme == "rk45" else 1e-12
     assert abs(t - float(z["t"])) <= t_tol
     assert abs(extra[0] - float(z["norm"])) <= 1e-12 * float(z["norm"])
+
+
+def fnum(v):
+    """A Fortran Ew.d field: with a three-digit exponent the E is dropped
+    (0.864549105-315)."""
+    try:
+        return float(v)
+    except ValueError:
+        k = max(v.rfind("-"), v.rfind("+"))
+        return float(v[:k] + "E" + v[k:])
+
+
+def read_pdf_file(path):
+    head, rows = {}, []
+    for line in open(path):
+        line = line.strip()  # the headers are right-justified (a10)
+        if line.startswith("#"):
+            key, *vals = line[1:].split()
+            head[key.rstrip(":")] = np.array([fnum(v) for v in vals])
+        else:
+            rows.append([fnum(v) for v in line.split()])
+    return head, np.array(rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["diag_ring64", "diag_per_o4_turb", "diag_ref_o4_trap_line"])
+def test_host_diagnostic_files_match_reference(tmp_path, name):
+    """The host's final-state diagnostics (save_pdf, save_vcf, save_mom)
+    written in the formats of save_velocity_pdf, save_vel_corr and
+    save_momentum (src/io.f90:222-311, 426-445), against the reference's
+    routines on the same state.  A run of zero steps keeps the input state.
+    The files hold 9 significant digits (e17.9), so values compare to 1e-8
+    relative; the PDF to one count per bin as in the device test."""
+    p, psi, z = load_fixture(name)
+    nbins = int(z["nbins"])
+    text = run_in_text(p, -0.5 * p.tau, nbins=nbins)
+    text = text.replace("save_pdf = .false.", "save_pdf = .true.").replace("save_vcf = .false.", "save_vcf = .true.")
+    text = text.replace("host_mode = 'resident'", "host_mode = 'resident'\n  save_mom = .true.")
+    (tmp_path / "run.in").write_text(text)
+    write_end_state(str(tmp_path / "end_state00.dat"), psi, 0, 0.0, 0.0, p.dt0())
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    mom = np.array([[fnum(v) for v in ln.split()] for ln in open(tmp_path / "momentum.dat")])
+    assert mom.shape == (1, 4) and mom[0, 0] == 0.0
+    scale = np.abs(z["mom"]).max()
+    assert np.all(np.abs(mom[0, 1:] - z["mom"]) <= 1e-8 * scale + 1e-12)
+    head, rows = read_pdf_file(tmp_path / "pdf_vel0000000.dat")
+    assert rows.shape == (nbins, 9)
+    assert np.allclose(head["Mean"], z["vmean"], rtol=1e-8, atol=1e-8 * np.abs(z["vstdev"]).max())
+    assert np.allclose(head["StDev"], z["vstdev"], rtol=1e-8)
+    assert np.allclose(head["Sum"], 1.0, rtol=1e-8)
+    for a in range(3):
+        i = np.arange(-nbins // 2 + 1, nbins // 2 + 1)
+        assert np.allclose(rows[:, 3 * a], 2.0 * i * z["vmax"][a] / nbins, rtol=1e-8, atol=1e-12)
+        dc = np.abs(rows[:, 3 * a + 1] - z["pdf"][a]) * z["counts"][a]
+        assert dc.max() <= 1.0 + 1e-3 and dc.sum() <= 2.0 + 1e-3
+        sd, mu = z["vstdev"][a], z["vmean"][a]
+        gauss = np.exp(-0.5 * (rows[:, 3 * a] - mu) ** 2 / sd ** 2) / (sd * np.sqrt(2 * np.pi))
+        assert np.allclose(rows[:, 3 * a + 2], gauss, rtol=1e-6, atol=1e-12)
+    vcf = np.array([[fnum(v) for v in ln.split()] for ln in open(tmp_path / "vcf0000000.dat") if "#" not in ln])
+    assert np.array_equal(vcf[:, 0], np.arange(p.nx))
+    assert np.allclose(vcf[:, 1], z["vcf"], rtol=1e-8, atol=1e-9)
