@@ -6,11 +6,14 @@
 # holds no parameters.in, so the compile-time configuration written here is
 # found on the include path, as the reference's own setup script intends).
 # mpif.h and the MPI library are the image's MPICH (/opt/conda).  The
-# drivers are oracle/ref/ref_driver.f90 and oracle/ref/ref_diag.f90 (this
-# project's harnesses).  Unused
-# code (FFT diagnostics, IC generators) is dropped at link time with
-# --gc-sections, so the absent FFTW2 library is never referenced.
-# Outputs go to oracle/_ref/<nx>x<ny>x<nz>/{ref_driver,ref_diag} only.
+# drivers are oracle/ref/ref_driver.f90, ref_diag.f90 and ref_ic.f90 (this
+# project's harnesses); ref_ic is linked once per initial condition of
+# oracle/ref/ic_cases.txt, each with that expression as the ic.in the
+# reference includes.  Unused code (the FFT routine and the FFT diagnostics)
+# is dropped at link time by LTO, so the absent FFTW2 library is never
+# referenced: random_phase, which needs it, is not built.
+# Outputs go to oracle/_ref/<nx>x<ny>x<nz>/{ref_driver,ref_diag} and
+# oracle/_ref/ic/<case>/ref_ic only.
 set -euo pipefail
 REF=${GPE_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -19,10 +22,8 @@ FC=${FC:-/opt/rocm/lib/llvm/bin/amdflang}
 MPI=${MPI_HOME:-/opt/conda}
 [ -d "$REF/examples/ring" ] || { echo "build_ref.sh: $REF not present, skipping"; exit 0; }
 
-build_grid() {
-  local nx=$1 ny=$2 nz=$3
-  local dir=$OUT/${nx}x${ny}x${nz}
-  mkdir -p "$dir"
+write_pin() {
+  local dir=$1 nx=$2 ny=$3 nz=$4
   cat > "$dir/parameters.in" <<PIN
   integer, parameter :: pr = selected_real_kind(15,307)
   integer, parameter :: nyprocs = 1
@@ -48,6 +49,13 @@ build_grid() {
   type (ring_param), parameter :: vr1 = ring_param('xy', 0.0_pr, 0.0_pr, &
     0.0_pr, 8.0_pr, 0.0_pr, 5, 0.0_pr, 10, 1.0_pr)
 PIN
+}
+
+build_grid() {
+  local nx=$1 ny=$2 nz=$3
+  local dir=$OUT/${nx}x${ny}x${nz}
+  mkdir -p "$dir"
+  write_pin "$dir" $nx $ny $nz
   # the run starts from a restart file, so the compile-time IC is a constant
   echo "      init_cond = cmplx(1.0_pr, 0.0_pr, pr)" > "$dir/ic.in"
   local flags="-O2 -flto -I$dir -I$MPI/include -module-dir $dir"
@@ -87,5 +95,36 @@ PIN
   echo "built $gdir/ref_driver_gpu"
 }
 
+# Initial conditions: the reference's ic module compiled once per case of
+# ref/ic_cases.txt, with that case's expression as its ic.in, linked with
+# the harness ref/ref_ic.f90.  Outputs: _ref/ic/<case>/ref_ic.
+build_ics() {
+  local nx=16 ny=12 nz=10
+  local dir=$OUT/ic
+  mkdir -p "$dir"
+  write_pin "$dir" $nx $ny $nz
+  # placeholder for the module build; each case compiles ic.f90 again
+  echo "      init_cond = cmplx(1.0_pr, 0.0_pr, pr)" > "$dir/ic.in"
+  local flags="-O2 -flto -I$dir -I$MPI/include -module-dir $dir"
+  for m in parameters constants error derivs ic variables; do
+    $FC $flags -c "$REF/examples/ring/$m.f90" -o "$dir/$m.o"
+  done
+  $FC $flags -c "$HERE/ref/ref_ic.f90" -o "$dir/ref_ic.o"
+  local name expr
+  while IFS='|' read -r name expr; do
+    case "$name" in ''|'#'*) continue ;; esac
+    local cdir=$dir/$name
+    mkdir -p "$cdir"
+    echo "  init_cond = $expr" > "$cdir/ic.in"
+    # the case's ic.in comes first on the include path; the module file
+    # is the same for every case and stays in $dir
+    $FC -O2 -flto -I"$cdir" -I$dir -I$MPI/include -module-dir "$cdir" -c "$REF/examples/ring/ic.f90" -o "$cdir/ic.o"
+    $FC -O2 -flto -o "$cdir/ref_ic" "$dir"/{parameters,constants,error,derivs}.o "$cdir/ic.o" "$dir"/{variables,ref_ic}.o \
+        -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -fuse-ld=lld
+    echo "built $cdir/ref_ic"
+  done < "$HERE/ref/ic_cases.txt"
+}
+
 build_grid 64 64 64
 build_grid 24 20 18
+build_ics

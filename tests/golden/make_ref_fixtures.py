@@ -5,6 +5,8 @@ the run parameters, the reference's final field, t, im_t, dt, p, its norm
 and energy (get_norm, energy) and the per-step (t, im_t, dt) trace.  The
 ref_diag_* fixtures hold an input field and the reference's momentum,
 velocity statistics and PDF (nbins = 64) and velocity correlation function.
+The ref_ic_* fixtures hold the reference's initial conditions (its ics with
+the expressions of oracle/ref/ic_cases.txt as ic.in) on a 16 x 12 x 10 grid.
 
 Run in the build container (the reference is not present on the GPU box):
     python tests/golden/make_ref_fixtures.py
@@ -151,6 +153,46 @@ def diag_cases():
                             nbins=nbins, **extra, **res)
 
 
+# Run parameters of each initial-condition case of oracle/ref/ic_cases.txt:
+# grid extents, scal (eqn_params) and the trap of fermi.
+IC_GRID = dict(nx=16, ny=12, nz=10)
+IC_CASES = {
+    "ring_xy": dict(xr=4.0, yr=3.5, zr=3.0, scal=1.0),
+    "ring_xz": dict(xr=4.0, yr=3.5, zr=3.0, scal=1.3),
+    "ring_yz": dict(xr=4.5, yr=4.0, zr=3.5, scal=1.0),
+    "line_x": dict(xr=4.0, yr=3.0, zr=2.5, scal=1.0),
+    "line_y_phase": dict(xr=4.0, yr=3.0, zr=2.5, scal=0.8),
+    "line_z": dict(xr=4.0, yr=3.0, zr=2.5, scal=1.2),
+    "trap_line": dict(xr=6.0, yr=5.0, zr=4.0, scal=1.0, mu=4.0, g=1.5, omx=0.6, omy=0.5, omz=0.7),
+    "pade_ring": dict(xr=6.0, yr=5.0, zr=4.5, scal=1.0),
+    "pade_pulse": dict(xr=6.0, yr=5.0, zr=4.5, scal=1.0),
+    "pair_sphere": dict(xr=15.0, yr=12.0, zr=14.0, scal=1.0),
+    "wall_sphere2": dict(xr=6.0, yr=5.0, zr=40.0, scal=1.0),
+}
+
+
+def ic_cases():
+    for name, kw in IC_CASES.items():
+        kw = dict(kw)
+        scal = kw.pop("scal")
+        p = RunParams(**IC_GRID, **kw)
+        exe = os.path.join(REF, "ic", name, "ref_ic")
+        if not os.path.exists(exe):
+            raise SystemExit(f"{exe} missing: run oracle/build_ref.sh first")
+        run = os.path.join(REF, "run_ic_" + name)
+        shutil.rmtree(run, ignore_errors=True)
+        os.makedirs(run)
+        text = run_in_text(p, 0.0, restart=False).replace("scal = 1.0", f"scal = {scal!r}")
+        with open(os.path.join(run, "run.in"), "w") as f:
+            f.write(text)
+        subprocess.run([exe, "out.dat"], cwd=run, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        out = np.fromfile(os.path.join(run, "out.dat"), dtype="<c16").reshape(p.nz, p.ny, p.nx)
+        shutil.rmtree(run)
+        print(f"ic {name}: max|psi| {np.abs(out).max():.4f}", flush=True)
+        np.savez_compressed(os.path.join(OUT, f"ref_ic_{name}.npz"), params=json.dumps(dataclasses.asdict(p)),
+                            scal=scal, out=out)
+
+
 def ring64():
     p, psi = ics.config_ring64(seed=0)
     res = run_reference("ring64_rk4_200", p, psi, nsteps=200)
@@ -158,10 +200,12 @@ def ring64():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "ring64", "diag"]
+    which = sys.argv[1:] or ["small", "ring64", "diag", "ic"]
     if "small" in which:
         small_cases()
     if "ring64" in which:
         ring64()
     if "diag" in which:
         diag_cases()
+    if "ic" in which:
+        ic_cases()
