@@ -19,6 +19,10 @@
 // at plane p = z + H over the whole block and the second stage at plane z
 // over the inner tile.  2H warm-up iterations before z0 fill the queues.
 //
+// F34's register queues are indexed by (plane - start) % (2H + 1) with the
+// z-loop unrolled 2H + 1 times (march_rotating), so they rotate by renaming
+// instead of being copied down one slot per plane.
+//
 // Per thread (own column): register queues of the inputs for planes z..z+2H,
 // of the first stage's result for planes z-H..z+H, and (F34) of acc for the
 // inner planes z..z+H.  LDS: the first stage's input plane p with an H halo (double
@@ -133,6 +137,24 @@ __device__ __forceinline__ void fused_march(int start, int z1, Fetch fetch, Step
     }
 }
 
+// Calls step(z, std::integral_constant<int, PH>) for z = start .. z1-1 with
+// PH = (z - start) % U known at compile time: register queues indexed by
+// (PH + d) % U rotate by renaming instead of copying every entry each plane.
+template <int U, int PH = 0, class Step>
+__device__ __forceinline__ void march_phases(int z, int z1, Step& step) {
+    if constexpr (PH < U) {
+        if (z < z1) {
+            step(z, std::integral_constant<int, PH>{});
+            march_phases<U, PH + 1>(z + 1, z1, step);
+        }
+    }
+}
+
+template <int U, class Step>
+__device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
+    for (int z = start; z < z1; z += U) march_phases<U>(z, z1, step);
+}
+
 template <int ORD>
 __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
                                                   double2* __restrict__ u2out, double2 hdt) {
@@ -212,7 +234,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
     using S = FusedShape<ORD>;
     // one plane of loads in flight: a second set would not fit beside the
     // three register queues without spilling
-    constexpr int H = S::H, D = 1, NB = H + 1;
+    constexpr int H = S::H, D = 1, NB = H + 1, Q = 2 * H + 1;
     __shared__ double2 ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ double2 ldsP[2][S::NTB];           // psi0 plane p
     __shared__ double2 ldsB[NB][S::NTB];          // u3 planes z..z+H
@@ -220,12 +242,11 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
 
-    double2 qu[2 * H + 1];  // u2, planes z..z+2H
-    double2 qp[2 * H + 1];  // psi0, planes z..z+2H
-    double2 qv[2 * H + 1];  // u3, planes z-H..z+H
-    double2 qa[H + 1];      // acc, planes z..z+H (inner tile)
+    // Register queues, slot (plane - start) % Q: u2 and psi0 for planes
+    // z..z+2H, u3 for planes z-H..z+H, acc for the inner planes z..z+H.
+    double2 qu[Q], qp[Q], qv[Q], qa[H + 1];
 #pragma unroll
-    for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
+    for (int d = 0; d < Q; ++d) qv[d] = mkc<double>(0.0, 0.0);
 #pragma unroll
     for (int d = 0; d <= H; ++d) qa[d] = mkc<double>(0.0, 0.0);
     if (c.active) {
@@ -251,15 +272,18 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
         for (int s = 0; s < S::NSLOT; ++s)
             if (c.hv[s]) F.hr[s] = *(u2 + Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), c.hoff[s]});
     };
-    auto step = [&](int z, Pf& F) {
+    Pf F{};
+    fetch(F, start);
+    auto step = [&](int z, auto phase) {
+        constexpr int PH = decltype(phase)::value;
         const int p = z + H;
-        qu[2 * H] = F.qnu;
-        qp[2 * H] = F.qnp;
+        qu[(PH + 2 * H) % Q] = F.qnu;
+        qp[(PH + 2 * H) % Q] = F.qnp;
         double2* A = ldsA[(z - start) & 1];
         double2* P = ldsP[(z - start) & 1];
         if (c.active) {
-            A[c.la] = qu[H];
-            P[c.lb] = qp[H];
+            A[c.la] = qu[(PH + H) % Q];
+            P[c.lb] = qp[(PH + H) % Q];
         }
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s)
@@ -272,11 +296,12 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
             sa.L = A;
             sa.lc = c.la;
 #pragma unroll
-            for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[d];
+            for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
             sa.w = w;
             const double2 f2 = rhs_eval<double>(k, sa, 0, 0, 0);
-            const double2 u3 = cfma<double>(dt, f2, qp[H]);
-            qv[2 * H] = u3;
+            const double2 p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
+            const double2 u3 = cfma<double>(dt, f2, p0);
+            qv[(PH + 2 * H) % Q] = u3;
             ldsB[(p - start) % NB][c.lb] = u3;
             if (c.inner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
@@ -284,10 +309,9 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
                 sp.L = P;
                 sp.lc = c.lb;
 #pragma unroll
-                for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[d];
+                for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[(PH + d) % Q];
                 sp.w = w;
                 const double2 f1 = rhs_eval<double>(k, sp, 0, 0, 0);
-                const double2 p0 = qp[H], uu = qu[H];
                 const double2 base = mkc<double>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
                 qa[H] = cfma<double>(dt3, f2, cfma<double>(dt6, f1, base));
             }
@@ -299,21 +323,15 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
             sb.L = ldsB[(z - start) % NB];
             sb.lc = c.lb;
 #pragma unroll
-            for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
+            for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
             const double2 nw = cfma<double>(dt6, rhs_eval<double>(k, sb, 0, 0, 0), qa[0]);
             if (c.store) st_stream<double>(out + Cell{g.plane * z, c.own}, nw);
         }
 #pragma unroll
-        for (int d = 0; d < 2 * H; ++d) {
-            qu[d] = qu[d + 1];
-            qp[d] = qp[d + 1];
-            qv[d] = qv[d + 1];
-        }
-#pragma unroll
         for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
     };
-    fused_march<D, Pf>(start, c.z1, fetch, step);
+    march_rotating<Q>(start, c.z1, step);
 }
 
 // Host side (gpe_k_fused.hip).
