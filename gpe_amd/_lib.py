@@ -36,6 +36,11 @@ class gpe_params(ct.Structure):
     ]
 
 
+class gpe_line(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in ("x0", "y0", "z0", "amp1", "amp2", "ll", "sgn")] + [
+        ("dir", ct.c_int32), ("imprint_phase", ct.c_int32)]
+
+
 class gpe_step_info(ct.Structure):
     _fields_ = [
         ("t", ct.c_double), ("im_t", ct.c_double), ("dt_re", ct.c_double), ("dt_im", ct.c_double),
@@ -66,6 +71,8 @@ PROTOTYPES = [
     ("gpe_energy", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_mass", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_momentum", _I, [_VP, ct.POINTER(_D)]),
+    ("gpe_imprint_line", _I, [_VP, ct.POINTER(gpe_line), _D]),
+    ("gpe_peek", _I, [_VP, ct.c_int32, ct.c_int32, ct.c_int32, ct.POINTER(_D)]),
     ("gpe_velocity_pdf", _I, [_VP, _I, ct.POINTER(_I64), ct.POINTER(_D), ct.POINTER(_D), ct.POINTER(_D),
                               ct.POINTER(_D)]),
     ("gpe_vcf", _I, [_VP, ct.POINTER(_D)]),

@@ -195,6 +195,19 @@ class Context:
         self._chk(self.lib.gpe_vcf(self.h, f.ctypes.data_as(ct.POINTER(ct.c_double))), "gpe_vcf")
         return f
 
+    def imprint_line(self, vl, scal: float = 1.0) -> None:
+        """psi <- |psi| vortex_line(vl) (imprint_vortex_line,
+        src/variables.f90:1148-1162); vl is an ics.LineParam."""
+        ln = _lib.gpe_line(vl.x0, vl.y0, vl.z0, vl.amp1, vl.amp2, vl.ll, vl.sgn, "xyz".index(vl.dir),
+                           int(bool(vl.imprint_phase)))
+        self._chk(self.lib.gpe_imprint_line(self.h, ct.byref(ln), float(scal)), "gpe_imprint_line")
+
+    def peek(self, i: int, j: int, k: int) -> complex:
+        """The state's value at cell (i, j, k)."""
+        v = (ct.c_double * 2)()
+        self._chk(self.lib.gpe_peek(self.h, int(i), int(j), int(k), v), "gpe_peek")
+        return complex(v[0], v[1])
+
     def mass(self) -> float:
         v = ct.c_double()
         self._chk(self.lib.gpe_mass(self.h, ct.byref(v)), "gpe_mass")

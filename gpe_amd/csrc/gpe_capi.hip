@@ -1034,6 +1034,51 @@ int gpe_renormalise(gpe_ctx* c, double norm) {
     return 0;
 }
 
+int gpe_imprint_line(gpe_ctx* c, const gpe_line* vl, double scal) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_imprint_line: ctx is NULL");
+    if (!vl) return fail(c, GPE_EINVAL, "gpe_imprint_line: vl is NULL");
+    if (vl->dir < 0 || vl->dir > 2) return fail(c, GPE_EINVAL, "gpe_imprint_line: dir must be 0 (x), 1 (y) or 2 (z)");
+    LineParam lp{vl->x0, vl->y0, vl->z0, vl->amp1, vl->amp2, vl->ll, vl->sgn, scal, 0.0, 0.0, vl->dir,
+                 vl->imprint_phase != 0};
+    // zero-wavelength guard of get_r / get_theta (src/ic.f90:741-755)
+    const double eps = 2.220446049250313e-16;
+    if (fabs(vl->ll) < eps) {
+        if (fabs(vl->amp1) >= eps || fabs(vl->amp2) >= eps)
+            return fail(c, GPE_EINVAL, "ERROR: vortex line disturbance wavelength is zero, but amplitude is non-zero.");
+        lp.safe1 = lp.safe2 = 1.0;
+    }
+    double d[3];
+    spacing(c->p, d);
+    const double lo[3] = {-c->p.xr, -c->p.yr, -c->p.zr};
+    if (c->p.storage != GPE_C64) {
+        HIPCHK(c, launch_imprint_line<double>((double2*)c->psi, c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
+    } else {
+        HIPCHK(c, launch_imprint_line<float>((float2*)c->psi, c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
+    }
+    c->f0_valid = false;
+    c->cand_valid = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int gpe_peek(gpe_ctx* c, int32_t i, int32_t j, int32_t k, double out[2]) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_peek: ctx is NULL");
+    if (!out) return fail(c, GPE_EINVAL, "gpe_peek: output is NULL");
+    if (i < 0 || i >= c->p.nx || j < 0 || j >= c->p.ny || k < 0 || k >= c->p.nz)
+        return fail(c, GPE_EINVAL, "gpe_peek: cell outside the grid");
+    const long long q = ((long long)k * c->p.ny + j) * c->p.nx + i;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->esz == 16) {
+        HIPCHK(c, hipMemcpy(out, (const char*)c->psi + q * 16, 16, hipMemcpyDeviceToHost));
+    } else {
+        float v[2];
+        HIPCHK(c, hipMemcpy(v, (const char*)c->psi + q * 8, 8, hipMemcpyDeviceToHost));
+        out[0] = v[0];
+        out[1] = v[1];
+    }
+    return 0;
+}
+
 int gpe_prof_enable(gpe_ctx* c, int on) {
     if (!c) return fail(nullptr, GPE_EINVAL, "gpe_prof_enable: ctx is NULL");
     HIPCHK(c, hipStreamSynchronize(c->stream));

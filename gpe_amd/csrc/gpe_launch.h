@@ -87,6 +87,17 @@ template <typename R>
 hipError_t launch_norm_partial(const C<R>* u, int nx, int ny, int nz, int weighted, double* partial, int nblocks,
                                hipStream_t s);
 template <typename R> hipError_t launch_scale(C<R>* u, long long n, double snn, double sn, hipStream_t s);
+
+// A vortex line for k_imprint_line: line_param of the reference plus scal
+// and the zero-wavelength guards safe1/safe2 of get_r / get_theta.
+struct LineParam {
+    double x0, y0, z0, amp1, amp2, ll, sgn, scal, safe1, safe2;
+    int dir;            // 0 x, 1 y, 2 z
+    int imprint_phase;  // 1: phase only
+};
+template <typename R>
+hipError_t launch_imprint_line(C<R>* u, int nx, int ny, long long n, const double lo[3], const double d[3],
+                               const LineParam& vl, hipStream_t s);
 template <typename R> hipError_t launch_from_c128(const double2* src, C<R>* dst, long long n, hipStream_t s);
 template <typename R> hipError_t launch_from_c64(const float2* src, C<R>* dst, long long n, hipStream_t s);
 template <typename R> hipError_t launch_to_c128(const C<R>* src, double2* dst, long long n, hipStream_t s);

@@ -82,6 +82,51 @@ template <typename R> hipError_t launch_scale(C<R>* u, long long n, double snn, 
     return hipGetLastError();
 }
 
+// psi <- |psi| vortex_line(vl) (imprint_vortex_line, src/variables.f90:1148-1162):
+// the line's phase exp(i theta), times amp(r) = 1 - exp(-0.7 r^1.15) unless
+// only the phase is imprinted, with theta and r of get_theta / get_r
+// (src/ic.f90:730-884) at the grid point x(i) = xl + i dx (get_grid).
+template <typename R>
+__global__ __launch_bounds__(kPwThreads) void k_imprint_line(C<R>* __restrict__ u, int nx, int ny, long long n,
+                                                             double xl, double yl, double zl, double dx, double dy,
+                                                             double dz, LineParam vl) {
+    constexpr double kPi = 3.1415926535897932384626433832795;
+    for (long long q = blockIdx.x * (long long)kPwThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kPwThreads) {
+        const int i = (int)(q % nx);
+        const long long r = q / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const double x = xl + (double)i * dx, y = yl + (double)j * dy, z = zl + (double)k * dz;
+        // (t, t0) along the line; (a, a0) displaced by amp1 sin, (b, b0) by amp2 cos
+        double t, t0, a, a0, b, b0;
+        if (vl.dir == 0) {
+            t = x, t0 = vl.x0, a = y, a0 = vl.y0, b = z, b0 = vl.z0;
+        } else if (vl.dir == 1) {
+            t = y, t0 = vl.y0, a = z, a0 = vl.z0, b = x, b0 = vl.x0;
+        } else {
+            t = z, t0 = vl.z0, a = x, a0 = vl.x0, b = y, b0 = vl.y0;
+        }
+        const double da = a - a0 - vl.amp1 * sin(2.0 * kPi * (t - t0) / (vl.ll + vl.safe1));
+        const double db = b - b0 - vl.amp2 * cos(2.0 * kPi * (t - t0) / (vl.ll + vl.safe2));
+        const double theta = vl.sgn * atan2(vl.scal * db, vl.scal * da);
+        double amp = 1.0;
+        if (!vl.imprint_phase) {
+            const double rr = sqrt(vl.scal * da * da + vl.scal * db * db);
+            amp = 1.0 - exp(-0.7 * pow(rr, 1.15));
+        }
+        const C<R> v = u[q];
+        const double m = hypot((double)v.x, (double)v.y) * amp;
+        u[q] = mkc<R>((R)(m * cos(theta)), (R)(m * sin(theta)));
+    }
+}
+
+template <typename R>
+hipError_t launch_imprint_line(C<R>* u, int nx, int ny, long long n, const double lo[3], const double d[3],
+                               const LineParam& vl, hipStream_t s) {
+    hipLaunchKernelGGL((k_imprint_line<R>), dim3(pw_blocks(n)), dim3(kPwThreads), 0, s, u, nx, ny, n, lo[0], lo[1],
+                       lo[2], d[0], d[1], d[2], vl);
+    return hipGetLastError();
+}
+
 template <typename A, typename B>
 __global__ __launch_bounds__(kPwThreads) void k_convert(const A* __restrict__ src, B* __restrict__ dst, long long n) {
     for (long long i = blockIdx.x * (long long)kPwThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kPwThreads) {
@@ -151,6 +196,8 @@ hipError_t launch_reduce_final(int mode, const double* partial, int n, int nval,
     template hipError_t launch_ck1_retry<R>(const C<R>*, const C<R>*, C<R>*, C<R>, R, long long, hipStream_t);     \
     template hipError_t launch_norm_partial<R>(const C<R>*, int, int, int, int, double*, int, hipStream_t);       \
     template hipError_t launch_scale<R>(C<R>*, long long, double, double, hipStream_t);                           \
+    template hipError_t launch_imprint_line<R>(C<R>*, int, int, long long, const double*, const double*,         \
+                                               const LineParam&, hipStream_t);                                     \
     template hipError_t launch_from_c128<R>(const double2*, C<R>*, long long, hipStream_t);                       \
     template hipError_t launch_from_c64<R>(const float2*, C<R>*, long long, hipStream_t);                         \
     template hipError_t launch_to_c128<R>(const C<R>*, double2*, long long, hipStream_t);                         \

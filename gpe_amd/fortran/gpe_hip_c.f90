@@ -26,6 +26,11 @@ module gpe_hip_c
     integer(c_int64_t) :: n_accept, n_reject, n_rhs
   end type gpe_step_info
 
+  type, bind(C) :: gpe_line
+    real(c_double) :: x0, y0, z0, amp1, amp2, ll, sgn
+    integer(c_int32_t) :: dir, imprint_phase
+  end type gpe_line
+
   interface
     integer(c_int) function gpe_abi_version() bind(C, name='gpe_abi_version')
       import
@@ -113,6 +118,20 @@ module gpe_hip_c
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: v
     end function gpe_energy
+
+    integer(c_int) function gpe_imprint_line(ctx, vl, scal) bind(C, name='gpe_imprint_line')
+      import
+      type(c_ptr), value :: ctx
+      type(gpe_line), intent(in) :: vl
+      real(c_double), value :: scal
+    end function gpe_imprint_line
+
+    integer(c_int) function gpe_peek(ctx, i, j, k, v) bind(C, name='gpe_peek')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: i, j, k
+      real(c_double), intent(out) :: v(2)
+    end function gpe_peek
 
     integer(c_int) function gpe_momentum(ctx, mom) bind(C, name='gpe_momentum')
       import

@@ -1,15 +1,26 @@
 ! Fortran host program of the GPE time-stepping path on one MI355X.  It
-! follows the reference driver (src/gpe.f90:33-312) for the parts that feed
-! the hot path: run.in namelists, the dt convention (76-80), the initial
-! state read from end_state00.dat in the reference's restart format
-! (src/ic.f90:117-186), the main loop `do while (t+im_t <= end_time)` with
-! the RUNNING sentinel (134-159), the eqn-4 norm / imaginary-time logic
-! (241-271) and the final end_state.dat (src/io.f90:840-891).  With
-! save_pdf / save_vcf (io_params) or save_mom (grid_params) it writes the
-! final state's velocity PDF, velocity correlation function and momentum in
-! the formats of save_velocity_pdf, save_vel_corr and save_momentum
-! (src/io.f90:222-311, 426-445) to pdf_vel<p>.dat, vcf<p>.dat (p as seven
-! digits, as itos forms it) and momentum.dat, from the device diagnostics.
+! follows the reference driver (src/gpe.f90:33-312): run.in namelists, the
+! dt convention (76-80), the initial state read from end_state00.dat in the
+! reference's restart format (src/ic.f90:117-186), and the main loop
+! `do while (t+im_t <= end_time)` (134-278) in the reference's order:
+!   * the RUNNING sentinel (136-159);
+!   * every save_rate steps (modulo(t+im_t, |dt| save_rate) < |dt|) the
+!     time series energy.dat and psi_time.dat (save_energy, save_time,
+!     src/io.f90:147-218), and momentum.dat with save_mom;
+!   * at save_rate3 intervals the velocity PDF and correlation function
+!     (save_pdf, save_vcf: pdf/pdf_vel<p>.dat, vcf/vcf<p>.dat,
+!     src/io.f90:222-311);
+!   * at save_rate2 intervals the 3D state with save_3d (idl_surface,
+!     proc00/dens<p>.dat, src/io.f90:489-529);
+!   * at p_save intervals the periodic end_state.dat and save.dat
+!     (src/io.f90:840-891);
+!   * the solver step, then the SAVE sentinel (save_run, src/io.f90:1566-1595);
+!   * for eqn 4 the norm, norm.dat, the imaginary-time switch, renorm and
+!     imprint_vl (gpe.f90:241-271).
+! After the loop: end_state.dat (RUNNING removed) and, with save_3d, the
+! final dens file.  p is written as seven digits, as itos forms it.  Not
+! built: condensed_particles / spectra (mass.dat, spectrum/, idf/), line
+! length (save_ll), zeros, contours and averages.
 !
 ! host_mode = 'parity' copies psi%new -> psi%old and calls
 ! solver(psi%old, psi%new) every step exactly as the reference does;
@@ -27,7 +38,8 @@ program gpe_hip_main
   type(var) :: psi
   real (pr) :: norm, prev_norm, relnorm
   logical :: state_exist, running, resident
-  integer :: u, ios, nx_prev, ny_prev, nz_prev, nsteps
+  integer :: u, ios, nx_prev, ny_prev, nz_prev, nsteps, ps, n, m
+  logical :: dirs_made(3) = .false.
   integer(8) :: c0, c1, crate
   complex (pr) :: dt_prev
   real (pr) :: secs
@@ -104,28 +116,65 @@ program gpe_hip_main
       if (dt_prev /= dt) p = int(p * dt_prev / dt)
   end select
 
+  relnorm = 0.0_pr
+  psi%old = 0.0_pr
+  if (resident) call state_put(psi%new)
+  if (save_3d) call idl_surface()
   if (eqn_to_solve == 4 .and. .not. real_time) then
-    call get_norm(psi%new, prev_norm)
-    if (renorm) call renormalise(psi%new, prev_norm)
+    if (resident) then
+      call state_norm(prev_norm)
+      if (renorm) call state_renormalise(prev_norm)
+    else
+      call get_norm(psi%new, prev_norm)
+      if (renorm) call renormalise(psi%new, prev_norm)
+    end if
+    call save_norm(t, prev_norm, relnorm)
   else
     prev_norm = 1.0_pr
   end if
   norm = prev_norm
 
-  psi%old = 0.0_pr
-  if (resident) call state_put(psi%new)
+  ps = int(t/p_save)
+  n = int(t/save_rate2)
+  m = int(t/save_rate3)
 
   nsteps = 0
   call system_clock(c0, crate)
   do while (t+im_t <= end_time)
     inquire (file='RUNNING', exist=running)
-    if (.not. running) exit
+    if (.not. running) then
+      print *, 'Stop requested.'
+      exit
+    end if
+    if (.not. resident) psi%old(:,js:je,ks:ke) = psi%new
+
+    if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) call time_series()
+    if (t+im_t >= save_rate3*m) then
+      if (save_pdf) call save_velocity_pdf()
+      if (save_vcf) call save_vel_corr()
+      m = m+1
+    end if
+    if (t+im_t >= save_rate2*n) then
+      if (save_3d) call idl_surface()
+      n = n+1
+    end if
+    if (t+im_t >= p_save*ps) then
+      call end_state(0)
+      ps = ps+1
+    end if
 
     if (resident) then
       call solver_resident(1)
     else
-      psi%old(:,js:je,ks:ke) = psi%new
       call solver(psi%old, psi%new)
+    end if
+
+    ! save_run (src/io.f90:1566-1595): a SAVE file asks for a 3D save
+    inquire (file='SAVE', exist=state_exist)
+    if (state_exist) then
+      call idl_surface()
+      open (newunit=u, file='SAVE')
+      close (u, status='delete')
     end if
 
     if (eqn_to_solve == 4) then
@@ -135,24 +184,31 @@ program gpe_hip_main
         call get_norm(psi%new, norm)
       end if
       relnorm = abs((norm-prev_norm)/prev_norm)
-      if (.not. real_time) then
-        if (relnorm < 1e-12_pr) then
-          if (stop_imag) exit
-          real_time = .true.
-          im_t = 0.0_pr
-          dt = cmplx(abs(aimag(dt)), abs(real(dt, pr)), pr)
-          print *, 'Switching to real time'
-        end if
-        if (renorm) then
-          if (resident) then
-            call state_renormalise(norm)
-          else
-            call renormalise(psi%new, norm)
-          end if
-        end if
-      end if
-      prev_norm = norm
+      if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) call save_norm(t, norm, relnorm)
     end if
+    if (eqn_to_solve == 4 .and. .not. real_time) then
+      if (relnorm < 1e-12_pr) then
+        if (stop_imag) exit
+        real_time = .true.
+        im_t = 0.0_pr
+        ps = int(t/p_save)
+        n = int(t/save_rate2)
+        m = int(t/save_rate3)
+        if (save_3d) call idl_surface()
+        dt = cmplx(abs(aimag(dt)), abs(real(dt, pr)), pr)
+        print *, 'Switching to real time'
+        print *, 't =', im_t
+        print *, 'p =', p
+      end if
+      if (resident) then
+        if (renorm) call state_renormalise(norm)
+        if (imprint_vl) call state_imprint_vortex_line()
+      else
+        if (renorm) call renormalise(psi%new, norm)
+        if (imprint_vl) psi%new = imprint_vortex_line(psi%new)
+      end if
+    end if
+    prev_norm = norm
     p = p + 1
     nsteps = nsteps + 1
   end do
@@ -160,73 +216,165 @@ program gpe_hip_main
   call system_clock(c1)
   secs = real(c1 - c0, pr) / real(crate, pr)
 
-  ! end_state (src/io.f90:852-862)
-  open (newunit=u, file='end_state.dat', access='stream', status='replace')
-  write (u) nx
-  write (u) ny
-  write (u) nz
-  write (u) p
-  write (u) t, im_t
-  write (u) dt
-  write (u) psi%new
-  close (u)
+  call end_state(1)
+  if (save_3d) call idl_surface()
 
-  if (save_pdf .or. save_vcf .or. save_mom) call final_diagnostics()
-
-  inquire (file='RUNNING', exist=running)
-  if (running) then
-    open (newunit=u, file='RUNNING')
-    close (u, status='delete')
-  end if
   write (*, '(a,i0,a,es12.5,a,es12.5,a,f10.4,a)') 'steps=', nsteps, ' t=', t, ' im_t=', im_t, ' wall=', secs, ' s'
   if (secs > 0) write (*, '(a,f12.4)') 'Gcell-steps/s=', real(nx,pr)*ny*nz*nsteps/secs/1e9_pr
   call close_ctx()
 
 contains
 
-  ! The device diagnostics of the final state (state_put makes it the
-  ! context's state in either host mode).
-  subroutine final_diagnostics()
-    real(c_double) :: mom(3), vmean(3), vstdev(3), vmax(3), f(0:nx-1)
-    real(c_double), allocatable :: pdf(:,:)
-    integer(c_int64_t) :: cnt(3)
-    real (pr) :: bins(3), g(3)
-    integer :: i, a
-    character(7) :: tag
-    write (tag, '(i7.7)') p
-    call state_put(psi%new)
+  ! p as seven digits (itos, src/io.f90:29-47)
+  character(7) function itos(k)
+    integer, intent(in) :: k
+    write (itos, '(i7.7)') k
+  end function itos
+
+  subroutine make_dir(ix, name)
+    integer, intent(in) :: ix
+    character(*), intent(in) :: name
+    if (.not. dirs_made(ix)) then
+      call execute_command_line('mkdir -p '//name)
+      dirs_made(ix) = .true.
+    end if
+  end subroutine make_dir
+
+  ! the current state on the device (parity mode keeps it on the host)
+  subroutine on_device()
+    if (.not. resident) call state_put(psi%new)
+  end subroutine on_device
+
+  ! the current state in psi%new (resident mode keeps it on the device)
+  subroutine on_host()
+    if (resident) call state_get(psi%new)
+  end subroutine on_host
+
+  ! save_energy, save_time (src/io.f90:147-218) and momentum.dat
+  subroutine time_series()
+    real (pr) :: E, mom(3)
+    real(c_double) :: md(3)
+    complex (pr) :: c
+    call on_device()
+    call state_energy(E)
+    open (newunit=u, file='energy.dat', status='unknown', position='append')
+    write (u, '(3e17.9)') t, im_t, E/real(nx*ny*nz, pr)
+    close (u)
+    c = state_peek(nx/2, ny/2, nz/2)
+    open (newunit=u, file='psi_time.dat', status='unknown', position='append')
+    write (u, '(6e17.9)') t, im_t, real(c, pr), aimag(c), abs(c)**2, atan2(aimag(c)+1.0e-6_pr, real(c, pr))
+    close (u)
     if (save_mom) then
-      call check(gpe_momentum(the_ctx(), mom), 'momentum')
+      call check(gpe_momentum(the_ctx(), md), 'momentum')
+      mom = md
       open (newunit=u, file='momentum.dat', status='unknown', position='append')
       write (u, '(4e17.9)') t, mom(1), mom(2), mom(3)
       close (u)
     end if
-    if (save_pdf) then
-      allocate(pdf(nbins, 3))
-      call check(gpe_velocity_pdf(the_ctx(), int(nbins, c_int), cnt, vmean, vstdev, vmax, pdf), 'velocity_pdf')
-      open (newunit=u, file='pdf_vel'//tag//'.dat', status='unknown')
-      write (u, '(a10,e17.9)') '# t:', t
-      write (u, '(a10,3e17.9)') '# Mean:', vmean(1), vmean(2), vmean(3)
-      write (u, '(a10,3e17.9)') '# StDev:', vstdev(1), vstdev(2), vstdev(3)
-      write (u, '(a10,3e17.9)') '# Sum:', sum(pdf(:, 1)), sum(pdf(:, 2)), sum(pdf(:, 3))
-      do i = -nbins/2+1, nbins/2
-        do a = 1, 3
-          bins(a) = 2.0_pr*real(i, pr)*vmax(a)/real(nbins, pr)
-          g(a) = 1.0_pr/(vstdev(a)*sqrt(2.0_pr*pi)) * exp((-0.5_pr*(bins(a)-vmean(a))**2) / vstdev(a)**2)
-        end do
-        write (u, '(9e17.9)') (bins(a), pdf(i+nbins/2, a), g(a), a = 1, 3)
+  end subroutine time_series
+
+  ! save_norm (src/io.f90:406-422)
+  subroutine save_norm(time, nrm, rel)
+    real (pr), intent(in) :: time, nrm, rel
+    open (newunit=u, file='norm.dat', status='unknown', position='append')
+    write (u, '(4e17.9)') time, im_t, nrm, rel
+    close (u)
+  end subroutine save_norm
+
+  ! save_velocity_pdf (src/io.f90:222-290) from the device statistics
+  subroutine save_velocity_pdf()
+    real(c_double) :: vmean(3), vstdev(3), vmax(3)
+    real(c_double), allocatable :: pdf(:,:)
+    integer(c_int64_t) :: cnt(3)
+    real (pr) :: bins(3), g(3)
+    integer :: i, a
+    call on_device()
+    allocate(pdf(nbins, 3))
+    call check(gpe_velocity_pdf(the_ctx(), int(nbins, c_int), cnt, vmean, vstdev, vmax, pdf), 'velocity_pdf')
+    call make_dir(1, 'pdf')
+    open (newunit=u, file='pdf/pdf_vel'//itos(p)//'.dat', status='unknown')
+    write (u, '(a10,e17.9)') '# t:', t
+    write (u, '(a10,3e17.9)') '# Mean:', vmean(1), vmean(2), vmean(3)
+    write (u, '(a10,3e17.9)') '# StDev:', vstdev(1), vstdev(2), vstdev(3)
+    write (u, '(a10,3e17.9)') '# Sum:', sum(pdf(:, 1)), sum(pdf(:, 2)), sum(pdf(:, 3))
+    do i = -nbins/2+1, nbins/2
+      do a = 1, 3
+        bins(a) = 2.0_pr*real(i, pr)*vmax(a)/real(nbins, pr)
+        g(a) = 1.0_pr/(vstdev(a)*sqrt(2.0_pr*pi)) * exp((-0.5_pr*(bins(a)-vmean(a))**2) / vstdev(a)**2)
       end do
-      close (u)
+      write (u, '(9e17.9)') (bins(a), pdf(i+nbins/2, a), g(a), a = 1, 3)
+    end do
+    close (u)
+  end subroutine save_velocity_pdf
+
+  ! save_vel_corr (src/io.f90:294-311)
+  subroutine save_vel_corr()
+    real(c_double) :: f(0:nx-1)
+    integer :: i
+    call on_device()
+    call check(gpe_vcf(the_ctx(), f), 'get_vcf')
+    call make_dir(2, 'vcf')
+    open (newunit=u, file='vcf/vcf'//itos(p)//'.dat', status='unknown')
+    write (u, '(a4,e17.9)') '# t:', t
+    do i = 0, nx-1
+      write (u, '(i5,e17.9)') i, f(i)
+    end do
+    close (u)
+  end subroutine save_vel_corr
+
+  ! idl_surface (src/io.f90:489-529): t+im_t, the grid, the (single) rank's
+  ! block and the field, then the coordinates, in the stream layout the
+  ! reference's readers expect; p appended to p_saved.dat
+  subroutine idl_surface()
+    integer :: i
+    call on_host()
+    call make_dir(3, 'proc00')
+    open (newunit=u, file='proc00/dens'//itos(p)//'.dat', access='stream', status='replace')
+    write (u) t+im_t
+    write (u) nx, ny, nz
+    write (u) 1, 1
+    write (u) js, je, ks, ke
+    write (u) psi%new
+    write (u) (xl + real(i, pr)*dx, i = 0, nx1)
+    write (u) (yl + real(i, pr)*dy, i = 0, ny1)
+    write (u) (zl + real(i, pr)*dz, i = 0, nz1)
+    close (u)
+    open (newunit=u, file='p_saved.dat', status='unknown', position='append')
+    write (u, '(i10)') p
+    close (u)
+  end subroutine idl_surface
+
+  ! end_state (src/io.f90:840-891): the restart file, save.dat, and with
+  ! flag = 1 the removal of RUNNING
+  subroutine end_state(flag)
+    integer, intent(in) :: flag
+    call on_host()
+    open (newunit=u, file='end_state.dat', access='stream', status='replace')
+    write (u) nx
+    write (u) ny
+    write (u) nz
+    write (u) p
+    write (u) t, im_t
+    write (u) dt
+    write (u) psi%new
+    close (u)
+    open (newunit=u, file='save.dat', status='replace')
+    write (u, *) 'Periodically saved state'
+    write (u, *) 't =', t
+    write (u, *) 'im_t =', im_t
+    write (u, *) 'dt =', dt
+    write (u, *) 'nx =', nx
+    write (u, *) 'ny =', ny
+    write (u, *) 'nz =', nz
+    write (u, *) 'p =', p
+    close (u)
+    if (flag == 1) then
+      inquire (file='RUNNING', exist=running)
+      if (running) then
+        open (newunit=u, file='RUNNING')
+        close (u, status='delete')
+      end if
     end if
-    if (save_vcf) then
-      call check(gpe_vcf(the_ctx(), f), 'get_vcf')
-      open (newunit=u, file='vcf'//tag//'.dat', status='unknown')
-      write (u, '(a4,e17.9)') '# t:', t
-      do i = 0, nx-1
-        write (u, '(i5,e17.9)') i, f(i)
-      end do
-      close (u)
-    end if
-  end subroutine final_diagnostics
+  end subroutine end_state
 
 end program gpe_hip_main

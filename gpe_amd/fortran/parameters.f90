@@ -70,9 +70,18 @@ module parameters
   character(5) :: storage = 'c128'   ! 'c128', 'c64' or 'mixed' device fields
   real (pr)    :: vscale = 0.0_pr     ! V term in eqn 1 (0 = reference)
   character(8) :: host_mode = 'resident' ! 'resident' or 'parity' (copy per step)
-  logical      :: save_mom = .false.  ! momentum.dat of the final state
+  logical      :: save_mom = .false.  ! momentum.dat with the save_rate time series
 
-  namelist /grid_params/ nx, ny, nz, storage, vscale, host_mode, save_mom
+  ! vortex line of imprint_vl: line_param of the reference's parameters.in
+  ! (a compile-time constant there), here read with grid_params
+  type :: line_param
+    real (pr)    :: x0, y0, z0, amp1, amp2, ll, sgn
+    character(1) :: dir
+    logical      :: imprint_phase
+  end type line_param
+  type (line_param) :: vl1 = line_param(-1.5_pr, 0.0_pr, 0.0_pr, 0.0_pr, 0.0_pr, 0.0_pr, 1.0_pr, 'z', .true.)
+
+  namelist /grid_params/ nx, ny, nz, storage, vscale, host_mode, save_mom, vl1
 
   ! Parameters for adaptive time stepping (src/parameters.f90:89-93)
   real (pr), parameter :: eps         = 1e-6_pr

@@ -1,7 +1,8 @@
 ! Drop-in for the hot-path routines of the reference's module variables:
 ! laplacian (src/variables.f90:55-71), get_norm (537-559), energy (563-587),
 ! mass (591-611), renormalise (698-709), with the reference's interfaces,
-! and for its diagnostics momentum (615-694) and get_vcf (467-507).
+! for its diagnostics momentum (615-694) and get_vcf (467-507), and for
+! imprint_vortex_line (1148-1162).
 ! velocity_pdf replaces the sequence get_pdf_velocity + get_pdf x 3 of
 ! save_velocity_pdf (src/io.f90:236-239) with one call that returns the
 ! same PDFs, maxima, means and deviations.
@@ -16,7 +17,7 @@ module variables
 
   private
   public :: laplacian, get_norm, energy, mass, renormalise, state_norm, state_renormalise, momentum, get_vcf, &
-    velocity_pdf
+    velocity_pdf, imprint_vortex_line, state_imprint_vortex_line, state_energy, state_peek
 
   type, public :: var
     complex (pr), allocatable, dimension(:,:,:) :: new
@@ -129,5 +130,46 @@ contains
     real (pr), intent(in) :: norm
     call check(gpe_renormalise(the_ctx(), real(norm, c_double)), 'renormalise')
   end subroutine state_renormalise
+
+  ! imprint_vortex_line (src/variables.f90:1148-1162): |in_var| vortex_line(vl1)
+  function imprint_vortex_line(in_var)
+    complex (pr), dimension(0:nx1,js:je,ks:ke), intent(in) :: in_var
+    complex (pr), dimension(0:nx1,js:je,ks:ke) :: imprint_vortex_line
+    call put_interior(in_var)
+    call state_imprint_vortex_line()
+    call check(gpe_download(the_ctx(), imprint_vortex_line, host_kind()), 'download')
+  end function imprint_vortex_line
+
+  subroutine state_imprint_vortex_line()
+    type(gpe_line) :: ln
+    integer(c_int32_t) :: d
+    select case (vl1%dir)
+      case ('x')
+        d = 0
+      case ('y')
+        d = 1
+      case default
+        d = 2
+    end select
+    ln = gpe_line(vl1%x0, vl1%y0, vl1%z0, vl1%amp1, vl1%amp2, vl1%ll, vl1%sgn, d, &
+                  merge(1_c_int32_t, 0_c_int32_t, vl1%imprint_phase))
+    call check(gpe_imprint_line(the_ctx(), ln, real(scal, c_double)), 'imprint_vortex_line')
+  end subroutine state_imprint_vortex_line
+
+  ! energy of the device-resident state
+  subroutine state_energy(E)
+    real (pr), intent(out) :: E
+    real(c_double) :: v
+    call check(gpe_energy(the_ctx(), v), 'energy')
+    E = v
+  end subroutine state_energy
+
+  ! value of the device-resident state at (i, j, k)
+  complex (pr) function state_peek(i, j, k)
+    integer, intent(in) :: i, j, k
+    real(c_double) :: v(2)
+    call check(gpe_peek(the_ctx(), int(i, c_int32_t), int(j, c_int32_t), int(k, c_int32_t), v), 'peek')
+    state_peek = cmplx(v(1), v(2), pr)
+  end function state_peek
 
 end module variables

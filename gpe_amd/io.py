@@ -1,8 +1,15 @@
-"""The reference's end_state restart file (src/io.f90:840-891, read back by
-state_restart src/ic.f90:117-186): stream-unformatted int32 nx, ny, nz, p;
-float64 t, im_t; complex128 dt; complex128 psi (x fastest).  Used to hand
-initial states to the Fortran host (and to the reference build) and to read
-their final states back."""
+"""File formats of the reference kept by this package.
+
+* end_state restart file (src/io.f90:840-891, read back by state_restart
+  src/ic.f90:117-186): stream-unformatted int32 nx, ny, nz, p; float64 t,
+  im_t; complex128 dt; complex128 psi (x fastest).  Used to hand initial
+  states to the Fortran host (and to the reference build) and to read their
+  final states back.
+* dens<p>.dat 3D saves of idl_surface (src/io.f90:489-529): float64 t+im_t;
+  int32 nx, ny, nz, nyprocs, nzprocs, js, je, ks, ke; complex128 psi of the
+  rank's block; float64 x, y, z.
+* run.in with the reference's namelists (src/io.f90:78-114).
+"""
 from __future__ import annotations
 
 import numpy as np
@@ -29,6 +36,54 @@ def read_end_state(path: str, n_extra: int = 0):
         if n_extra:
             out = out + (np.fromfile(f, dtype="<f8", count=n_extra),)
     return out
+
+
+def write_dens(path: str, psi: np.ndarray, p, time: float) -> None:
+    """A one-rank dens file as idl_surface writes it; p: RunParams."""
+    nz, ny, nx = psi.shape
+    x, y, z = p.axes()
+    with open(path, "wb") as f:
+        np.array([time], dtype="<f8").tofile(f)
+        np.array([nx, ny, nz, 1, 1, 0, ny - 1, 0, nz - 1], dtype="<i4").tofile(f)
+        np.ascontiguousarray(psi, dtype="<c16").tofile(f)
+        for a in (x, y, z):
+            np.asarray(a, dtype="<f8").tofile(f)
+
+
+def read_dens(path: str) -> dict:
+    """Reads a dens file of one rank: time, grid, block bounds, psi (z, y, x
+    of the block), x, y, z."""
+    with open(path, "rb") as f:
+        time = float(np.fromfile(f, dtype="<f8", count=1)[0])
+        nx, ny, nz, nyp, nzp, js, je, ks, ke = (int(v) for v in np.fromfile(f, dtype="<i4", count=9))
+        nyl, nzl = je - js + 1, ke - ks + 1
+        psi = np.fromfile(f, dtype="<c16", count=nx * nyl * nzl).reshape(nzl, nyl, nx)
+        x = np.fromfile(f, dtype="<f8", count=nx)
+        y = np.fromfile(f, dtype="<f8", count=ny)
+        z = np.fromfile(f, dtype="<f8", count=nz)
+        rest = f.read()
+    if rest:
+        raise ValueError(f"{path}: {len(rest)} trailing bytes")
+    return dict(time=time, nx=nx, ny=ny, nz=nz, nyprocs=nyp, nzprocs=nzp, js=js, je=je, ks=ks, ke=ke, psi=psi,
+                x=x, y=y, z=z)
+
+
+def set_namelist(text: str, **values) -> str:
+    """Sets `name = value` entries of a run.in text (the entry must exist);
+    values are Python bools, numbers or strings."""
+    import re
+
+    for key, v in values.items():
+        if isinstance(v, bool):
+            lit = ".true." if v else ".false."
+        elif isinstance(v, str):
+            lit = f"'{v}'"
+        else:
+            lit = repr(v)
+        text, k = re.subn(rf"(?m)^(\s*{re.escape(key)}\s*=\s*).*$", lambda m: m.group(1) + lit, text)
+        if k != 1:
+            raise KeyError(f"run.in has no single entry {key!r}")
+    return text
 
 
 def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10, nbins: int = 256) -> str:
@@ -101,5 +156,6 @@ def run_in_text(p, end_time: float, restart: bool = True, save_rate: int = 10, n
   storage = '{p.storage}'
   vscale = {r(p.vscale)}
   host_mode = 'resident'
+  save_mom = .false.
 /
 """

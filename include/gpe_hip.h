@@ -167,6 +167,26 @@ int gpe_rhs(gpe_ctx* ctx, const void* in, int64_t ny_alloc, int64_t y_off, int64
 int gpe_norm(gpe_ctx* ctx, double* norm);
 int gpe_energy(gpe_ctx* ctx, double* energy);
 int gpe_mass(gpe_ctx* ctx, double* mass);
+/* A vortex line: line_param of the reference's parameters.in (a point on
+ * the line, displacement amplitudes amp1 (sin) and amp2 (cos) of
+ * wavelength ll, sign of the winding, direction, phase-only imprint). */
+typedef struct gpe_line {
+    double x0, y0, z0, amp1, amp2, ll, sgn;
+    int32_t dir;            /* 0 'x', 1 'y', 2 'z' */
+    int32_t imprint_phase;  /* 1: exp(i theta) only, no core */
+} gpe_line;
+
+/* psi <- |psi| vortex_line(vl) on the device state, with scal of
+ * eqn_params: imprint_vortex_line (src/variables.f90:1148-1162) over
+ * vortex_line, get_r and get_theta (src/ic.f90:327-348, 730-884).  Fails as
+ * get_r does when ll = 0 with a non-zero amplitude. */
+int gpe_imprint_line(gpe_ctx* ctx, const gpe_line* vl, double scal);
+
+/* The state's value at cell (i, j, k) (x fastest).  No reference
+ * counterpart: save_time (src/io.f90:147-195) reads psi at the box centre,
+ * which a device-resident host fetches with this instead of the field. */
+int gpe_peek(gpe_ctx* ctx, int32_t i, int32_t j, int32_t k, double out[2]);
+
 /* Momentum (variables::momentum, src/variables.f90:615-694): mom[a] is the
  * end-corrected quadrature (weights as get_norm) of
  * (Re psi - 1) Im(d_a psi*) - Im psi Re(d_a psi), a = x, y, z. */

@@ -53,6 +53,8 @@ def test_struct_layouts_match_header():
     # gpe_params: 8 int32 then 16 doubles; gpe_step_info: 6 doubles + 3 int64
     assert ct.sizeof(_lib.gpe_params) == 8 * 4 + 16 * 8
     assert ct.sizeof(_lib.gpe_step_info) == 6 * 8 + 3 * 8
+    # gpe_line: 7 doubles then 2 int32
+    assert ct.sizeof(_lib.gpe_line) == 7 * 8 + 2 * 4
 
 
 @pytest.mark.parametrize("field,value,msg", [

@@ -305,3 +305,37 @@ def test_fused_rk4_imaginary_time_and_fallback(orc):
         ctx.upload(psi)
         prof4 = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(0.01 + 0j)))
     assert "rk4_fused12" not in prof4 and "rk4_stage3" in prof4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["c128", "c64"])
+def test_imprint_line_matches_host_generator(storage):
+    """gpe_imprint_line: psi <- |psi| vortex_line(vl) against the host
+    restatement of vortex_line (itself pinned to the reference build) for
+    every direction, a displaced line, a phase-only line and scal != 1.
+    1e-13 (c128): device sin/cos/atan2/exp/pow and numpy's differ by a few
+    ulp; c64 rounds the stored result (2e-7)."""
+    from gpe_amd import ics
+
+    p = RunParams(nx=20, ny=18, nz=16, xr=4.0, yr=3.5, zr=3.0, storage=storage)
+    rng = np.random.default_rng(3)
+    psi = (1.0 + 0.3 * rng.standard_normal(p.shape) + 0.3j * rng.standard_normal(p.shape)).astype(np.complex128)
+    cases = [(ics.LineParam(0.3, -0.5, 0.2, 0.4, 0.3, 6.0, 1.0, "x", False), 1.0),
+             (ics.LineParam(-0.2, 0.1, 0.4, 0.5, 0.2, 5.0, -1.0, "y", True), 0.8),
+             (ics.LineParam(0.25, -0.35, 0.0, 0.0, 0.0, 0.0, 1.0, "z", False), 1.2)]
+    tol = 1e-13 if storage == "c128" else 2e-7
+    with Context(p) as ctx:
+        for vl, scal in cases:
+            ctx.upload(psi)
+            ctx.imprint_line(vl, scal)
+            got = ctx.download()
+            exp = np.abs(psi) * ics.vortex_line(p, vl, scal)
+            err = np.abs(got - exp).max() / np.abs(exp).max()
+            print(f"{storage} {vl.dir}: {err:.2e}")
+            assert err <= tol
+        assert ctx.peek(3, 4, 5) == got[5, 4, 3] or storage == "c64"
+        assert abs(ctx.peek(3, 4, 5) - got[5, 4, 3]) <= tol * abs(got[5, 4, 3])
+        with pytest.raises(GpeError):
+            ctx.imprint_line(ics.LineParam(amp1=0.5, ll=0.0))
+        with pytest.raises(GpeError):
+            ctx.peek(20, 0, 0)
