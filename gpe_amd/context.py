@@ -202,6 +202,22 @@ class Context:
                            int(bool(vl.imprint_phase)))
         self._chk(self.lib.gpe_imprint_line(self.h, ct.byref(ln), float(scal)), "gpe_imprint_line")
 
+    def condensed_particles(self, spectrum: bool = True) -> dict:
+        """condensed_particles (src/io.f90:315-404) and spectrum (632-725) of
+        the state: row = M/(8 xr yr zr), n0/V, M, n0, temp, temp2, H/V (the
+        mass.dat values after time and im_t); with spectrum, eta and nharm per
+        log2 shell (7) and F(kk), kk = 1..nx/2."""
+        row = (ct.c_double * 7)()
+        if not spectrum:
+            self._chk(self.lib.gpe_condensed_particles(self.h, row, None, None, None), "gpe_condensed_particles")
+            return dict(row=np.array(row[:]))
+        eta = (ct.c_double * 7)()
+        nharm = (ct.c_int64 * 7)()
+        F = np.zeros(self.params.nx // 2, dtype=np.float64)
+        self._chk(self.lib.gpe_condensed_particles(self.h, row, eta, nharm, F.ctypes.data_as(ct.POINTER(ct.c_double))),
+                  "gpe_condensed_particles")
+        return dict(row=np.array(row[:]), eta=np.array(eta[:]), nharm=np.array(nharm[:], dtype=np.int64), F=F)
+
     def peek(self, i: int, j: int, k: int) -> complex:
         """The state's value at cell (i, j, k)."""
         v = (ct.c_double * 2)()

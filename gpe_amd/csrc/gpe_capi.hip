@@ -11,6 +11,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <hipfft/hipfft.h>
+
 #include "../../include/gpe_hip.h"
 #include "gpe_fused.h"
 #include "gpe_host.h"
@@ -49,6 +51,12 @@ struct gpe_ctx {
     // diagnostics: velocity components (3 n doubles), histogram counts
     void* vel = nullptr;
     void* hist = nullptr;
+    // spectral diagnostics: FFT of the state, its hipFFT plan, row sums
+    void* fbuf = nullptr;
+    hipfftHandle fplan = 0;
+    bool fplan_ok = false;
+    double* sres = nullptr;
+    size_t sres_cap = 0;
     size_t hist_cap = 0;
     bool f0_valid = false;
     bool cand_valid = false;
@@ -743,8 +751,9 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     prof_clear(c);
     void* bufs[] = {c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
                     c->pnew,  c->sin,  c->sout, c->stage, c->partial, c->dres, c->dtab, c->mx[0], c->mx[1],
-                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist};
+                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist, c->fbuf, c->sres};
     for (void* b : bufs) dfree(c, b);
+    if (c->fplan_ok) (void)hipfftDestroy(c->fplan);
     if (c->hres) (void)hipHostFree(c->hres);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1075,6 +1084,94 @@ int gpe_peek(gpe_ctx* c, int32_t i, int32_t j, int32_t k, double out[2]) {
         HIPCHK(c, hipMemcpy(v, (const char*)c->psi + q * 8, 8, hipMemcpyDeviceToHost));
         out[0] = v[0];
         out[1] = v[1];
+    }
+    return 0;
+}
+
+int gpe_condensed_particles(gpe_ctx* c, double row[7], double eta[7], int64_t nharm[7], double* F) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_condensed_particles: ctx is NULL");
+    if (!row) return fail(c, GPE_EINVAL, "gpe_condensed_particles: row is NULL");
+    const bool spec = eta || nharm || F;
+    if (spec && !(eta && nharm && F))
+        return fail(c, GPE_EINVAL, "gpe_condensed_particles: eta, nharm and F are all given or all NULL");
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const bool dbl = c->esz == 16;
+    int rc = dalloc(c, &c->fbuf, (size_t)c->n * c->esz, "fft buffer");
+    if (rc) return rc;
+    if (!c->fplan_ok) {
+        // row-major plan: z slowest, x fastest
+        if (hipfftPlan3d(&c->fplan, nz, ny, nx, dbl ? HIPFFT_Z2Z : HIPFFT_C2C) != HIPFFT_SUCCESS)
+            return fail(c, GPE_EHIP, "gpe_condensed_particles: hipfftPlan3d failed");
+        c->fplan_ok = true;
+    }
+    if (hipfftSetStream(c->fplan, c->stream) != HIPFFT_SUCCESS)
+        return fail(c, GPE_EHIP, "gpe_condensed_particles: hipfftSetStream failed");
+    // a = FFT+(psi) (FFTW_BACKWARD), normalised by 1/sqrt(N) in the sums below
+    HIPCHK(c, hipMemcpyAsync(c->fbuf, c->psi, (size_t)c->n * c->esz, hipMemcpyDeviceToDevice, c->stream));
+    hipfftResult fr = dbl ? hipfftExecZ2Z(c->fplan, (hipfftDoubleComplex*)c->fbuf, (hipfftDoubleComplex*)c->fbuf,
+                                          HIPFFT_BACKWARD)
+                          : hipfftExecC2C(c->fplan, (hipfftComplex*)c->fbuf, (hipfftComplex*)c->fbuf, HIPFFT_BACKWARD);
+    if (fr != HIPFFT_SUCCESS) return fail(c, GPE_EHIP, "gpe_condensed_particles: hipfftExec failed");
+    const double V = (double)nx * (double)ny * (double)nz;
+    double a0[2];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (dbl) {
+        HIPCHK(c, hipMemcpy(a0, c->fbuf, 16, hipMemcpyDeviceToHost));
+    } else {
+        float f[2];
+        HIPCHK(c, hipMemcpy(f, c->fbuf, 8, hipMemcpyDeviceToHost));
+        a0[0] = f[0];
+        a0[1] = f[1];
+    }
+    // n0 = |a(0,0,0)|^2 (src/io.f90:335-343)
+    const double n0 = (a0[0] * a0[0] + a0[1] * a0[1]) / V;
+    double M;
+    if ((rc = norm_mass(c, &M, 0))) return rc;
+    const int nb = 512;
+    const int nhalf = nx / 2;
+    if ((rc = ensure_partial(c, (size_t)nb * (2 * kSpectrumShells + nhalf + 1)))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    const double rho0 = n0 / V;
+    HIPCHK(c, launch_condensed_tot(nx, ny, nz, d, rho0, c->partial, nb, c->dres, c->stream));
+    double tot;
+    if ((rc = read_result(c, &tot))) return rc;
+    // the mass.dat row after (time, im_t) (src/io.f90:375-386)
+    const double temp = (M - n0) / tot;
+    const double E0 = (1.0 / (2.0 * V)) * (M * M + (M - n0) * (M - n0));
+    const double H = E0 + temp * (V - 1.0);
+    const double vol = 8.0 * c->p.xr * c->p.yr * c->p.zr;
+    const double temp2 = ((M / vol) - (n0 / V)) / tot;
+    const double r7[7] = {M / vol, n0 / V, M, n0, temp, temp2, H / V};
+    for (int q = 0; q < 7; ++q) row[q] = r7[q];
+    if (!spec) return 0;
+    const size_t nrows = 2 * kSpectrumShells + nhalf;
+    if (c->sres_cap < nrows) {
+        dfree(c, c->sres);
+        c->sres = nullptr;
+        c->sres_cap = 0;
+        if ((rc = dalloc(c, (void**)&c->sres, nrows * sizeof(double), "spectrum sums"))) return rc;
+        c->sres_cap = nrows;
+    }
+    if (dbl) {
+        HIPCHK(c, launch_spectrum<double>((const double2*)c->fbuf, nx, ny, nz, nhalf, c->partial, nb, c->sres,
+                                          c->stream));
+    } else {
+        HIPCHK(c, launch_spectrum<float>((const float2*)c->fbuf, nx, ny, nz, nhalf, c->partial, nb, c->sres,
+                                         c->stream));
+    }
+    std::vector<double> h(nrows);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(h.data(), c->sres, nrows * sizeof(double), hipMemcpyDeviceToHost));
+    // spectrum (src/io.f90:700-716): mean occupation per shell, integral distribution
+    for (int m = 0; m < kSpectrumShells; ++m) {
+        nharm[m] = (int64_t)h[kSpectrumShells + m];
+        eta[m] = h[m] == 0.0 ? 0.0 : h[m] / (double)nharm[m];
+    }
+    double acc = 0.0;
+    for (int kk = 0; kk < nhalf; ++kk) {
+        acc += h[2 * kSpectrumShells + kk];
+        F[kk] = acc;
     }
     return 0;
 }

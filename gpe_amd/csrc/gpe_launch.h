@@ -113,6 +113,15 @@ hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const doub
 hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,
                       hipStream_t s);
 
+// Spectral diagnostics (gpe_spectral.hip); out holds 2 kSpectrumShells + nhalf
+// values: shell sums, shell counts, F bins.
+constexpr int kSpectrumShells = 7;
+template <typename R>
+hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, double* partial, int nb, double* out,
+                           hipStream_t s);
+hipError_t launch_condensed_tot(int nx, int ny, int nz, const double d[3], double rho0, double* partial, int nb,
+                                double* out, hipStream_t s);
+
 constexpr int kNormBlocks = 2048;
 
 }  // namespace gpe

@@ -1,0 +1,143 @@
+// Spectral diagnostics of the state, after its normalised backward FFT
+// a = FFT+(psi) / sqrt(nx ny nz) (`fft(in_var, a, 'backward')`,
+// src/ic.f90:905-1001, done by hipFFT in the caller):
+//   * spectrum (src/io.f90:632-725): per log2 shell m = 1..7 of
+//     log2k = log2(sqrt(k2) / (2 pi)) (|log2k| in [m-1, m)) the sum of |a|^2
+//     and the mode count, and the integral distribution
+//     F(kk) = sum over modes with sqrt(k2) <= kk of |a|^2, kk = 1..nx/2;
+//   * condensed_particles (src/io.f90:315-404): the k-grid sum
+//     tot = sum (k2 + rho0) / (k2^2 + 2 rho0 k2) over the 4th-order symbol k2
+//     of kx = -pi + i dk, dk = 2 pi / (nx - 1) (the same dk on every axis).
+// k2 of a mode index follows random_phase / spectrum: i^2 for
+// i <= (n-1)/2 + 1, else (n - i)^2.  Sums are FP64: per-thread, then per
+// block (fixed order), then over blocks in a fixed order; the F bins
+// accumulate in LDS with atomics, so their last bits depend on timing.
+#include <hip/hip_runtime.h>
+
+#include "gpe_launch.h"
+
+namespace gpe {
+
+constexpr int kSpThreads = 256;
+constexpr double kSpPi = 3.1415926535897932384626433832795;
+
+__device__ __forceinline__ long long mode_sq(int i, int n) {
+    const int n1 = n - 1;
+    const long long m = i <= n1 / 2 + 1 ? i : n1 - i + 1;
+    return m * m;
+}
+
+// Per block b, rows of partial[row * nb + b]: 0..6 the shell sums of |a|^2,
+// 7..13 the shell counts, 14 + (s - 1) for s = 1..nhalf the sum of |a|^2 over
+// the modes whose smallest kk with k2 <= kk^2 is s.
+template <typename R>
+__global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __restrict__ a, int nx, int ny, int nz,
+                                                                 int nhalf, double inv_n, double* __restrict__ partial) {
+    extern __shared__ double hsh[];  // nhalf bins, then kSpThreads / 64 reduction slots
+    double* red = hsh + nhalf;
+    for (int s = threadIdx.x; s < nhalf; s += kSpThreads) hsh[s] = 0.0;
+    __syncthreads();
+    double e[kSpectrumShells], cnt[kSpectrumShells];
+#pragma unroll
+    for (int m = 0; m < kSpectrumShells; ++m) e[m] = cnt[m] = 0.0;
+    const long long n = (long long)nx * ny * nz;
+    const double l2 = log(2.0);
+    for (long long q = blockIdx.x * (long long)kSpThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kSpThreads) {
+        const int i = (int)(q % nx);
+        const long long r = q / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const long long k2 = mode_sq(i, nx) + mode_sq(j, ny) + mode_sq(k, nz);
+        if (k2 == 0) continue;
+        const C<R> v = a[q];
+        const double p2 = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * inv_n;
+        // smallest kk with kk^2 >= k2, i.e. sqrt(k2) <= kk
+        long long s = (long long)sqrt((double)k2);
+        while (s * s < k2) ++s;
+        while (s > 1 && (s - 1) * (s - 1) >= k2) --s;
+        if (s <= nhalf) atomicAdd(&hsh[s - 1], p2);
+        const double lk = fabs(log(0.5 * sqrt((double)k2) / kSpPi) / l2);
+#pragma unroll
+        for (int m = 0; m < kSpectrumShells; ++m) {
+            if (lk < (double)(m + 1) && lk >= (double)m) {
+                e[m] += p2;
+                cnt[m] += 1.0;
+            }
+        }
+    }
+    const int nb = gridDim.x;
+#pragma unroll
+    for (int m = 0; m < kSpectrumShells; ++m) {
+        const double se = block_reduce<kSpThreads, RED_SUM>(e[m], red);
+        const double sc = block_reduce<kSpThreads, RED_SUM>(cnt[m], red);
+        if (threadIdx.x == 0) {
+            partial[(long long)m * nb + blockIdx.x] = se;
+            partial[(long long)(kSpectrumShells + m) * nb + blockIdx.x] = sc;
+        }
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nhalf; s += kSpThreads)
+        partial[(long long)(2 * kSpectrumShells + s) * nb + blockIdx.x] = hsh[s];
+}
+
+// -(1/12) (-2 cos(2 k d) + 32 cos(k d) - 30) / d^2 for one axis (the
+// 4th-order symbol of condensed_particles, src/io.f90:357-365)
+__device__ __forceinline__ double sym4(double kk, double d, double d2) {
+    return (-2.0 * cos(2.0 * kk * d) + 32.0 * cos(kk * d) - 30.0) / d2;
+}
+
+__global__ __launch_bounds__(kSpThreads) void k_tot_partial(int nx, int ny, int nz, double dk, double dx, double dy,
+                                                            double dz, double rho0, double* __restrict__ partial) {
+    __shared__ double red[kSpThreads / 64];
+    const long long n = (long long)nx * ny * nz;
+    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    double acc = 0.0;
+    for (long long q = blockIdx.x * (long long)kSpThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kSpThreads) {
+        const int i = (int)(q % nx);
+        const long long r = q / nx;
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const double kx = -kSpPi + (double)i * dk, ky = -kSpPi + (double)j * dk, kz = -kSpPi + (double)k * dk;
+        const double k2 = -(1.0 / 12.0) * (sym4(kx, dx, dx2) + sym4(ky, dy, dy2) + sym4(kz, dz, dz2));
+        if (k2 == 0.0) continue;
+        const double k4 = k2 * k2;
+        acc += (k2 + rho0) / (k4 + 2.0 * rho0 * k2);
+    }
+    const double s = block_reduce<kSpThreads, RED_SUM>(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[row] = sum over blocks b (in order) of partial[row * nb + b]
+__global__ __launch_bounds__(kSpThreads) void k_sum_rows(const double* __restrict__ partial, int nrows, int nb,
+                                                         double* __restrict__ out) {
+    const int row = blockIdx.x * kSpThreads + threadIdx.x;
+    if (row >= nrows) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[(long long)row * nb + b];
+    out[row] = s;
+}
+
+template <typename R>
+hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, double* partial, int nb, double* out,
+                           hipStream_t s) {
+    const double inv_n = 1.0 / ((double)nx * (double)ny * (double)nz);
+    const size_t lds = ((size_t)nhalf + kSpThreads / 64) * sizeof(double);
+    hipLaunchKernelGGL((k_spectrum_partial<R>), dim3(nb), dim3(kSpThreads), lds, s, a, nx, ny, nz, nhalf, inv_n,
+                       partial);
+    const int nrows = 2 * kSpectrumShells + nhalf;
+    hipLaunchKernelGGL(k_sum_rows, dim3((nrows + kSpThreads - 1) / kSpThreads), dim3(kSpThreads), 0, s, partial, nrows,
+                       nb, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_condensed_tot(int nx, int ny, int nz, const double d[3], double rho0, double* partial, int nb,
+                                double* out, hipStream_t s) {
+    const double dk = 2.0 * kSpPi / (double)(nx - 1);
+    hipLaunchKernelGGL(k_tot_partial, dim3(nb), dim3(kSpThreads), 0, s, nx, ny, nz, dk, d[0], d[1], d[2], rho0,
+                       partial);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kSpThreads), 0, s, partial, 1, nb, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_spectrum<double>(const double2*, int, int, int, int, double*, int, double*, hipStream_t);
+template hipError_t launch_spectrum<float>(const float2*, int, int, int, int, double*, int, double*, hipStream_t);
+
+}  // namespace gpe

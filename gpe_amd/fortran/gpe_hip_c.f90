@@ -126,6 +126,14 @@ module gpe_hip_c
       real(c_double), value :: scal
     end function gpe_imprint_line
 
+    integer(c_int) function gpe_condensed_particles(ctx, row, eta, nharm, f) &
+        bind(C, name='gpe_condensed_particles')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: row(7)
+      type(c_ptr), value :: eta, nharm, f
+    end function gpe_condensed_particles
+
     integer(c_int) function gpe_peek(ctx, i, j, k, v) bind(C, name='gpe_peek')
       import
       type(c_ptr), value :: ctx

@@ -7,9 +7,9 @@
 !   * every save_rate steps (modulo(t+im_t, |dt| save_rate) < |dt|) the
 !     time series energy.dat and psi_time.dat (save_energy, save_time,
 !     src/io.f90:147-218), and momentum.dat with save_mom;
-!   * at save_rate3 intervals the velocity PDF and correlation function
-!     (save_pdf, save_vcf: pdf/pdf_vel<p>.dat, vcf/vcf<p>.dat,
-!     src/io.f90:222-311);
+!   * at save_rate3 intervals condensed_particles and the velocity PDF and
+!     correlation function (save_pdf, save_vcf: pdf/pdf_vel<p>.dat,
+!     vcf/vcf<p>.dat, src/io.f90:222-311);
 !   * at save_rate2 intervals the 3D state with save_3d (idl_surface,
 !     proc00/dens<p>.dat, src/io.f90:489-529);
 !   * at p_save intervals the periodic end_state.dat and save.dat
@@ -17,20 +17,22 @@
 !   * the solver step, then the SAVE sentinel (save_run, src/io.f90:1566-1595);
 !   * for eqn 4 the norm, norm.dat, the imaginary-time switch, renorm and
 !     imprint_vl (gpe.f90:241-271).
-! After the loop: end_state.dat (RUNNING removed) and, with save_3d, the
-! final dens file.  p is written as seven digits, as itos forms it.  Not
-! built: condensed_particles / spectra (mass.dat, spectrum/, idf/), line
-! length (save_ll), zeros, contours and averages.
+! condensed_particles (mass.dat; with save_spectrum spectrum/, idf/ and
+! eta_time.dat, src/io.f90:315-404, 632-725) runs before the loop and at
+! save_rate3 intervals, as in the reference.  After the loop: end_state.dat
+! (RUNNING removed) and, with save_3d, the final dens file.  p is written as
+! seven digits, as itos forms it.  Not built: line length (save_ll), zeros,
+! contours, averages and filtered surfaces.
 !
 ! host_mode = 'parity' copies psi%new -> psi%old and calls
 ! solver(psi%old, psi%new) every step exactly as the reference does;
 ! host_mode = 'resident' keeps psi on the GPU for the whole run.
 program gpe_hip_main
-  use iso_c_binding, only : c_double, c_int, c_int64_t
+  use iso_c_binding, only : c_double, c_int, c_int64_t, c_loc, c_null_ptr
   use parameters
   use error
   use gpe_context, only : close_ctx, gpe_storage, gpe_vscale, the_ctx, check
-  use gpe_hip_c, only : gpe_momentum, gpe_velocity_pdf, gpe_vcf
+  use gpe_hip_c, only : gpe_momentum, gpe_velocity_pdf, gpe_vcf, gpe_condensed_particles
   use solve
   use variables
   implicit none
@@ -39,7 +41,7 @@ program gpe_hip_main
   real (pr) :: norm, prev_norm, relnorm
   logical :: state_exist, running, resident
   integer :: u, ios, nx_prev, ny_prev, nz_prev, nsteps, ps, n, m
-  logical :: dirs_made(3) = .false.
+  logical :: dirs_made(4) = .false.
   integer(8) :: c0, c1, crate
   complex (pr) :: dt_prev
   real (pr) :: secs
@@ -133,6 +135,7 @@ program gpe_hip_main
     prev_norm = 1.0_pr
   end if
   norm = prev_norm
+  call condensed_particles(t)
 
   ps = int(t/p_save)
   n = int(t/save_rate2)
@@ -150,6 +153,7 @@ program gpe_hip_main
 
     if (modulo(t+im_t, abs(dt)*save_rate) < abs(dt)) call time_series()
     if (t+im_t >= save_rate3*m) then
+      call condensed_particles(t)
       if (save_pdf) call save_velocity_pdf()
       if (save_vcf) call save_vel_corr()
       m = m+1
@@ -272,6 +276,41 @@ contains
       close (u)
     end if
   end subroutine time_series
+
+  ! condensed_particles (src/io.f90:315-404): mass.dat, and with
+  ! save_spectrum the spectrum (src/io.f90:632-725): spectrum/spectrum<p>.dat,
+  ! idf/idf<p>.dat, eta_time.dat
+  subroutine condensed_particles(time)
+    real (pr), intent(in) :: time
+    real(c_double) :: row(7)
+    real(c_double), target :: eta(7), f(nx/2)
+    integer(c_int64_t), target :: nharm(7)
+    integer :: i
+    call on_device()
+    if (save_spectrum) then
+      call check(gpe_condensed_particles(the_ctx(), row, c_loc(eta), c_loc(nharm), c_loc(f)), 'condensed_particles')
+    else
+      call check(gpe_condensed_particles(the_ctx(), row, c_null_ptr, c_null_ptr, c_null_ptr), 'condensed_particles')
+    end if
+    open (newunit=u, file='mass.dat', status='unknown', position='append')
+    write (u, '(9e17.9)') time, im_t, (row(i), i = 1, 7)
+    close (u)
+    if (.not. save_spectrum) return
+    call make_dir(4, 'spectrum idf')
+    open (newunit=u, file='spectrum/spectrum'//itos(p)//'.dat', status='unknown')
+    do i = 1, 7
+      write (u, '(2i9,e17.9)') i, int(nharm(i)), eta(i)
+    end do
+    close (u)
+    open (newunit=u, file='idf/idf'//itos(p)//'.dat', status='unknown')
+    do i = 1, nx/2
+      write (u, '(i9,e17.9)') i, f(i)
+    end do
+    close (u)
+    open (newunit=u, file='eta_time.dat', status='unknown', position='append')
+    write (u, '(4e17.9)') t, eta(1), eta(2), eta(3)
+    close (u)
+  end subroutine condensed_particles
 
   ! save_norm (src/io.f90:406-422)
   subroutine save_norm(time, nrm, rel)

@@ -187,6 +187,18 @@ int gpe_imprint_line(gpe_ctx* ctx, const gpe_line* vl, double scal);
  * which a device-resident host fetches with this instead of the field. */
 int gpe_peek(gpe_ctx* ctx, int32_t i, int32_t j, int32_t k, double out[2]);
 
+/* Condensed particles and spectra of the state: condensed_particles
+ * (src/io.f90:315-404) and, when eta, nharm and F are given, spectrum
+ * (src/io.f90:632-725), from the normalised backward FFT
+ * a = FFT+(psi)/sqrt(nx ny nz) (fft 'backward', src/ic.f90:905-1001; hipFFT
+ * here).  row = the mass.dat values after (time, im_t): M/(8 xr yr zr),
+ * n0/V, M, n0, temp, temp2, H/V, with n0 = |a(0,0,0)|^2 and M the mass.
+ * eta[m], nharm[m]: mean |a|^2 and mode count of log2 shell m+1 (eta 0 where
+ * the shell holds nothing); F[kk-1], kk = 1..nx/2: sum of |a|^2 over the
+ * modes with sqrt(k2) <= kk.  The first call allocates a field-sized FFT
+ * buffer and a hipFFT plan, kept until gpe_ctx_destroy. */
+int gpe_condensed_particles(gpe_ctx* ctx, double row[7], double eta[7], int64_t nharm[7], double* F);
+
 /* Momentum (variables::momentum, src/variables.f90:615-694): mom[a] is the
  * end-corrected quadrature (weights as get_norm) of
  * (Re psi - 1) Im(d_a psi*) - Im psi Re(d_a psi), a = x, y, z. */

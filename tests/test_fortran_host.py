@@ -199,7 +199,8 @@ def schedule(nsteps, tau, save_rate, save_rate2, save_rate3, p_save):
 def test_host_save_schedule_and_files(tmp_path, mode):
     """The main loop's outputs follow the reference's schedule: energy.dat
     and psi_time.dat every save_rate steps (modulo(t, |dt| save_rate) < |dt|
-    on the accumulated t), the velocity PDF / VCF at save_rate3 intervals,
+    on the accumulated t), condensed_particles (mass.dat, spectra) before the
+    loop and with the velocity PDF / VCF at save_rate3 intervals,
     dens files at save_rate2 intervals and at the end, end_state.dat and
     save.dat at p_save intervals and at the end.  Every value is checked
     against the same run driven step by step through the C ABI."""
@@ -208,21 +209,24 @@ def test_host_save_schedule_and_files(tmp_path, mode):
     nsteps, nbins = 25, 32
     cfg = dict(save_rate=4, save_rate2=0.07, save_rate3=0.05, p_save=0.12)
     text = run_in_text(p, (nsteps - 0.5) * p.tau, nbins=nbins)
-    text = set_namelist(text, save_3d=True, save_pdf=True, save_vcf=True, save_mom=True, host_mode=mode, **cfg)
+    text = set_namelist(text, save_3d=True, save_pdf=True, save_vcf=True, save_mom=True, save_spectrum=True,
+                        host_mode=mode, **cfg)
     (tmp_path / "run.in").write_text(text)
     write_end_state(str(tmp_path / "end_state00.dat"), psi, 0, 0.0, 0.0, p.dt0())
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     events, t_end = schedule(nsteps, p.tau, cfg["save_rate"], cfg["save_rate2"], cfg["save_rate3"], cfg["p_save"])
-    ts_rows, dens_ps, diag_ps = [], [], []
+    ts_rows, dens_ps, diag_ps, cond = [], [], [], []
     with Context(p) as ctx:
         ctx.upload(psi)
+        cond.append((0, 0.0, ctx.condensed_particles()))  # before the loop (src/gpe.f90:113)
         info = StepInfo(p.dt0())
         for ev in events:
             if ev["ts"]:
                 c = ctx.peek(p.nx // 2, p.ny // 2, p.nz // 2)
                 ts_rows.append((ev["t"], ctx.energy() / (p.nx * p.ny * p.nz), c, ctx.momentum()))
             if ev["diag"]:
+                cond.append((ev["p"], ev["t"], ctx.condensed_particles()))
                 diag_ps.append((ev["p"], ctx.velocity_pdf(nbins), ctx.vcf()))
             if ev["dens"]:
                 dens_ps.append((ev["p"], ev["t"], ctx.download()))
@@ -247,6 +251,20 @@ def test_host_save_schedule_and_files(tmp_path, mode):
         assert np.allclose(head["Mean"], st["vmean"], rtol=1e-8, atol=1e-12)
         assert np.allclose(rows[:, 1], st["pdf"][0], rtol=1e-8, atol=1e-12)
         assert np.allclose(read_rows(tmp_path / "vcf" / f"vcf{pp:07d}.dat")[:, 1], f, rtol=1e-8, atol=1e-9)
+    # condensed_particles: mass.dat rows; spectrum, idf and eta_time.dat
+    ma = read_rows(tmp_path / "mass.dat")
+    et = read_rows(tmp_path / "eta_time.dat")
+    assert len(ma) == len(et) == len(cond)
+    for k, (pp, t, c) in enumerate(cond):
+        assert abs(ma[k, 0] - t) <= 1e-8 * max(t, 1e-3) and ma[k, 1] == 0.0
+        assert np.allclose(ma[k, 2:], c["row"], rtol=1e-8, atol=0)
+        assert np.allclose(et[k, 1:], c["eta"][:3], rtol=1e-8, atol=1e-300)
+        sp = read_rows(tmp_path / "spectrum" / f"spectrum{pp:07d}.dat")
+        assert np.array_equal(sp[:, 0], np.arange(1, 8)) and np.array_equal(sp[:, 1], c["nharm"])
+        assert np.allclose(sp[:, 2], c["eta"], rtol=1e-8, atol=1e-300)
+        idf = read_rows(tmp_path / "idf" / f"idf{pp:07d}.dat")
+        assert np.array_equal(idf[:, 0], np.arange(1, p.nx // 2 + 1))
+        assert np.allclose(idf[:, 1], c["F"], rtol=1e-8, atol=1e-300)
     # dens files: save_rate2 schedule plus the final state
     dens = sorted(os.listdir(tmp_path / "proc00"))
     want = sorted({f"dens{pp:07d}.dat" for pp, _, _ in dens_ps} | {f"dens{nsteps:07d}.dat"})
