@@ -72,6 +72,7 @@ PROTOTYPES = [
     ("gpe_mass", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_momentum", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_imprint_line", _I, [_VP, ct.POINTER(gpe_line), _D]),
+    ("gpe_linelength", _I, [_VP, ct.POINTER(_D)]),
     ("gpe_condensed_particles", _I, [_VP, ct.POINTER(_D), ct.POINTER(_D), ct.POINTER(ct.c_int64), ct.POINTER(_D)]),
     ("gpe_peek", _I, [_VP, ct.c_int32, ct.c_int32, ct.c_int32, ct.POINTER(_D)]),
     ("gpe_velocity_pdf", _I, [_VP, _I, ct.POINTER(_I64), ct.POINTER(_D), ct.POINTER(_D), ct.POINTER(_D),

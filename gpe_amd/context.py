@@ -202,6 +202,13 @@ class Context:
                            int(bool(vl.imprint_phase)))
         self._chk(self.lib.gpe_imprint_line(self.h, ct.byref(ln), float(scal)), "gpe_imprint_line")
 
+    def linelength(self) -> float:
+        """linelength (src/variables.f90:879-1145): total length of the zero
+        lines of psi (dx = dy = dz assumed, as the reference does)."""
+        v = ct.c_double()
+        self._chk(self.lib.gpe_linelength(self.h, ct.byref(v)), "gpe_linelength")
+        return v.value
+
     def condensed_particles(self, spectrum: bool = True) -> dict:
         """condensed_particles (src/io.f90:315-404) and spectrum (632-725) of
         the state: row = M/(8 xr yr zr), n0/V, M, n0, temp, temp2, H/V (the

@@ -57,6 +57,10 @@ struct gpe_ctx {
     bool fplan_ok = false;
     double* sres = nullptr;
     size_t sres_cap = 0;
+    // line length: per-line records, carries, block partials (+ a complex128
+    // copy of a complex64 state)
+    double* llbuf = nullptr;
+    void* llpsi = nullptr;
     size_t hist_cap = 0;
     bool f0_valid = false;
     bool cand_valid = false;
@@ -751,7 +755,7 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     prof_clear(c);
     void* bufs[] = {c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
                     c->pnew,  c->sin,  c->sout, c->stage, c->partial, c->dres, c->dtab, c->mx[0], c->mx[1],
-                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist, c->fbuf, c->sres};
+                    c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist, c->fbuf, c->sres, c->llbuf, c->llpsi};
     for (void* b : bufs) dfree(c, b);
     if (c->fplan_ok) (void)hipfftDestroy(c->fplan);
     if (c->hres) (void)hipHostFree(c->hres);
@@ -1085,6 +1089,31 @@ int gpe_peek(gpe_ctx* c, int32_t i, int32_t j, int32_t k, double out[2]) {
         out[0] = v[0];
         out[1] = v[1];
     }
+    return 0;
+}
+
+int gpe_linelength(gpe_ctx* c, double* length) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_linelength: ctx is NULL");
+    if (!length) return fail(c, GPE_EINVAL, "gpe_linelength: output is NULL");
+    const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
+    const long long nl = linelength_lines(ny, nz), nb = linelength_blocks(ny, nz);
+    int rc = dalloc(c, (void**)&c->llbuf, (size_t)(7 * nl + nb + 1) * sizeof(double), "line length");
+    if (rc) return rc;
+    const double2* src = (const double2*)c->psi;
+    if (c->p.storage == GPE_C64) {
+        if ((rc = dalloc(c, &c->llpsi, (size_t)c->n * 16, "line length c128 copy"))) return rc;
+        HIPCHK(c, launch_to_c128<float>((const float2*)c->psi, (double2*)c->llpsi, c->n, c->stream));
+        src = (const double2*)c->llpsi;
+    }
+    double* last = c->llbuf;
+    double* carry = last + 4 * nl;
+    double* partial = carry + 3 * nl;
+    HIPCHK(c, launch_linelength(src, nx, ny, nz, last, carry, partial, c->dres, c->stream));
+    double s;
+    if ((rc = read_result(c, &s))) return rc;
+    double d[3];
+    spacing(c->p, d);
+    *length = s * d[0];  // linelength = linelength*dx (src/variables.f90:1141)
     return 0;
 }
 

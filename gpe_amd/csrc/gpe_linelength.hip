@@ -1,0 +1,316 @@
+// Line length of the zero lines of psi (linelength, src/variables.f90:879-1145).
+//
+// Per interior cell (i, j, k), each axis in 1..n-2, the reference looks on
+// the cell's six faces for the crossing point of the zero lines of Re psi
+// and Im psi (a "hit"), ends the cell with 1/2 on a face where one part
+// vanishes at all four corners while the other changes sign, or with 1/4 on
+// an edge where both vanish, and otherwise adds a |h1 - h2| for the first
+// two hits (a = 1, 1/2 or 1/4 for hits on shared faces).  Its hit slots are
+// local arrays that persist from cell to cell, so a cell with a single hit
+// pairs it with the second hit of the last earlier cell (in k, j, i order)
+// that recorded two.  That one serial dependency is carried here in three
+// passes:
+//   1. one wavefront per x-line (j, k) finds the line's last two-hit cell;
+//   2. one block scans the lines in order: the carry into each line;
+//   3. one wavefront per line evaluates its cells in chunks of 64, resolves
+//      single-hit cells against the last two-hit lane before them in the
+//      chunk (ballot + shuffle) or the carry, and sums the contributions.
+// The cell arithmetic keeps the reference's operation order with
+// contraction off, so every cell's contribution and hits are bit-identical
+// to the reference's; only the order of the final sum differs.
+#include <hip/hip_runtime.h>
+
+#include "gpe_launch.h"
+
+namespace gpe {
+
+constexpr int kLlThreads = 256;  // 4 wavefronts = 4 lines per block
+
+// corners of face n (u(1)..u(4)) as (di, dj, dk)
+constexpr signed char kLlFace[6][4][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}},  // 1 front  (k)
+    {{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},  // 2 back   (k+1)
+    {{1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}},  // 3 right  (i+1)
+    {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}},  // 4 left   (i)
+    {{0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}},  // 5 top    (j+1)
+    {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}},  // 6 bottom (j)
+};
+
+struct LlCell {
+    int nhits;     // hits recorded (before an ending face or edge)
+    int ended;     // 0, 1 (face: 1/2), 2 (edge: 1/4)
+    double h1[3];  // first hit
+    double h2[3];  // second hit (nhits >= 2)
+};
+
+// Sign-change crossings of the zero line of w along the edges 1-2, 2-3,
+// 3-4, 4-1 of a face: the first two distinct ones in (xs, ys)[0..1];
+// returns how many were kept (the reference's lu - 1).
+__device__ __forceinline__ int ll_cross(const double* w, double* xs, double* ys) {
+#pragma clang fp contract(off)
+    int n = 0;
+    double px = 0.0, py = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double a = w[q], b = w[q + 1];
+        if (a != 0.0 && a * b < 0.0) {
+            double x, y;
+            if (q == 0) {
+                x = fabs(a / (a - b));
+                y = 0.0;
+            } else if (q == 1) {
+                x = 1.0;
+                y = fabs(a / (a - b));
+            } else if (q == 2) {
+                x = fabs(b / (a - b));
+                y = 1.0;
+            } else {
+                x = 0.0;
+                y = fabs(b / (a - b));
+            }
+            if (n == 0 || x != px || y != py) {
+                if (n == 0) {
+                    xs[0] = x;
+                    ys[0] = y;
+                } else if (n == 1) {
+                    xs[1] = x;
+                    ys[1] = y;
+                }
+                ++n;
+            }
+            px = x;
+            py = y;
+        }
+    }
+    return n;
+}
+
+__device__ LlCell ll_cell(const double2* __restrict__ psi, int nx, int ny, int i, int j, int k) {
+#pragma clang fp contract(off)
+    LlCell r;
+    r.nhits = 0;
+    r.ended = 0;
+    for (int c = 0; c < 3; ++c) r.h1[c] = r.h2[c] = 0.0;
+    double2 cr[2][2][2];
+#pragma unroll
+    for (int dk = 0; dk < 2; ++dk)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di)
+                cr[dk][dj][di] = psi[((long long)(k + dk) * ny + (j + dj)) * nx + (i + di)];
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+        double u[5], v[5];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double2 z = cr[kLlFace[n][c][2]][kLlFace[n][c][1]][kLlFace[n][c][0]];
+            u[c] = z.x;
+            v[c] = z.y;
+        }
+        u[4] = u[0];
+        v[4] = v[0];
+        const bool uzero = u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0 && u[3] == 0.0;
+        const bool vzero = v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0 && v[3] == 0.0;
+        const bool vchange = v[0] * v[1] < 0.0 || v[1] * v[2] < 0.0 || v[2] * v[3] < 0.0 || v[3] * v[0] < 0.0;
+        const bool uchange = u[0] * u[1] < 0.0 || u[1] * u[2] < 0.0 || u[2] * u[3] < 0.0 || u[3] * u[0] < 0.0;
+        if ((uzero && vchange) || (vzero && uchange)) {
+            r.ended = 1;
+            return r;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (u[q] == 0.0 && u[q + 1] == 0.0 && v[q] == 0.0 && v[q + 1] == 0.0) {
+                r.ended = 2;
+                return r;
+            }
+        double xu[2], yu[2], xv[2], yv[2];
+        if (ll_cross(u, xu, yu) < 2 || ll_cross(v, xv, yv) < 2) continue;
+        double den = xv[1] * (yu[0] - yu[1]) + xv[0] * (yu[1] - yu[0]) + (xu[0] - xu[1]) * (yv[0] - yv[1]);
+        if (den == 0.0) den = den + 0.0000001;
+        const double xx = (xu[0] * (xv[1] * (yv[0] - yu[1]) + xv[0] * (yu[1] - yv[1])) +
+                           xu[1] * (xv[1] * (yu[0] - yv[0]) + xv[0] * (yv[1] - yu[0]))) /
+                          den;
+        const double yy = (xv[1] * (yu[0] - yu[1]) * yv[0] + xu[0] * yu[1] * yv[0] - xv[0] * yu[0] * yv[1] -
+                           xu[0] * yu[1] * yv[1] + xv[0] * yu[1] * yv[1] + xu[1] * yu[0] * (yv[1] - yv[0])) /
+                          den;
+        if (xx >= 0.0 && xx <= 1.0 && yy >= 0.0 && yy <= 1.0) {
+            double h[3];
+            switch (n) {
+                case 0: h[0] = xx, h[1] = yy, h[2] = 0.0; break;
+                case 1: h[0] = xx, h[1] = yy, h[2] = 1.0; break;
+                case 2: h[0] = 1.0, h[1] = yy, h[2] = xx; break;
+                case 3: h[0] = 0.0, h[1] = yy, h[2] = xx; break;
+                case 4: h[0] = xx, h[1] = 1.0, h[2] = yy; break;
+                default: h[0] = xx, h[1] = 0.0, h[2] = yy; break;
+            }
+            if (r.nhits == 0)
+                for (int c = 0; c < 3; ++c) r.h1[c] = h[c];
+            else if (r.nhits == 1)
+                for (int c = 0; c < 3; ++c) r.h2[c] = h[c];
+            ++r.nhits;
+        }
+    }
+    return r;
+}
+
+// a |h1 - h2| of a cell that ended its face loop with hits
+__device__ __forceinline__ double ll_pair(const double* h1, const double* h2) {
+#pragma clang fp contract(off)
+    const bool xs = h1[0] == h2[0] && (h1[0] == 0.0 || h1[0] == 1.0);
+    const bool ys = h1[1] == h2[1] && (h1[1] == 0.0 || h1[1] == 1.0);
+    const bool zs = h1[2] == h2[2] && (h1[2] == 0.0 || h1[2] == 1.0);
+    double a = 1.0;
+    if (xs || ys) {  // the reference tests y twice where z is meant
+        a = a * 0.5;
+        if ((xs && ys) || (ys && zs)) a = a * 0.5;
+    }
+    const double dx = h1[0] - h2[0], dy = h1[1] - h2[1], dz = h1[2] - h2[2];
+    return a * sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+struct LlGeom {
+    int nx, ny, nz;
+    long long nlines;  // (ny - 2) (nz - 2)
+};
+
+__device__ __forceinline__ void ll_line(const LlGeom& g, long long line, int& j, int& k) {
+    j = 1 + (int)(line % (g.ny - 2));
+    k = 1 + (int)(line / (g.ny - 2));
+}
+
+__device__ __forceinline__ int ll_last_lane(unsigned long long mask) { return 63 - __clzll(mask); }
+
+// Pass 1: last[line] = (1, h2) of the line's last two-hit cell, or (0, ...).
+__global__ __launch_bounds__(kLlThreads) void k_ll_last(const double2* __restrict__ psi, LlGeom g,
+                                                        double* __restrict__ last) {
+    const int lane = threadIdx.x & 63;
+    const long long line = (long long)blockIdx.x * (kLlThreads / 64) + (threadIdx.x >> 6);
+    if (line >= g.nlines) return;  // uniform per wavefront
+    int j, k;
+    ll_line(g, line, j, k);
+    double has = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+    for (int i0 = 1; i0 <= g.nx - 2; i0 += 64) {
+        const int i = i0 + lane;
+        LlCell c{};
+        if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
+        const unsigned long long w = __ballot(c.nhits >= 2);
+        if (w) {
+            const int src = ll_last_lane(w);
+            lx = __shfl(c.h2[0], src, 64);
+            ly = __shfl(c.h2[1], src, 64);
+            lz = __shfl(c.h2[2], src, 64);
+            has = 1.0;
+        }
+    }
+    if (lane == 0) {
+        last[4 * line + 0] = has;
+        last[4 * line + 1] = lx;
+        last[4 * line + 2] = ly;
+        last[4 * line + 3] = lz;
+    }
+}
+
+// Pass 2 (one block of 1024 threads): carry[line] = the second hit of the
+// last two-hit cell of the lines before it, (0, 0, 0) before any.
+constexpr int kLlScan = 1024;
+__global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__ last, long long nlines,
+                                                      double* __restrict__ carry) {
+    __shared__ double chunk_last[kLlScan][4];
+    __shared__ double chunk_in[kLlScan][3];
+    const long long per = (nlines + kLlScan - 1) / kLlScan;
+    const long long l0 = (long long)threadIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
+    double has = 0.0, x = 0.0, y = 0.0, z = 0.0;
+    for (long long l = l1 - 1; l >= l0; --l)
+        if (last[4 * l] != 0.0) {
+            has = 1.0, x = last[4 * l + 1], y = last[4 * l + 2], z = last[4 * l + 3];
+            break;
+        }
+    chunk_last[threadIdx.x][0] = has;
+    chunk_last[threadIdx.x][1] = x;
+    chunk_last[threadIdx.x][2] = y;
+    chunk_last[threadIdx.x][3] = z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int t = 0; t < kLlScan; ++t) {
+            chunk_in[t][0] = cx, chunk_in[t][1] = cy, chunk_in[t][2] = cz;
+            if (chunk_last[t][0] != 0.0) cx = chunk_last[t][1], cy = chunk_last[t][2], cz = chunk_last[t][3];
+        }
+    }
+    __syncthreads();
+    x = chunk_in[threadIdx.x][0], y = chunk_in[threadIdx.x][1], z = chunk_in[threadIdx.x][2];
+    for (long long l = l0; l < l1; ++l) {
+        carry[3 * l + 0] = x, carry[3 * l + 1] = y, carry[3 * l + 2] = z;
+        if (last[4 * l] != 0.0) x = last[4 * l + 1], y = last[4 * l + 2], z = last[4 * l + 3];
+    }
+}
+
+// Pass 3: partial[block] = the block's sum of cell contributions.
+__global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict__ psi, LlGeom g,
+                                                       const double* __restrict__ carry, double* __restrict__ partial) {
+    __shared__ double red[kLlThreads / 64];
+    const int lane = threadIdx.x & 63;
+    const long long line = (long long)blockIdx.x * (kLlThreads / 64) + (threadIdx.x >> 6);
+    double acc = 0.0;
+    if (line < g.nlines) {  // uniform per wavefront
+        int j, k;
+        ll_line(g, line, j, k);
+        double cx = carry[3 * line + 0], cy = carry[3 * line + 1], cz = carry[3 * line + 2];
+        for (int i0 = 1; i0 <= g.nx - 2; i0 += 64) {
+            const int i = i0 + lane;
+            LlCell c{};
+            if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
+            const unsigned long long w = __ballot(c.nhits >= 2);
+            // the last two-hit lane before this one, else the carry
+            const unsigned long long before = w & ((1ull << lane) - 1ull);
+            const int src = before ? ll_last_lane(before) : lane;
+            const double sx = __shfl(c.h2[0], src, 64), sy = __shfl(c.h2[1], src, 64), sz = __shfl(c.h2[2], src, 64);
+            if (c.ended == 1) {
+                acc += 0.5;
+            } else if (c.ended == 2) {
+                acc += 0.25;
+            } else if (c.nhits >= 2) {
+                acc += ll_pair(c.h1, c.h2);
+            } else if (c.nhits == 1) {
+                const double h2[3] = {before ? sx : cx, before ? sy : cy, before ? sz : cz};
+                acc += ll_pair(c.h1, h2);
+            }
+            if (w) {
+                const int s = ll_last_lane(w);
+                cx = __shfl(c.h2[0], s, 64);
+                cy = __shfl(c.h2[1], s, 64);
+                cz = __shfl(c.h2[2], s, 64);
+            }
+        }
+    }
+    const double s = block_reduce<kLlThreads, RED_SUM>(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kLlThreads) void k_ll_final(const double* __restrict__ partial, long long nb,
+                                                         double* __restrict__ out) {
+    __shared__ double red[kLlThreads / 64];
+    double s = 0.0;
+    for (long long b = threadIdx.x; b < nb; b += kLlThreads) s += partial[b];
+    s = block_reduce<kLlThreads, RED_SUM>(s, red);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+long long linelength_lines(int ny, int nz) { return (long long)(ny - 2) * (nz - 2); }
+long long linelength_blocks(int ny, int nz) {
+    return (linelength_lines(ny, nz) + kLlThreads / 64 - 1) / (kLlThreads / 64);
+}
+
+hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double* last, double* carry, double* partial,
+                             double* out, hipStream_t s) {
+    const LlGeom g{nx, ny, nz, linelength_lines(ny, nz)};
+    const long long nb = linelength_blocks(ny, nz);
+    hipLaunchKernelGGL(k_ll_last, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, last);
+    hipLaunchKernelGGL(k_ll_carry, dim3(1), dim3(kLlScan), 0, s, last, g.nlines, carry);
+    hipLaunchKernelGGL(k_ll_sum, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, carry, partial);
+    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(kLlThreads), 0, s, partial, nb, out);
+    return hipGetLastError();
+}
+
+}  // namespace gpe

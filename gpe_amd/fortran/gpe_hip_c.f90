@@ -126,6 +126,12 @@ module gpe_hip_c
       real(c_double), value :: scal
     end function gpe_imprint_line
 
+    integer(c_int) function gpe_linelength(ctx, length) bind(C, name='gpe_linelength')
+      import
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: length
+    end function gpe_linelength
+
     integer(c_int) function gpe_condensed_particles(ctx, row, eta, nharm, f) &
         bind(C, name='gpe_condensed_particles')
       import

@@ -6,7 +6,8 @@
 !   * the RUNNING sentinel (136-159);
 !   * every save_rate steps (modulo(t+im_t, |dt| save_rate) < |dt|) the
 !     time series energy.dat and psi_time.dat (save_energy, save_time,
-!     src/io.f90:147-218), and momentum.dat with save_mom;
+!     src/io.f90:147-218), linelength.dat with save_ll (save_linelength,
+!     src/io.f90:1406-1438) and momentum.dat with save_mom;
 !   * at save_rate3 intervals condensed_particles and the velocity PDF and
 !     correlation function (save_pdf, save_vcf: pdf/pdf_vel<p>.dat,
 !     vcf/vcf<p>.dat, src/io.f90:222-311);
@@ -21,8 +22,8 @@
 ! eta_time.dat, src/io.f90:315-404, 632-725) runs before the loop and at
 ! save_rate3 intervals, as in the reference.  After the loop: end_state.dat
 ! (RUNNING removed) and, with save_3d, the final dens file.  p is written as
-! seven digits, as itos forms it.  Not built: line length (save_ll), zeros,
-! contours, averages and filtered surfaces.
+! seven digits, as itos forms it.  Not built: zeros, contours, averages and
+! filtered surfaces.
 !
 ! host_mode = 'parity' copies psi%new -> psi%old and calls
 ! solver(psi%old, psi%new) every step exactly as the reference does;
@@ -32,7 +33,7 @@ program gpe_hip_main
   use parameters
   use error
   use gpe_context, only : close_ctx, gpe_storage, gpe_vscale, the_ctx, check
-  use gpe_hip_c, only : gpe_momentum, gpe_velocity_pdf, gpe_vcf, gpe_condensed_particles
+  use gpe_hip_c, only : gpe_momentum, gpe_velocity_pdf, gpe_vcf, gpe_condensed_particles, gpe_linelength
   use solve
   use variables
   implicit none
@@ -257,7 +258,7 @@ contains
   ! save_energy, save_time (src/io.f90:147-218) and momentum.dat
   subroutine time_series()
     real (pr) :: E, mom(3)
-    real(c_double) :: md(3)
+    real(c_double) :: md(3), ll
     complex (pr) :: c
     call on_device()
     call state_energy(E)
@@ -268,6 +269,13 @@ contains
     open (newunit=u, file='psi_time.dat', status='unknown', position='append')
     write (u, '(6e17.9)') t, im_t, real(c, pr), aimag(c), abs(c)**2, atan2(aimag(c)+1.0e-6_pr, real(c, pr))
     close (u)
+    if (save_ll) then
+      ! save_linelength (src/io.f90:1406-1438)
+      call check(gpe_linelength(the_ctx(), ll), 'linelength')
+      open (newunit=u, file='linelength.dat', status='unknown', position='append')
+      write (u, '(3e17.9)') t, im_t, ll
+      close (u)
+    end if
     if (save_mom) then
       call check(gpe_momentum(the_ctx(), md), 'momentum')
       mom = md

@@ -1,8 +1,8 @@
 ! Drop-in for the hot-path routines of the reference's module variables:
 ! laplacian (src/variables.f90:55-71), get_norm (537-559), energy (563-587),
 ! mass (591-611), renormalise (698-709), with the reference's interfaces,
-! for its diagnostics momentum (615-694) and get_vcf (467-507), and for
-! imprint_vortex_line (1148-1162).
+! for its diagnostics momentum (615-694), get_vcf (467-507) and
+! linelength (879-1145), and for imprint_vortex_line (1148-1162).
 ! velocity_pdf replaces the sequence get_pdf_velocity + get_pdf x 3 of
 ! save_velocity_pdf (src/io.f90:236-239) with one call that returns the
 ! same PDFs, maxima, means and deviations.
@@ -17,7 +17,7 @@ module variables
 
   private
   public :: laplacian, get_norm, energy, mass, renormalise, state_norm, state_renormalise, momentum, get_vcf, &
-    velocity_pdf, imprint_vortex_line, state_imprint_vortex_line, state_energy, state_peek
+    velocity_pdf, imprint_vortex_line, state_imprint_vortex_line, state_energy, state_peek, linelength
 
   type, public :: var
     complex (pr), allocatable, dimension(:,:,:) :: new
@@ -130,6 +130,16 @@ contains
     real (pr), intent(in) :: norm
     call check(gpe_renormalise(the_ctx(), real(norm, c_double)), 'renormalise')
   end subroutine state_renormalise
+
+  ! linelength (src/variables.f90:879-1145): total length of the zero lines
+  real (pr) function linelength(time, in_var)
+    real (pr), intent(in) :: time
+    complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
+    real(c_double) :: v
+    call put_halo(in_var)
+    call check(gpe_linelength(the_ctx(), v), 'linelength')
+    linelength = v
+  end function linelength
 
   ! imprint_vortex_line (src/variables.f90:1148-1162): |in_var| vortex_line(vl1)
   function imprint_vortex_line(in_var)

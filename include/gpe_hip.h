@@ -187,6 +187,13 @@ int gpe_imprint_line(gpe_ctx* ctx, const gpe_line* vl, double scal);
  * which a device-resident host fetches with this instead of the field. */
 int gpe_peek(gpe_ctx* ctx, int32_t i, int32_t j, int32_t k, double out[2]);
 
+/* Total length of the vortex lines (zero lines of psi) of the state:
+ * linelength (src/variables.f90:879-1145), including its face and edge
+ * rules and its reuse of the previous two-hit cell's second hit by a cell
+ * with one hit (zero before the first such cell, where the reference reads
+ * an undefined value).  Assumes dx = dy = dz, as the reference does. */
+int gpe_linelength(gpe_ctx* ctx, double* length);
+
 /* Condensed particles and spectra of the state: condensed_particles
  * (src/io.f90:315-404) and, when eta, nharm and F are given, spectrum
  * (src/io.f90:632-725), from the normalised backward FFT

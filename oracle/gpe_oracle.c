@@ -793,3 +793,143 @@ void orc_renormalise(const orc_params* p, double* var, double norm) {
     double snn = sqrt(p->nn), sn = sqrt(norm);
     for (size_t o = 0; o < n; o++) a[o] = cdivr(rmul(snn, a[o]), sn); /* sqrt(nn)*var/sqrt(norm) */
 }
+
+
+/* ------------------------------------------------------------------------
+ * Line length of the zero lines of psi (linelength, src/variables.f90:879-1145).
+ * Each interior cell (i, j, k) in 1..n-2 on every axis, visited in the
+ * reference's order (k, then j, then i fastest), looks for the point where
+ * the zero lines of Re psi and Im psi cross on each of its six faces: per
+ * face the first two sign-change crossings of u (and of v) along the edges
+ * 1-2, 2-3, 3-4, 4-1 define two straight lines whose intersection, if it lies
+ * in the unit face, is a hit.  A face where u (or v) vanishes at all four
+ * corners while the other changes sign adds 1/2 and ends the cell; a face
+ * with an edge where both vanish adds 1/4 and ends the cell.  Otherwise a
+ * cell with hits adds a sqrt(|h1 - h2|^2) with a = 1/2 or 1/4 for hits on
+ * shared faces (the reference's test compares y twice where z is meant).
+ * The hit slots persist from cell to cell, as the reference's local arrays
+ * do: a cell with a single hit pairs it with the second hit of the last
+ * cell that recorded one (0 before any, where the reference reads an
+ * undefined value).  The sum is multiplied by dx.
+ * counts: [0] cells with two or more hits, [1] cells with exactly one,
+ * [2] cells ended by a face, [3] cells ended by an edge.
+ * ---------------------------------------------------------------------- */
+
+/* corners of face n (u(1), u(2), u(3), u(4)) as (di, dj, dk) */
+static const int ll_face[6][4][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}},  /* 1 front  (k)   */
+    {{0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}},  /* 2 back   (k+1) */
+    {{1, 0, 0}, {1, 0, 1}, {1, 1, 1}, {1, 1, 0}},  /* 3 right  (i+1) */
+    {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}},  /* 4 left   (i)   */
+    {{0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}},  /* 5 top    (j+1) */
+    {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}},  /* 6 bottom (j)   */
+};
+
+/* the crossings of the zero line of w (w[0..4], w[4] = w[0]) with the face
+ * edges: returns lu - 1, the number of distinct sign-change crossings kept,
+ * with their positions in (xs, ys) */
+static int ll_crossings(const double* w, double* xs, double* ys) {
+    int lu = 1;  /* 1-based slot, as the reference */
+    for (int q = 1; q <= 4; q++) {
+        const double a = w[q - 1], b = w[q];
+        if (a == 0.0) {
+            /* a corner on the line: stored in slot lu, not counted */
+            static const double cx_[4] = {0.0, 1.0, 1.0, 0.0}, cy_[4] = {0.0, 0.0, 1.0, 1.0};
+            xs[lu] = cx_[q - 1];
+            ys[lu] = cy_[q - 1];
+        } else if (a * b < 0.0) {
+            switch (q) {
+                case 1: xs[lu] = fabs(a / (a - b)); ys[lu] = 0.0; break;
+                case 2: xs[lu] = 1.0; ys[lu] = fabs(a / (a - b)); break;
+                case 3: xs[lu] = fabs(b / (a - b)); ys[lu] = 1.0; break;
+                default: xs[lu] = 0.0; ys[lu] = fabs(b / (a - b)); break;
+            }
+            if (lu == 1 || xs[lu] != xs[lu - 1] || ys[lu] != ys[lu - 1]) lu++;
+        }
+    }
+    return lu;
+}
+
+double orc_linelength(const orc_params* p, const double* in, long long counts[4]) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const cx* a = (const cx*)in;
+    const double dx = (p->xr + p->xr) / (nx - 1);
+    double hx[7] = {0}, hy[7] = {0}, hz[7] = {0};  /* hit slots 1..6, kept across cells */
+    double total = 0.0;
+    for (int q = 0; q < 4; q++) counts[q] = 0;
+    for (int k = 1; k <= nz - 2; k++)
+        for (int j = 1; j <= ny - 2; j++)
+            for (int i = 1; i <= nx - 2; i++) {
+                int m = 1, ended = 0;
+                for (int n = 0; n < 6 && !ended; n++) {
+                    double u[5], v[5];
+                    for (int c = 0; c < 4; c++) {
+                        const int *o = ll_face[n][c];
+                        const cx z = a[((size_t)(k + o[2]) * ny + (j + o[1])) * nx + (i + o[0])];
+                        u[c] = z.re;
+                        v[c] = z.im;
+                    }
+                    u[4] = u[0];
+                    v[4] = v[0];
+                    const int uzero = u[0] == 0.0 && u[1] == 0.0 && u[2] == 0.0 && u[3] == 0.0;
+                    const int vzero = v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0 && v[3] == 0.0;
+                    const int vchange = v[0] * v[1] < 0.0 || v[1] * v[2] < 0.0 || v[2] * v[3] < 0.0 || v[3] * v[0] < 0.0;
+                    const int uchange = u[0] * u[1] < 0.0 || u[1] * u[2] < 0.0 || u[2] * u[3] < 0.0 || u[3] * u[0] < 0.0;
+                    if ((uzero && vchange) || (vzero && uchange)) {
+                        total += 0.5;
+                        counts[2]++;
+                        ended = 1;
+                        break;
+                    }
+                    for (int q = 0; q < 4; q++)
+                        if (u[q] == 0.0 && u[q + 1] == 0.0 && v[q] == 0.0 && v[q + 1] == 0.0) {
+                            ended = 1;
+                            break;
+                        }
+                    if (ended) {
+                        total += 0.25;
+                        counts[3]++;
+                        break;
+                    }
+                    double xu[7], yu[7], xv[7], yv[7];
+                    const int lu = ll_crossings(u, xu, yu);
+                    const int lv = ll_crossings(v, xv, yv);
+                    if (lu > 2 && lv > 2) {
+                        double den = xv[2] * (yu[1] - yu[2]) + xv[1] * (yu[2] - yu[1]) + (xu[1] - xu[2]) * (yv[1] - yv[2]);
+                        if (den == 0.0) den = den + 0.0000001;
+                        const double xx = (xu[1] * (xv[2] * (yv[1] - yu[2]) + xv[1] * (yu[2] - yv[2])) +
+                                           xu[2] * (xv[2] * (yu[1] - yv[1]) + xv[1] * (yv[2] - yu[1]))) / den;
+                        const double yy = (xv[2] * (yu[1] - yu[2]) * yv[1] + xu[1] * yu[2] * yv[1] - xv[1] * yu[1] * yv[2] -
+                                           xu[1] * yu[2] * yv[2] + xv[1] * yu[2] * yv[2] + xu[2] * yu[1] * (yv[2] - yv[1])) / den;
+                        if (xx >= 0.0 && xx <= 1.0 && yy >= 0.0 && yy <= 1.0) {
+                            double h[3];
+                            switch (n + 1) {
+                                case 1: h[0] = xx; h[1] = yy; h[2] = 0.0; break;
+                                case 2: h[0] = xx; h[1] = yy; h[2] = 1.0; break;
+                                case 3: h[0] = 1.0; h[1] = yy; h[2] = xx; break;
+                                case 4: h[0] = 0.0; h[1] = yy; h[2] = xx; break;
+                                case 5: h[0] = xx; h[1] = 1.0; h[2] = yy; break;
+                                default: h[0] = xx; h[1] = 0.0; h[2] = yy; break;
+                            }
+                            hx[m] = h[0];
+                            hy[m] = h[1];
+                            hz[m] = h[2];
+                            m++;
+                        }
+                    }
+                }
+                if (ended || m == 1) continue;
+                counts[m == 2 ? 1 : 0]++;
+                const int xs = hx[1] == hx[2] && (hx[1] == 0.0 || hx[1] == 1.0);
+                const int ys = hy[1] == hy[2] && (hy[1] == 0.0 || hy[1] == 1.0);
+                const int zs = hz[1] == hz[2] && (hz[1] == 0.0 || hz[1] == 1.0);
+                double w = 1.0;
+                if (xs || ys || ys) {
+                    w = w * 0.5;
+                    if ((xs && (ys || ys)) || (ys && zs)) w = w * 0.5;
+                }
+                const double ddx = hx[1] - hx[2], ddy = hy[1] - hy[2], ddz = hz[1] - hz[2];
+                total = total + w * sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            }
+    return total * dx;
+}

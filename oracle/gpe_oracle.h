@@ -70,6 +70,10 @@ double orc_energy(const orc_params* p, const double* in);
 double orc_mass(const orc_params* p, const double* in);
 void   orc_renormalise(const orc_params* p, double* var, double norm);
 
+/* Line length of the zero lines of psi (linelength, src/variables.f90:879-1145),
+ * times dx; counts: cells with >= 2 hits, with 1 hit, ended by a face, by an edge. */
+double orc_linelength(const orc_params* p, const double* in, long long counts[4]);
+
 /* Number of OpenMP threads the stencil loops use (1 without OpenMP). */
 int orc_threads(void);
 

@@ -61,6 +61,7 @@ def lib() -> ct.CDLL:
             "orc_energy": (ct.c_double, [P, dp]),
             "orc_mass": (ct.c_double, [P, dp]),
             "orc_renormalise": (None, [P, dp, ct.c_double]),
+            "orc_linelength": (ct.c_double, [P, dp, dp]),
             "orc_threads": (ct.c_int, []),
         }
         for k, (r, a) in sig.items():
@@ -152,6 +153,15 @@ def energy(p, psi):
 def mass(p, psi):
     psi = _c(psi)
     return lib().orc_mass(ct.byref(params(p)), psi.ctypes.data)
+
+
+def linelength(p, psi):
+    """linelength (src/variables.f90:879-1145) times dx, and the counts of
+    cells with >= 2 hits, 1 hit, ended by a face, ended by an edge."""
+    psi = _c(psi)
+    counts = np.zeros(4, dtype=np.int64)
+    v = lib().orc_linelength(ct.byref(params(p)), psi.ctypes.data, counts.ctypes.data)
+    return v, counts
 
 
 def renormalise(p, psi, nrm):

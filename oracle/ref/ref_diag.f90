@@ -5,9 +5,11 @@
 !   momentum (src/variables.f90:615-694),
 !   get_pdf_velocity and get_pdf per component (src/variables.f90:377-463,
 !     as save_velocity_pdf does, src/io.f90:222-285),
-!   get_vcf (src/variables.f90:467-507)
+!   get_vcf (src/variables.f90:467-507),
+!   linelength (src/variables.f90:879-1145)
 ! and writes the results (stream): mom(3), vmean(3), vstdev(3), vmax(3),
-! the valid counts (3 int32), pdf_vx, pdf_vy, pdf_vz (nbins each), f(0:nx1).
+! the valid counts (3 int32), pdf_vx, pdf_vy, pdf_vz (nbins each), f(0:nx1),
+! the line length.
 program ref_diag
   use parameters
   use variables
@@ -18,6 +20,7 @@ program ref_diag
   integer :: u, nxp, nyp, nzp
   complex (pr) :: dt_file
   real (pr), dimension(3) :: mom, vmean, vstdev, vmax
+  real (pr) :: ll
   real (pr), allocatable, dimension(:) :: vx, vy, vz, pdf_vx, pdf_vy, pdf_vz, f
   character(len=512) :: fin, fout
 
@@ -81,12 +84,14 @@ program ref_diag
   call get_pdf(vy, pdf_vy, vmax(2))
   call get_pdf(vz, pdf_vz, vmax(3))
   call get_vcf(psi%old, f)
+  ll = linelength(0.0_pr, psi%old)
 
   open (newunit=u, file=trim(fout), access='stream', status='replace')
   write (u) mom, vmean, vstdev, vmax
   write (u) size(vx), size(vy), size(vz)
   write (u) pdf_vx, pdf_vy, pdf_vz
   write (u) f
+  write (u) ll
   close (u)
 
   call MPI_FINALIZE(ierr)

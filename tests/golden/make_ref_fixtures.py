@@ -4,8 +4,10 @@ oracle/_ref) on seeded inputs.  Each stepping fixture holds the input field,
 the run parameters, the reference's final field, t, im_t, dt, p, its norm
 and energy (get_norm, energy) and the per-step (t, im_t, dt) trace.  The
 ref_diag_* fixtures hold an input field and the reference's momentum,
-velocity statistics and PDF (nbins = 64) and velocity correlation function.
-The ref_ic_* fixtures hold the reference's initial conditions (its ics with
+velocity statistics and PDF (nbins = 64), velocity correlation function and
+line length.
+The ref_ll_* fixtures hold inputs and the reference's line length for the
+branches smooth fields do not reach.  The ref_ic_* fixtures hold the reference's initial conditions (its ics with
 the expressions of oracle/ref/ic_cases.txt as ic.in) on a 16 x 12 x 10 grid.
 
 Run in the build container (the reference is not present on the GPU box):
@@ -127,10 +129,11 @@ def run_reference_diag(name, p, psi, nbins):
     counts = np.frombuffer(raw, "<i4", 3, o); o += 12
     pdf = np.frombuffer(raw, "<f8", 3 * nbins, o).reshape(3, nbins); o += 24 * nbins
     vcf = np.frombuffer(raw, "<f8", p.nx, o); o += 8 * p.nx
+    linelength = float(np.frombuffer(raw, "<f8", 1, o)[0]); o += 8
     assert o == len(raw), (o, len(raw))
     print(f"{name}: momentum {head[:3]}, valid {counts}", flush=True)
     return dict(mom=head[0:3].copy(), vmean=head[3:6].copy(), vstdev=head[6:9].copy(), vmax=head[9:12].copy(),
-                counts=counts.copy(), pdf=pdf.copy(), vcf=vcf.copy())
+                counts=counts.copy(), pdf=pdf.copy(), vcf=vcf.copy(), linelength=linelength)
 
 
 def diag_cases():
@@ -193,6 +196,29 @@ def ic_cases():
                             scal=scal, out=out)
 
 
+def ll_cases():
+    """Line-length fixtures for the branches smooth fields never reach: white
+    noise (cells with a single hit, which reuse the previous cell's second
+    hit), a vortex line along a grid edge (the edge branch) and a zero plane
+    of Re psi (the face branch).  The grid spacing is 1, so the zeros fall
+    exactly on grid points."""
+    p = RunParams(nx=24, ny=20, nz=18, xr=11.5, yr=9.5, zr=8.5)
+    x, y, z = p.axes()
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    # seed 8: the first cell with a hit has two (the reference's hit slots
+    # are undefined before the first such cell)
+    rng = np.random.default_rng(8)
+    cases = [("noise", rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape)),
+             ("edge_line", (X - 0.5) + 1j * (Y - 0.5)),
+             ("zero_plane", (X - 0.5) + 1j * Y + 0.3j * np.sin(0.7 * Z))]
+    for name, psi in cases:
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        res = run_reference_diag("ll_" + name, p, psi, 64)
+        np.savez_compressed(os.path.join(OUT, f"ref_ll_{name}.npz"), params=json.dumps(dataclasses.asdict(p)),
+                            inp=psi, linelength=res["linelength"])
+        print(f"ll {name}: {res['linelength']!r}", flush=True)
+
+
 def ring64():
     p, psi = ics.config_ring64(seed=0)
     res = run_reference("ring64_rk4_200", p, psi, nsteps=200)
@@ -200,7 +226,7 @@ def ring64():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "ring64", "diag", "ic"]
+    which = sys.argv[1:] or ["small", "ring64", "diag", "ic", "ll"]
     if "small" in which:
         small_cases()
     if "ring64" in which:
@@ -209,3 +235,5 @@ if __name__ == "__main__":
         diag_cases()
     if "ic" in which:
         ic_cases()
+    if "ll" in which:
+        ll_cases()

@@ -148,3 +148,68 @@ def test_gpu_diagnostics_c64_and_larger_grid(orc):
         m64, st64 = ctx.momentum(), ctx.velocity_pdf(nbins)
     assert np.all(np.abs(m64 - exp_m) <= 1e-5 * momentum_scale(p, psi))
     assert np.allclose(st64["vstdev"], exp_st["vstdev"], rtol=1e-4)
+
+
+LL_FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_ll_*.npz")))
+
+
+def ll_cases():
+    """(name, params, psi, reference line length) of every fixture that holds one."""
+    out = []
+    for path in FIXTURES + LL_FIXTURES:
+        p, psi, z = load(path)
+        out.append((os.path.basename(path)[4:-4], p, psi, float(z["linelength"])))
+    return out
+
+
+def test_oracle_linelength_matches_reference_bitwise(orc):
+    """The C restatement visits cells in the reference's order with the same
+    operation order, so it reproduces the reference's sum exactly: smooth
+    fields (vortex ring, trapped line, turbulence), white noise (682 cells
+    with one hit that reuse the previous cell's second hit), a line along a
+    grid edge (the 1/4 rule) and a zero plane (the 1/2 rule)."""
+    from oracle import oracle as orc_
+
+    seen = set()
+    for name, p, psi, ref in ll_cases():
+        v, counts = orc_.linelength(p, psi)
+        print(f"{name}: {v!r} vs {ref!r}, counts {counts}")
+        assert v == ref
+        seen.update(k for k in range(4) if counts[k])
+    assert seen == {0, 1, 2, 3}  # every branch is exercised
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["c128", "mixed"])
+def test_gpu_linelength_matches_reference(storage):
+    """Per cell the device reproduces the reference's arithmetic (contraction
+    off); only the order of the final sum differs: 1e-13 relative."""
+    for name, p, psi, ref in ll_cases():
+        with Context(p.replace(storage=storage)) as ctx:
+            ctx.upload(psi)
+            got = ctx.linelength()
+        print(f"{name} {storage}: {got!r} vs {ref!r}")
+        assert abs(got - ref) <= 1e-13 * max(ref, 1.0)
+
+
+@pytest.mark.gpu
+def test_gpu_linelength_of_a_ring_and_long_lines(orc):
+    """A vortex ring of radius 20 on a 128^3 grid (length close to 2 pi 20),
+    a white-noise field on a ragged grid with lines longer than one chunk of
+    64 cells, both against the restatement."""
+    from oracle import oracle as orc_
+
+    p = RunParams(nx=128, ny=128, nz=128, xr=31.75, yr=31.75, zr=31.75)
+    psi = ics.vortex_ring(p, ics.RingParam("xz", 0.3, 0.2, -0.1, 20.0))
+    ref, _ = orc_.linelength(p, psi)
+    assert abs(ref - 2 * np.pi * 20) < 0.5
+    rng = np.random.default_rng(4)
+    q = RunParams(nx=150, ny=37, nz=21, xr=7.45, yr=1.8, zr=1.0)
+    noise = rng.standard_normal(q.shape) + 1j * rng.standard_normal(q.shape)
+    ref2, counts = orc_.linelength(q, noise)
+    assert counts[1] > 0
+    for pp, f, r in ((p, psi, ref), (q, noise, ref2)):
+        with Context(pp) as ctx:
+            ctx.upload(f)
+            got = ctx.linelength()
+        assert abs(got - r) <= 1e-13 * r

@@ -210,7 +210,7 @@ def test_host_save_schedule_and_files(tmp_path, mode):
     cfg = dict(save_rate=4, save_rate2=0.07, save_rate3=0.05, p_save=0.12)
     text = run_in_text(p, (nsteps - 0.5) * p.tau, nbins=nbins)
     text = set_namelist(text, save_3d=True, save_pdf=True, save_vcf=True, save_mom=True, save_spectrum=True,
-                        host_mode=mode, **cfg)
+                        save_ll=True, host_mode=mode, **cfg)
     (tmp_path / "run.in").write_text(text)
     write_end_state(str(tmp_path / "end_state00.dat"), psi, 0, 0.0, 0.0, p.dt0())
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
@@ -224,7 +224,7 @@ def test_host_save_schedule_and_files(tmp_path, mode):
         for ev in events:
             if ev["ts"]:
                 c = ctx.peek(p.nx // 2, p.ny // 2, p.nz // 2)
-                ts_rows.append((ev["t"], ctx.energy() / (p.nx * p.ny * p.nz), c, ctx.momentum()))
+                ts_rows.append((ev["t"], ctx.energy() / (p.nx * p.ny * p.nz), c, ctx.momentum(), ctx.linelength()))
             if ev["diag"]:
                 cond.append((ev["p"], ev["t"], ctx.condensed_particles()))
                 diag_ps.append((ev["p"], ctx.velocity_pdf(nbins), ctx.vcf()))
@@ -236,8 +236,10 @@ def test_host_save_schedule_and_files(tmp_path, mode):
     en = read_rows(tmp_path / "energy.dat")
     pt = read_rows(tmp_path / "psi_time.dat")
     mo = read_rows(tmp_path / "momentum.dat")
-    assert len(en) == len(ts_rows) == len(pt) == len(mo) and len(ts_rows) >= 5
-    for k, (t, e, c, m) in enumerate(ts_rows):
+    ll = read_rows(tmp_path / "linelength.dat")
+    assert len(en) == len(ts_rows) == len(pt) == len(mo) == len(ll) and len(ts_rows) >= 5
+    for k, (t, e, c, m, length) in enumerate(ts_rows):
+        assert np.isclose(ll[k, 2], length, rtol=1e-8) and length > 0
         assert abs(en[k, 0] - t) <= 1e-8 * max(t, 1e-3) and en[k, 1] == 0.0
         assert np.isclose(en[k, 2], e, rtol=1e-8)
         exp = [c.real, c.imag, abs(c) ** 2, np.arctan2(c.imag + 1e-6, c.real)]

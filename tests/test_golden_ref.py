@@ -18,7 +18,7 @@ from gpe_amd import RunParams, ics
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(f for f in glob.glob(os.path.join(HERE, "golden", "ref_*.npz"))
-                  if not os.path.basename(f).startswith(("ref_diag_", "ref_ic_")))
+                  if not os.path.basename(f).startswith(("ref_diag_", "ref_ic_", "ref_ll_")))
 
 
 def load(path):
