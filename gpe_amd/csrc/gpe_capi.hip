@@ -1158,7 +1158,7 @@ int gpe_condensed_particles(gpe_ctx* c, double row[7], double eta[7], int64_t nh
     if ((rc = norm_mass(c, &M, 0))) return rc;
     const int nb = 512;
     const int nhalf = nx / 2;
-    if ((rc = ensure_partial(c, (size_t)nb * (2 * kSpectrumShells + nhalf + 1)))) return rc;
+    if ((rc = ensure_partial(c, (size_t)nb * (2 * kSpectrumShells + nhalf + 1) + nx + ny + nz))) return rc;
     double d[3];
     spacing(c->p, d);
     const double rho0 = n0 / V;

@@ -99,6 +99,19 @@ __device__ LlCell ll_cell(const double2* __restrict__ psi, int nx, int ny, int i
 #pragma unroll
             for (int di = 0; di < 2; ++di)
                 cr[dk][dj][di] = psi[((long long)(k + dk) * ny + (j + dj)) * nx + (i + di)];
+    // Re psi (or Im psi) of one strict sign at all eight corners: no face
+    // has a zero, a crossing or a sign change of it, so no rule applies
+    // (most cells of a field with sparse vortex lines end here)
+    bool upos = true, uneg = true, vpos = true, vneg = true;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double2 z = cr[q >> 2][(q >> 1) & 1][q & 1];
+        upos = upos && z.x > 0.0;
+        uneg = uneg && z.x < 0.0;
+        vpos = vpos && z.y > 0.0;
+        vneg = vneg && z.y < 0.0;
+    }
+    if (upos || uneg || vpos || vneg) return r;
 #pragma unroll
     for (int n = 0; n < 6; ++n) {
         double u[5], v[5];

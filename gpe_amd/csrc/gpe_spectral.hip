@@ -85,18 +85,29 @@ __device__ __forceinline__ double sym4(double kk, double d, double d2) {
     return (-2.0 * cos(2.0 * kk * d) + 32.0 * cos(kk * d) - 30.0) / d2;
 }
 
-__global__ __launch_bounds__(kSpThreads) void k_tot_partial(int nx, int ny, int nz, double dk, double dx, double dy,
-                                                            double dz, double rho0, double* __restrict__ partial) {
+// sym[0..nx) = the x terms, then ny y terms, then nz z terms
+__global__ __launch_bounds__(kSpThreads) void k_tot_axes(int nx, int ny, int nz, double dk, double dx, double dy,
+                                                         double dz, double* __restrict__ sym) {
+    const int q = blockIdx.x * kSpThreads + threadIdx.x;
+    if (q >= nx + ny + nz) return;
+    const int i = q < nx ? q : (q < nx + ny ? q - nx : q - nx - ny);
+    const double d = q < nx ? dx : (q < nx + ny ? dy : dz);
+    sym[q] = sym4(-kSpPi + (double)i * dk, d, d * d);
+}
+
+__global__ __launch_bounds__(kSpThreads) void k_tot_partial(int nx, int ny, int nz, const double* __restrict__ sym,
+                                                            double rho0, double* __restrict__ partial) {
     __shared__ double red[kSpThreads / 64];
     const long long n = (long long)nx * ny * nz;
-    const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dz * dz;
+    const double* sx = sym;
+    const double* sy = sym + nx;
+    const double* sz = sym + nx + ny;
     double acc = 0.0;
     for (long long q = blockIdx.x * (long long)kSpThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kSpThreads) {
         const int i = (int)(q % nx);
         const long long r = q / nx;
         const int j = (int)(r % ny), k = (int)(r / ny);
-        const double kx = -kSpPi + (double)i * dk, ky = -kSpPi + (double)j * dk, kz = -kSpPi + (double)k * dk;
-        const double k2 = -(1.0 / 12.0) * (sym4(kx, dx, dx2) + sym4(ky, dy, dy2) + sym4(kz, dz, dz2));
+        const double k2 = -(1.0 / 12.0) * (sx[i] + sy[j] + sz[k]);
         if (k2 == 0.0) continue;
         const double k4 = k2 * k2;
         acc += (k2 + rho0) / (k4 + 2.0 * rho0 * k2);
@@ -131,7 +142,10 @@ hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, dou
 hipError_t launch_condensed_tot(int nx, int ny, int nz, const double d[3], double rho0, double* partial, int nb,
                                 double* out, hipStream_t s) {
     const double dk = 2.0 * kSpPi / (double)(nx - 1);
-    hipLaunchKernelGGL(k_tot_partial, dim3(nb), dim3(kSpThreads), 0, s, nx, ny, nz, dk, d[0], d[1], d[2], rho0,
+    double* sym = partial + nb;  // nx + ny + nz values after the block partials
+    hipLaunchKernelGGL(k_tot_axes, dim3((nx + ny + nz + kSpThreads - 1) / kSpThreads), dim3(kSpThreads), 0, s, nx, ny,
+                       nz, dk, d[0], d[1], d[2], sym);
+    hipLaunchKernelGGL(k_tot_partial, dim3(nb), dim3(kSpThreads), 0, s, nx, ny, nz, (const double*)sym, rho0,
                        partial);
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(kSpThreads), 0, s, partial, 1, nb, out);
     return hipGetLastError();

@@ -14,11 +14,16 @@
 
   attempts   RKF45 attempts per storage with per-kernel times (the RKCK
              profiling target).
+  diag       wall time per call of the state diagnostics (momentum, velocity
+             PDF, VCF, condensed particles + spectra, line length) on the
+             random-phase workload, with the algorithmic bytes each moves
+             (the diagnostics profiling target).
 
 Each prints one JSON object per line.  Usage:
     python -m gpe_amd.studies drift [--n 256] [--steps 2000]
     python -m gpe_amd.studies time_to_t [--n 512] [--T 20]
     python -m gpe_amd.studies attempts [--n 512] [--attempts 6] [--storages c128,mixed]
+    python -m gpe_amd.studies diag [--n 512] [--reps 5]
 """
 from __future__ import annotations
 
@@ -149,9 +154,41 @@ def attempts(n: int, count: int, storages, copy_mib: int = 0):
                       wall_ms_per_attempt=1e3 * wall / max(na, 1), kernels=ker))
 
 
+# Algorithmic HBM bytes per cell of each diagnostic (complex128 state):
+#   momentum   one stencil pass over psi                                    16
+#   vel_pdf    velocity pass (psi 16 + v 24), statistics (24), histogram (24) 88
+#   vcf        velocity pass (16 + 24), one read of v_x (8)                  48
+#   condensed  copy for the FFT (16 + 16), spectrum read (16); the FFT's
+#              own traffic (library) is not counted                         48
+#   linelength two passes over psi (the corners of a cell are shared)       32
+DIAG_BYTES = dict(momentum=16, vel_pdf=88, vcf=48, condensed=48, linelength=32)
+
+
+def diag(n: int, reps: int):
+    """Wall time per call of each state diagnostic on the random-phase
+    workload (the state has vortex lines everywhere)."""
+    p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk4", tau=0.005)
+    calls = dict(momentum=lambda c: c.momentum(), vel_pdf=lambda c: c.velocity_pdf(256), vcf=lambda c: c.vcf(),
+                 condensed=lambda c: c.condensed_particles(), linelength=lambda c: c.linelength())
+    cells = p.nx * p.ny * p.nz
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        for name, fn in calls.items():
+            fn(ctx)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                out = fn(ctx)
+            ms = 1e3 * (time.perf_counter() - t0) / reps
+            gbs = DIAG_BYTES[name] * cells / (ms * 1e-3) / 1e9
+            extra = dict(value=out) if name == "linelength" else {}
+            emit(dict(study="diag", n=n, diagnostic=name, ms=round(ms, 4), bytes_per_cell=DIAG_BYTES[name],
+                      GB_s=round(gbs, 1), hbm_frac=round(gbs / 8000.0, 4), **extra))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["drift", "time_to_t", "attempts"])
+    ap.add_argument("study", choices=["drift", "time_to_t", "attempts", "diag"])
+    ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--every", type=int, default=100)
@@ -166,6 +203,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.study == "drift":
         drift(a.n or 256, a.steps, a.every)
+    elif a.study == "diag":
+        diag(a.n or 512, a.reps)
     elif a.study == "attempts":
         attempts(a.n or 512, a.attempts, tuple(a.storages.split(",")), a.copy_probe_mib)
     else:
