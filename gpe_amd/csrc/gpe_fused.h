@@ -29,7 +29,7 @@
 // buffered), F34's psi0 plane p (double buffered), and the first stage's
 // results in a ring of planes z..z+H, written when computed and read H
 // iterations later, so one barrier per iteration orders every LDS reuse.
-// F12 keeps the loads of the next GPE_FUSED_DEPTH planes in flight while a
+// F12 keeps the loads of the next GPE_FUSED_DEPTH (3) planes in flight while a
 // plane is evaluated; F34 one plane.
 //
 // Cells outside the domain are evaluated at their unfolded positions with
@@ -112,9 +112,9 @@ template <int ORD> struct FusedCtx {
     }
 };
 
-// Planes of loads in flight (1 or 2) in the fused kernels.
+// Planes of loads in flight (1, 2 or 3) in the fused kernels.
 #ifndef GPE_FUSED_DEPTH
-#define GPE_FUSED_DEPTH 2
+#define GPE_FUSED_DEPTH 3
 #endif
 
 // Runs step(z, F) for z = start .. z1-1 with DEPTH sets of in-flight loads:
@@ -126,6 +126,16 @@ __device__ __forceinline__ void fused_march(int start, int z1, Fetch fetch, Step
         Pf F0{};
         fetch(F0, start);
         for (int z = start; z < z1; ++z) step(z, F0);
+    } else if constexpr (DEPTH == 3) {
+        Pf F0{}, F1{}, F2{};
+        fetch(F0, start);
+        if (start + 1 < z1) fetch(F1, start + 1);
+        if (start + 2 < z1) fetch(F2, start + 2);
+        for (int z = start; z < z1; z += 3) {
+            step(z, F0);
+            if (z + 1 < z1) step(z + 1, F1);
+            if (z + 2 < z1) step(z + 2, F2);
+        }
     } else {
         Pf F0{}, F1{};
         fetch(F0, start);
