@@ -15,7 +15,9 @@ live with HIP events around each launch on the context's stream, the
 four-stage-kernel run (13 field accesses per cell, the bandwidth-bound
 formulation), the complex64- and mixed-storage runs, and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
-workload: --cpu-steps RK4 steps of the same grid and IC).
+workload: --cpu-steps RK4 steps of the same grid and IC), and the
+reference's own Fortran build timed on its 64^3 vortex-ring case
+(cpu_reference, --ref-steps steps, where oracle/_ref is present).
 """
 from __future__ import annotations
 
@@ -105,6 +107,39 @@ def cpu_baseline(p, psi, steps):
             "kind": "port", "sample": f"{steps} RK4 step(s) of the same {p.nx}^3 workload ({s:.1f} s)"}
 
 
+def cpu_reference(steps):
+    """The reference's own Fortran time-stepping path (oracle/_ref, compiled
+    from its unmodified sources; one MPI rank, one core) on `steps` RK4 steps
+    of the 64^3 vortex-ring workload, the only grid of the benchmark
+    configurations it is built for.  None where that build is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from gpe_amd import ics
+    from gpe_amd.io import run_in_text, write_end_state
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "64x64x64", "ref_driver")
+    if not os.access(exe, os.X_OK):
+        return None
+    p, psi = ics.config_ring64(seed=0)
+    run = tempfile.mkdtemp(prefix="gpe_ref_")
+    try:
+        with open(os.path.join(run, "run.in"), "w") as f:
+            f.write(run_in_text(p, (steps - 0.5) * p.tau))
+        write_end_state(os.path.join(run, "in.dat"), psi, 0, 0.0, 0.0, p.dt0())
+        t0 = time.perf_counter()
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {exe} in.dat out.dat trace.dat"], cwd=run,
+                           capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+        s = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(run, ignore_errors=True)
+    if r.returncode != 0:
+        return None
+    return {"value": round(p.cells * steps / s / 1e9, 6), "unit": "Gcell-steps/s", "cores": 1, "kind": "reference",
+            "sample": f"{steps} RK4 steps of the 64^3 vortex-ring workload, reference build, 1 MPI rank ({s:.1f} s)"}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +148,8 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=512, help="grid points per axis (512 = the benchmark workload)")
     ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU baseline sample (0: skip)")
+    ap.add_argument("--ref-steps", type=int, default=40,
+                    help="RK4 steps of the reference build's 64^3 sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
     ap.add_argument("--unfused", action="store_true", help="four stage kernels per RK4 step instead of two fused")
@@ -201,6 +238,7 @@ def main(argv=None) -> int:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:
         result["cpu_baseline"] = None
+    result["cpu_reference"] = cpu_reference(args.ref_steps) if args.ref_steps > 0 else None
     print(json.dumps(result), flush=True)
     return 0
 
