@@ -14,6 +14,16 @@
 //                       the order of their steps is free) while its fields
 //                       stay cache resident; the chunk is captured in a
 //                       hipGraph and replayed
+//   GPE_ENS_FUSED       the fused RK4 of a single context (two kernels of two
+//                       stages each, 5 field accesses per cell) for all
+//                       members: one persistent launch per stage pair over a
+//                       work list of (member, fused block) items, largest
+//                       members first; each member's state alternates between
+//                       two buffers from step to step
+//   GPE_ENS_FUSED_GRAPH GPE_ENS_FUSED captured in a hipGraph (an even number of
+//                       steps) and replayed
+// The fused modes need complex128 members whose right-hand side has no
+// optional terms, as the single-context fused path does.
 // Each member's arithmetic is the single-context kernel's (same operators,
 // same per-member tiling), so results do not depend on the mode.
 #include <math.h>
@@ -22,6 +32,7 @@
 #include <string>
 #include <vector>
 
+#include "gpe_fused.h"
 #include "gpe_host.h"
 
 using namespace gpe;
@@ -130,6 +141,16 @@ struct gpe_ens {
     int gsteps = 0;
     int gmode = -1;
     double gre = 0, gim = 0;
+    // fused modes: descriptors (largest member first), work items, grid, the
+    // buffer holding each member's state (0: psi, 1: ub) and their own graph
+    void* fdesc = nullptr;
+    int ftotal = 0, fgrid = 0;
+    bool fused_ok = false;
+    int fflip = 0;
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fgexec = nullptr;
+    int fgsteps = 0;
+    double fgre = 0, fgim = 0;
     std::string err;
 };
 
@@ -158,6 +179,58 @@ void drop_graph(gpe_ens* e) {
     e->gexec = nullptr;
     e->graph = nullptr;
     e->gsteps = 0;
+}
+
+void drop_fused_graph(gpe_ens* e) {
+    if (e->fgexec) (void)hipGraphExecDestroy(e->fgexec);
+    if (e->fgraph) (void)hipGraphDestroy(e->fgraph);
+    e->fgexec = nullptr;
+    e->fgraph = nullptr;
+    e->fgsteps = 0;
+}
+
+// Fused-mode descriptors, written in place (a captured graph keeps reading
+// the same buffer).  z-chunks of 32 planes keep items of similar size; the
+// members go largest first so the persistent blocks end together.
+int build_fused_desc(gpe_ens* e) {
+    e->fused_ok = e->mp[0].storage == GPE_C128 && !e->general;
+    if (!e->fused_ok) return 0;
+    int dev = 0, ncu = 0;
+    EHIP(e, hipGetDevice(&dev));
+    EHIP(e, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<int> idx(e->n);
+    for (int m = 0; m < e->n; ++m) idx[m] = m;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return e->cells[a] > e->cells[b]; });
+    std::vector<FusedMember> h(e->n);
+    int begin = 0;
+    for (int s = 0; s < e->n; ++s) {
+        const int m = idx[s];
+        const gpe_params& p = e->mp[m];
+        FusedMember& d = h[s];
+        d.g = Geom{};
+        d.g.nx = p.nx;
+        d.g.ny = p.ny;
+        d.g.nz = p.nz;
+        d.g.bcs = p.bcs;
+        d.g.plane = (long long)p.nx * p.ny;
+        if (p.order == 2) fused_tiles<2>(p.nx, p.ny, p.nz, ncu, d.g);
+        else fused_tiles<4>(p.nx, p.ny, p.nz, ncu, d.g);
+        d.g.tz = p.nz < 32 ? p.nz : 32;
+        d.g.nblocks = d.g.ntx * d.g.nty * ((p.nz + d.g.tz - 1) / d.g.tz);
+        d.w = make_w<double>(p);
+        d.k = make_rhs_p<double>(p, nullptr);
+        d.psi = (double2*)e->psi[m];
+        d.alt = (double2*)e->ub[m];
+        d.u2 = (double2*)e->ua[m];
+        d.begin = begin;
+        begin += d.g.nblocks;
+    }
+    e->ftotal = begin;
+    e->fgrid = ncu < begin ? ncu : begin;  // one 1024-thread block per CU (LDS-limited)
+    if (!e->fdesc) EHIP(e, hipMalloc(&e->fdesc, sizeof(FusedMember) * e->n));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    EHIP(e, hipMemcpy(e->fdesc, h.data(), sizeof(FusedMember) * e->n, hipMemcpyHostToDevice));
+    return 0;
 }
 
 int default_tz(const gpe_params& p, int tx, int ty) {
@@ -231,7 +304,72 @@ template <typename R> int build_desc(gpe_ens* e) {
     EHIP(e, hipMemcpy(e->gdesc, g.data(), sizeof(EnsMember<R>) * e->n, hipMemcpyHostToDevice));
     e->desc_dirty = false;
     drop_graph(e);
+    return build_fused_desc(e);
+}
+
+// One fused RK4 step of every member (two launches); toggles the state buffer.
+int step_fused(gpe_ens* e, double dre, double dim) {
+    const double2 dt = mkc<double>(dre, dim);
+    const FusedMember* mem = (const FusedMember*)e->fdesc;
+    const int ord = e->mp[0].order;
+    for (int pair : {12, 34}) {
+        hipError_t x = ord == 2 ? launch_rk4_fused_batched<2>(mem, e->n, e->ftotal, e->fgrid, dt, pair, e->fflip,
+                                                              e->stream)
+                                : launch_rk4_fused_batched<4>(mem, e->n, e->ftotal, e->fgrid, dt, pair, e->fflip,
+                                                              e->stream);
+        EHIP(e, x);
+    }
+    e->fflip ^= 1;
     return 0;
+}
+
+// After an odd number of fused steps the states sit in the alternate
+// buffers: exchange the roles (host pointers and every descriptor).
+template <typename R> int settle_fused(gpe_ens* e) {
+    if (!e->fflip) return 0;
+    for (int m = 0; m < e->n; ++m) std::swap(e->psi[m], e->ub[m]);
+    e->fflip = 0;
+    return build_desc<R>(e);
+}
+
+template <typename R> int ens_step_fused(gpe_ens* e, int mode, double dre, double dim, int nsteps) {
+    if (!e->fused_ok)
+        return efail(e, GPE_EINVAL, "gpe_ens_step: the fused modes need complex128 members whose right-hand side "
+                                    "has no optional terms (trap, Urhs, dissipation, eqn 3)");
+    int done = 0;
+    if (mode == GPE_ENS_FUSED_GRAPH) {
+        int chunk = nsteps < 50 ? nsteps : 50;
+        chunk -= chunk % 2;  // a replay starts and ends with the states in psi
+        if (chunk >= 2) {
+            if (!e->fgexec || e->fgsteps != chunk || e->fgre != dre || e->fgim != dim) {
+                drop_fused_graph(e);
+                EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                int rc = 0;
+                for (int s = 0; s < chunk && !rc; ++s) rc = step_fused(e, dre, dim);
+                hipGraph_t g = nullptr;
+                hipError_t x = hipStreamEndCapture(e->stream, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                EHIP(e, x);
+                e->fgraph = g;
+                EHIP(e, hipGraphInstantiate(&e->fgexec, e->fgraph, nullptr, nullptr, 0));
+                e->fgsteps = chunk;
+                e->fgre = dre;
+                e->fgim = dim;
+            }
+            while (done + e->fgsteps <= nsteps) {
+                EHIP(e, hipGraphLaunch(e->fgexec, e->stream));
+                done += e->fgsteps;
+            }
+        }
+    }
+    for (; done < nsteps; ++done) {
+        int rc = step_fused(e, dre, dim);
+        if (rc) return rc;
+    }
+    return settle_fused<R>(e);
 }
 
 template <typename R>
@@ -296,6 +434,7 @@ template <typename R> int ens_step(gpe_ens* e, int mode, double dre, double dim,
         int rc = build_desc<R>(e);
         if (rc) return rc;
     }
+    if (mode == GPE_ENS_FUSED || mode == GPE_ENS_FUSED_GRAPH) return ens_step_fused<R>(e, mode, dre, dim, nsteps);
     if (mode == GPE_ENS_PER_MEMBER) {
         for (int s = 0; s < nsteps; ++s)
             for (int m = 0; m < e->n; ++m) {
@@ -410,6 +549,8 @@ void gpe_ens_destroy(gpe_ens* e) {
     if (!e) return;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graph(e);
+    drop_fused_graph(e);
+    if (e->fdesc) (void)hipFree(e->fdesc);
     for (int m = 0; m < e->n; ++m) {
         void* b[] = {e->psi[m], e->ua[m], e->ub[m], e->acc[m]};
         for (void* p : b)
@@ -457,7 +598,7 @@ int gpe_ens_download(gpe_ens* e, int m, void* host, int host_dtype) {
 
 int gpe_ens_step(gpe_ens* e, int mode, double dt_re, double dt_im, int nsteps) {
     if (!e) return efail(nullptr, GPE_EINVAL, "gpe_ens_step: ens is NULL");
-    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_GROUPED || nsteps < 0)
+    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_FUSED_GRAPH || nsteps < 0)
         return efail(e, GPE_EINVAL, "gpe_ens_step: bad mode or nsteps");
     return e->mp[0].storage == GPE_C128 ? ens_step<double>(e, mode, dt_re, dt_im, nsteps)
                                         : ens_step<float>(e, mode, dt_re, dt_im, nsteps);

@@ -165,14 +165,17 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
     for (int z = start; z < z1; z += U) march_phases<U>(z, z1, step);
 }
 
+// Stages 1+2 of the fused block `bid` of a grid (the body of k_rk4_f12 and
+// of its batched form).
 template <int ORD>
-__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
-                                                  double2* __restrict__ u2out, double2 hdt) {
+__device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
+                                             const double2* __restrict__ psi, double2* __restrict__ u2out, double2 hdt,
+                                             int bid) {
     using S = FusedShape<ORD>;
     constexpr int H = S::H, D = GPE_FUSED_DEPTH;
     __shared__ double2 ldsA[2][S::LHA * S::LWA];
     __shared__ double2 ldsB[H + 1][S::NTB];
-    const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
+    const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;  // uniform per block
     const int start = c.z0 - 2 * H;
 
@@ -238,9 +241,16 @@ __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<do
 }
 
 template <int ORD>
-__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ u2,
-                                                  const double2* __restrict__ psi, double2* __restrict__ out,
-                                                  double2 dt, double2 dt3, double2 dt6) {
+__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
+                                                  double2* __restrict__ u2out, double2 hdt) {
+    rk4_f12_body<ORD>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+}
+
+// Stages 3+4 of the fused block `bid` of a grid.
+template <int ORD>
+__device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
+                                             const double2* __restrict__ u2, const double2* __restrict__ psi,
+                                             double2* __restrict__ out, double2 dt, double2 dt3, double2 dt6, int bid) {
     using S = FusedShape<ORD>;
     // one plane of loads in flight: a second set would not fit beside the
     // three register queues without spilling
@@ -248,7 +258,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
     __shared__ double2 ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ double2 ldsP[2][S::NTB];           // psi0 plane p
     __shared__ double2 ldsB[NB][S::NTB];          // u3 planes z..z+H
-    const FusedCtx<ORD> c(g, xcd_remap(blockIdx.x, g.nblocks));
+    const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
 
@@ -344,10 +354,65 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<do
     march_rotating<Q>(start, c.z1, step);
 }
 
+template <int ORD>
+__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ u2,
+                                                  const double2* __restrict__ psi, double2* __restrict__ out,
+                                                  double2 dt, double2 dt3, double2 dt6) {
+    rk4_f34_body<ORD>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
+}
+
+// Batched form for an ensemble of grids: persistent blocks walk a work list
+// of (member, fused block) items, members in descriptor order.  A member's
+// state alternates between psi and alt from step to step (flip = 1: the
+// current state is in alt), so no copy closes a step.
+struct FusedMember {
+    Geom g;  // fused_tiles geometry of the member
+    StW<double> w;
+    RhsC<double> k;
+    double2* psi;
+    double2* alt;
+    double2* u2;
+    int begin;  // first work item
+};
+
+__device__ __forceinline__ int fused_member_of(const FusedMember* __restrict__ mem, int nmem, int item) {
+    int lo = 0, hi = nmem - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mem[mid].begin <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int ORD>
+__global__ __launch_bounds__(1024) void k_rk4_f12_batched(const FusedMember* __restrict__ mem, int nmem, int total,
+                                                          double2 hdt, int flip) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
+        rk4_f12_body<ORD>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
+        __syncthreads();  // the next item reuses the LDS planes
+    }
+}
+
+template <int ORD>
+__global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __restrict__ mem, int nmem, int total,
+                                                          double2 dt, double2 dt3, double2 dt6, int flip) {
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
+        rk4_f34_body<ORD>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
+                          item - M.begin);
+        __syncthreads();
+    }
+}
+
 // Host side (gpe_k_fused.hip).
 template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);
 template <int ORD>
 hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
                             double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s);
+template <int ORD>
+hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
+                                    int flip, hipStream_t s);
 
 }  // namespace gpe

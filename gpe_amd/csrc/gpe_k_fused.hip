@@ -45,11 +45,28 @@ hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<doub
     return hipGetLastError();
 }
 
+template <int ORD>
+hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
+                                    int flip, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    if (stage_pair == 12) {
+        const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
+        hipLaunchKernelGGL((k_rk4_f12_batched<ORD>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, hdt, flip);
+    } else {
+        const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
+        hipLaunchKernelGGL((k_rk4_f34_batched<ORD>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, dt, dt3, dt6,
+                           flip);
+    }
+    return hipGetLastError();
+}
+
 template int fused_tiles<2>(int, int, int, int, Geom&);
 template int fused_tiles<4>(int, int, int, int, Geom&);
 template hipError_t launch_rk4_fused<2>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
                                         double2*, double2, int, hipStream_t);
 template hipError_t launch_rk4_fused<4>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
                                         double2*, double2, int, hipStream_t);
+template hipError_t launch_rk4_fused_batched<2>(const FusedMember*, int, int, int, double2, int, int, hipStream_t);
+template hipError_t launch_rk4_fused_batched<4>(const FusedMember*, int, int, int, double2, int, int, hipStream_t);
 
 }  // namespace gpe

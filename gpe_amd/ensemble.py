@@ -10,8 +10,9 @@ RK4 with dt = 0.005 for 1000 steps.
     python -m gpe_amd.ensemble [--steps 1000] [--members 48] [--no-tune]
 
 prints one JSON line per mode (one launch per member, batched persistent
-launches, hipGraph replay, cache-resident groups) with total cell-steps/s,
-and the autotuner log.
+launches, hipGraph replay, cache-resident groups, and for complex128 the
+fused RK4 of all members in two persistent launches per step, eager and in a
+hipGraph) with total cell-steps/s, and the autotuner log.
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ import numpy as np
 from . import _lib, ics
 from .params import RunParams, xr_for_spacing
 
-MODES = {"per_member": 0, "batched": 1, "graph": 2, "grouped": 3}
+MODES = {"per_member": 0, "batched": 1, "graph": 2, "grouped": 3, "fused": 4, "fused_graph": 5}
 SIZES = (64, 96, 128, 160, 192)
 
 
@@ -187,7 +188,8 @@ def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
                               "tz_by_size": {str(m.nx): t for m, t in zip(members, tzm)}, "log": log}), flush=True)
         rows = []
         print(json.dumps({"ensemble": "groups", "count": ens.num_groups()}), flush=True)
-        for mode in ("per_member", "batched", "graph", "grouped"):
+        modes = ["per_member", "batched", "graph", "grouped"] + (["fused", "fused_graph"] if storage == "c128" else [])
+        for mode in modes:
             for m, f in enumerate(fields):
                 ens.upload(m, f)
             ens.step(mode, dt, 2)  # warm-up (and graph capture)
@@ -196,9 +198,13 @@ def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
             t0 = time.perf_counter()
             ms = timer.time(ens, lambda: ens.step(mode, dt, steps))
             wall = time.perf_counter() - t0
+            # field accesses per cell-step: 13 for the stage kernels, 5 fused
+            acc = 5 if mode.startswith("fused") else 13
+            esz = 16 if storage == "c128" else 8
             row = {"ensemble": mode, "steps": steps, "ms_per_step": ms / steps,
-                   "gcell_steps_s": cells * steps / (ms * 1e-3) / 1e9, "wall_s": wall,
-                   "hbm_frac": cells * 13 * (16 if storage == "c128" else 8) * steps / (ms * 1e-3) / 8e12}
+                   "gcell_steps_s": cells * steps / (ms * 1e-3) / 1e9, "wall_s": wall, "accesses": acc,
+                   "hbm_frac": cells * acc * esz * steps / (ms * 1e-3) / 8e12,
+                   "equivalent_13_access_frac": cells * 13 * esz * steps / (ms * 1e-3) / 8e12}
             rows.append(row)
             print(json.dumps(row), flush=True)
         base = rows[0]["gcell_steps_s"]

@@ -265,6 +265,9 @@ int gpe_bench_stream5(void* c, void* d, const void* a, const void* b, const void
 #define GPE_ENS_BATCHED 1     /* one persistent launch per stage for all members */
 #define GPE_ENS_GRAPH 2       /* GPE_ENS_BATCHED captured in a hipGraph and replayed */
 #define GPE_ENS_GROUPED 3     /* cache-sized groups advanced a chunk of steps each, in a hipGraph */
+#define GPE_ENS_FUSED 4       /* fused RK4 (two persistent launches per step) for all members;
+                                 complex128 members with the plain right-hand side */
+#define GPE_ENS_FUSED_GRAPH 5 /* GPE_ENS_FUSED captured in a hipGraph and replayed */
 typedef struct gpe_ens gpe_ens;
 int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out);
 void gpe_ens_destroy(gpe_ens* ens);

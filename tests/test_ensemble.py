@@ -52,7 +52,7 @@ def test_create_rejects_mixed_members():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
-@pytest.mark.parametrize("mode", ["per_member", "batched", "graph", "grouped"])
+@pytest.mark.parametrize("mode", ["per_member", "batched", "graph", "grouped", "fused", "fused_graph"])
 def test_modes_match_single_context(order, bcs, mode):
     ms = members(order, bcs)
     fs = fields(ms)
@@ -109,3 +109,37 @@ def test_c64_ensemble_matches_single_context():
                 exp = ctx.download(np.complex64)
             got = ens.download(i)
             assert np.linalg.norm(got - exp) <= 1e-6 * np.linalg.norm(exp)
+
+
+@pytest.mark.gpu
+def test_fused_modes_alternate_buffers_and_replay():
+    """Fused modes keep each member's state in one of two buffers: odd step
+    counts, graph replays with eager fused steps between and downloads in
+    between must agree bit for bit with eager fused steps alone, and with the
+    single-context fused path to 1e-14 (the batched kernels inline the same
+    stage code into another kernel, which may contract differently)."""
+    ms = members(2, 1)
+    fs = fields(ms, seed=9)
+    dt = 0.004 + 0j
+    with Ensemble(ms) as ens, Ensemble(ms) as eager:
+        for e in (ens, eager):
+            for i, f in enumerate(fs):
+                e.upload(i, f)
+        total = 0
+        for mode, n in (("fused", 3), ("fused_graph", 6), ("fused_graph", 7), ("fused", 1), ("fused_graph", 4)):
+            ens.step(mode, dt, n)
+            eager.step("fused", dt, n)
+            total += n
+            for i, (m, f) in enumerate(zip(ms, fs)):
+                got = ens.download(i)
+                assert np.array_equal(got, eager.download(i)), (mode, n, i)
+                exp = single(m, f, total, dt)
+                assert np.linalg.norm(got - exp) <= 1e-14 * np.linalg.norm(exp), (mode, n, i)
+
+
+@pytest.mark.gpu
+def test_fused_modes_reject_optional_terms():
+    ms = [m.replace(eqn_to_solve=4, g=1.0, mu=2.0, omx=0.1, omy=0.1, omz=0.1) for m in members(2, 2)]
+    with Ensemble(ms) as ens:
+        with pytest.raises(RuntimeError, match="fused modes"):
+            ens.step("fused", 0.004 + 0j, 1)
