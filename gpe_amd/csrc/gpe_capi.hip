@@ -316,8 +316,11 @@ template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, dou
 }
 
 // Fused RK4 applies to complex128 storage with a plain right-hand side.
+// Fused RK4 applies to complex128 fields and a right-hand side whose only
+// optional term is the trap V.
 static bool fused_rk4(const gpe_ctx* c, int scheme) {
-    return c->fused && scheme == GPE_RK4 && c->p.storage == GPE_C128 && make_rhs_p<double>(c->p, nullptr).flags == 0;
+    const int f = make_rhs_p<double>(c->p, nullptr).flags;
+    return c->fused && scheme == GPE_RK4 && c->p.storage == GPE_C128 && (f == 0 || f == RHS_V);
 }
 
 static int need_rk_buffers(gpe_ctx* c, int scheme) {

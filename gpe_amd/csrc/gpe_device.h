@@ -147,6 +147,8 @@ template <typename R> struct RhsC {
 template <typename R, int ORD, int LW, bool GEN = true> struct Sten {
     static constexpr int H = ORD / 2;
     static constexpr bool kGeneral = GEN;
+    // optional right-hand-side terms compiled in (each still gated by RhsC::flags)
+    static constexpr int kTerms = GEN ? (RHS_V | RHS_U | RHS_DISS | RHS_MASK) : 0;
     const C<R>* L;
     int lc;
     C<R> q[2 * H + 1];
@@ -205,6 +207,12 @@ template <typename R, int ORD, int LW, bool GEN = true> struct Sten {
     }
 };
 
+// Sten with the trap term V as its only optional term: keeps the registers of
+// the other terms out of kernels that run the V-only right-hand side.
+template <typename R, int ORD, int LW> struct StenV : Sten<R, ORD, LW, true> {
+    static constexpr int kTerms = RHS_V;
+};
+
 template <typename R, class S>
 __device__ __forceinline__ C<R> rhs_eval(const RhsC<R>& k, const S& st, int x, int y, int z) {
     const C<R> p = st.c();
@@ -220,12 +228,17 @@ __device__ __forceinline__ C<R> rhs_eval(const RhsC<R>& k, const S& st, int x, i
         }
         const C<R> w = mkc<R>(k.a * L.x + s * p.x, k.a * L.y + s * p.y);
         C<R> f = mkc<R>(-k.pim * w.y, k.pim * w.x);
-        if (k.flags & RHS_DISS) {
-            // diss = diss_amp*(tx(x) + ty(y) + tz(z)); (0.5*(i + diss)) * w adds 0.5*diss*w
-            const R diss = k.diss_amp * ((k.dtab[x] + k.dtab[k.nx + y]) + k.dtab[k.nx + k.ny + z]);
-            f = caxpy<R>(R(0.5) * diss, w, f);
+        if constexpr ((S::kTerms & RHS_DISS) != 0) {
+            if (k.flags & RHS_DISS) {
+                // diss = diss_amp*(tx(x) + ty(y) + tz(z)); (0.5*(i + diss)) * w adds 0.5*diss*w
+                const R diss = k.diss_amp * ((k.dtab[x] + k.dtab[k.nx + y]) + k.dtab[k.nx + k.ny + z]);
+                f = caxpy<R>(R(0.5) * diss, w, f);
+            }
         }
-        if (k.flags & RHS_U) f = caxpy<R>(k.U, st.d1x(), f);
+        if constexpr ((S::kTerms & RHS_U) != 0) {
+            if (k.flags & RHS_U) f = caxpy<R>(k.U, st.d1x(), f);
+        }
+        if constexpr ((S::kTerms & RHS_MASK) == 0) return f;
         if (k.flags & RHS_MASK) {
             // sphere() (src/ic.f90:635-657): 0.5*(1 + tanh(r - 10 - 2))
             const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;

@@ -1,6 +1,6 @@
 // Fused RK4: the four stages of a classic RK4 step in two kernels that each
 // evaluate two stages (temporal blocking in the z-march), for complex128
-// fields and a right-hand side without optional terms (RhsC::flags == 0).
+// fields and a right-hand side whose only optional term is the trap V.
 //
 //   F12: u1 = psi0 + dt/2 f(psi0) on the whole block, then
 //        u2 = psi0 + dt/2 f(u1) on the block's inner tile      reads psi0        writes u2
@@ -36,8 +36,11 @@
 // bcmap-mapped inputs.  With periodic boundaries that is the wrapped value;
 // with even reflection the axis stencils of the Laplacian are symmetric, so
 // the unfolded value equals the value at the mirrored index bit for bit.
-// This is why the optional terms (position-dependent V, d/dx, masks) are
-// excluded: the unfused kernels handle them.
+// The trap V (TERMS = RHS_V), a pointwise position-dependent term, is
+// evaluated at the mapped indices, so it keeps that equality.  The other
+// optional terms are left to the unfused kernels: Urhs d/dx is antisymmetric
+// under reflection, and the dissipation and eqn-3 mask would push these
+// kernels past their registers.
 #pragma once
 
 #include "gpe_kernels.h"
@@ -61,6 +64,7 @@ template <int ORD> struct FusedCtx {
     int z0, z1;
     bool active, inner, store;
     int lx, ly, gx, gy;
+    int mx, my;  // the column's in-domain (mapped) indices
     unsigned col, own;
     int la, lb;
     unsigned hoff[S::NSLOT];
@@ -82,7 +86,9 @@ template <int ORD> struct FusedCtx {
         gy = oy0 - S::H + ly;
         inner = active && lx >= S::H && lx < S::TXB - S::H && ly >= S::H && ly < S::TYB - S::H;
         store = inner && gx < g.nx && gy < g.ny;
-        col = (unsigned)bcmap(gx, g.nx, g.bcs) + (unsigned)g.nx * (unsigned)bcmap(gy, g.ny, g.bcs);
+        mx = bcmap(gx, g.nx, g.bcs);
+        my = bcmap(gy, g.ny, g.bcs);
+        col = (unsigned)mx + (unsigned)g.nx * (unsigned)my;
         own = store ? (unsigned)gx + (unsigned)g.nx * (unsigned)gy : 0u;
         la = (ly + S::H) * S::LWA + lx + S::H;
         lb = ly * S::TXB + lx;
@@ -167,7 +173,11 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
 
 // Stages 1+2 of the fused block `bid` of a grid (the body of k_rk4_f12 and
 // of its batched form).
-template <int ORD>
+// TERMS: 0 (no optional right-hand-side term) or RHS_V (the trap only).
+template <int ORD, int LW, int TERMS>
+using FusedSten = std::conditional_t<TERMS == 0, Sten<double, ORD, LW, false>, StenV<double, ORD, LW>>;
+
+template <int ORD, int TERMS>
 __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
                                              const double2* __restrict__ psi, double2* __restrict__ u2out, double2 hdt,
                                              int bid) {
@@ -210,25 +220,25 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w
         if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // first stage at plane p, whole block
-            Sten<double, ORD, S::LWA, false> sa;
+            FusedSten<ORD, S::LWA, TERMS> sa;
             sa.L = A;
             sa.lc = c.la;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qp[d];
             sa.w = w;
-            const double2 u1 = cfma<double>(hdt, rhs_eval<double>(k, sa, 0, 0, 0), qp[H]);
+            const double2 u1 = cfma<double>(hdt, rhs_eval<double>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs)), qp[H]);
             qv[2 * H] = u1;
             ldsB[(p - start) % (H + 1)][c.lb] = u1;
         }
         if (z >= c.z0 && c.inner) {
             // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
-            Sten<double, ORD, S::TXB, false> sb;
+            FusedSten<ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[(z - start) % (H + 1)];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
             sb.w = w;
-            const double2 u2 = cfma<double>(hdt, rhs_eval<double>(k, sb, 0, 0, 0), qp[0]);
+            const double2 u2 = cfma<double>(hdt, rhs_eval<double>(k, sb, c.mx, c.my, z), qp[0]);
             if (c.store) st_stream<double>(u2out + Cell{g.plane * z, c.own}, u2);
         }
 #pragma unroll
@@ -240,14 +250,14 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w
     fused_march<D, Pf>(start, c.z1, fetch, step);
 }
 
-template <int ORD>
+template <int ORD, int TERMS>
 __global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
                                                   double2* __restrict__ u2out, double2 hdt) {
-    rk4_f12_body<ORD>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+    rk4_f12_body<ORD, TERMS>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
-template <int ORD>
+template <int ORD, int TERMS>
 __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
                                              const double2* __restrict__ u2, const double2* __restrict__ psi,
                                              double2* __restrict__ out, double2 dt, double2 dt3, double2 dt6, int bid) {
@@ -312,26 +322,26 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
         if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
-            Sten<double, ORD, S::LWA, false> sa;
+            FusedSten<ORD, S::LWA, TERMS> sa;
             sa.L = A;
             sa.lc = c.la;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
             sa.w = w;
-            const double2 f2 = rhs_eval<double>(k, sa, 0, 0, 0);
+            const double2 f2 = rhs_eval<double>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
             const double2 p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
             const double2 u3 = cfma<double>(dt, f2, p0);
             qv[(PH + 2 * H) % Q] = u3;
             ldsB[(p - start) % NB][c.lb] = u3;
             if (c.inner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
-                Sten<double, ORD, S::TXB, false> sp;
+                FusedSten<ORD, S::TXB, TERMS> sp;
                 sp.L = P;
                 sp.lc = c.lb;
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[(PH + d) % Q];
                 sp.w = w;
-                const double2 f1 = rhs_eval<double>(k, sp, 0, 0, 0);
+                const double2 f1 = rhs_eval<double>(k, sp, c.mx, c.my, bcmap(p, g.nz, g.bcs));
                 const double2 base = mkc<double>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
                 qa[H] = cfma<double>(dt3, f2, cfma<double>(dt6, f1, base));
             }
@@ -339,13 +349,13 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
         if (z >= c.z0 && c.inner) {
             // fourth stage at plane z, inner tile (ldsB of plane z was written
             // H iterations ago)
-            Sten<double, ORD, S::TXB, false> sb;
+            FusedSten<ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[(z - start) % NB];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
-            const double2 nw = cfma<double>(dt6, rhs_eval<double>(k, sb, 0, 0, 0), qa[0]);
+            const double2 nw = cfma<double>(dt6, rhs_eval<double>(k, sb, c.mx, c.my, z), qa[0]);
             if (c.store) st_stream<double>(out + Cell{g.plane * z, c.own}, nw);
         }
 #pragma unroll
@@ -354,11 +364,11 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
     march_rotating<Q>(start, c.z1, step);
 }
 
-template <int ORD>
+template <int ORD, int TERMS>
 __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ u2,
                                                   const double2* __restrict__ psi, double2* __restrict__ out,
                                                   double2 dt, double2 dt3, double2 dt6) {
-    rk4_f34_body<ORD>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
+    rk4_f34_body<ORD, TERMS>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Batched form for an ensemble of grids: persistent blocks walk a work list
@@ -390,7 +400,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f12_batched(const FusedMember* __r
                                                           double2 hdt, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f12_body<ORD>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
+        rk4_f12_body<ORD, 0>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
         __syncthreads();  // the next item reuses the LDS planes
     }
 }
@@ -400,7 +410,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __r
                                                           double2 dt, double2 dt3, double2 dt6, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f34_body<ORD>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
+        rk4_f34_body<ORD, 0>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
                           item - M.begin);
         __syncthreads();
     }
@@ -410,7 +420,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __r
 template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);
 template <int ORD>
 hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
-                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s);
+                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s);  // g.general: GEN
 template <int ORD>
 hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
                                     int flip, hipStream_t s);

@@ -36,11 +36,18 @@ hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<doub
     if (g.nblocks <= 0) return hipSuccess;
     if (stage_pair == 12) {
         const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
-        hipLaunchKernelGGL((k_rk4_f12<ORD>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+        if (g.general)
+            hipLaunchKernelGGL((k_rk4_f12<ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+        else
+            hipLaunchKernelGGL((k_rk4_f12<ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
     } else {
         const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
-        hipLaunchKernelGGL((k_rk4_f34<ORD>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const double2*)u2, psi, out,
-                           dt, dt3, dt6);
+        if (g.general)
+            hipLaunchKernelGGL((k_rk4_f34<ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const double2*)u2,
+                               psi, out, dt, dt3, dt6);
+        else
+            hipLaunchKernelGGL((k_rk4_f34<ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k,
+                               (const double2*)u2, psi, out, dt, dt3, dt6);
     }
     return hipGetLastError();
 }

@@ -286,7 +286,7 @@ def test_fused_rk4_matches_oracle_and_stage_kernels(orc, shape, order, bcs, eqn)
 
 def test_fused_rk4_imaginary_time_and_fallback(orc):
     """Complex dt through the fused kernels (eqn 2, imaginary time), and a
-    right-hand side with a trap (eqn 4), which falls back to the stage kernels."""
+    right-hand side with Urhs d/dx, which falls back to the stage kernels."""
     p = RunParams(nx=30, ny=26, nz=22, xr=8.0, yr=7.0, zr=6.0, order=4, bcs=2, eqn_to_solve=2, real_time=False,
                   tau=0.004)
     psi = smooth_field(p, seed=8)
@@ -300,11 +300,37 @@ def test_fused_rk4_imaginary_time_and_fallback(orc):
     assert rel(got, exp) < 1e-12
     assert abs(info.im_t - t[1]) < 1e-15
     assert set(prof) == {"rk4_fused12", "rk4_fused34"}
-    p4 = p.replace(eqn_to_solve=4, g=1.0, mu=1.0, omx=0.1, omy=0.1, omz=0.1, real_time=True)
-    with Context(p4) as ctx:
+    pu = p.replace(Urhs=0.3, real_time=True)
+    with Context(pu) as ctx:
         ctx.upload(psi)
-        prof4 = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(0.01 + 0j)))
-    assert "rk4_fused12" not in prof4 and "rk4_stage3" in prof4
+        profu = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(0.01 + 0j)))
+    assert "rk4_fused12" not in profu and "rk4_stage3" in profu
+
+
+@pytest.mark.parametrize("shape", [(30, 26, 22), (37, 21, 19), (6, 6, 6)])
+@pytest.mark.parametrize("order,bcs", [(4, 2), (2, 2), (4, 1)])
+@pytest.mark.parametrize("real_time", [True, False])
+def test_fused_rk4_with_trap(orc, shape, order, bcs, real_time):
+    """Eqn 4 with an anisotropic trap through the fused kernels (the trap
+    evaluated at the mapped indices of unfolded cells): against the FP64
+    oracle (1e-12) and the four stage kernels (1e-13), in real and
+    imaginary time."""
+    nx, ny, nz = shape
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=0.3 * nx, yr=0.3 * ny, zr=0.3 * nz, order=order, bcs=bcs, eqn_to_solve=4,
+                  g=1.0, mu=2.0, omx=0.3, omy=0.2, omz=0.4, real_time=real_time, tau=0.004)
+    psi = smooth_field(p, seed=11)
+    exp, t = orc.rk4_n(p, psi, p.dt0(), 5)
+    outs = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            ctx.step("rk4", 5, StepInfo(p.dt0()))
+            outs[fused] = ctx.download()
+            prof = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(p.dt0())))
+            assert ("rk4_fused12" in prof) == fused
+    assert rel(outs[True], exp) < 1e-12
+    assert rel(outs[True], outs[False]) < 1e-13
 
 
 @pytest.mark.gpu
