@@ -316,11 +316,12 @@ template <typename R, int ORD> static int step_fixed(gpe_ctx* c, int scheme, dou
 }
 
 // Fused RK4 applies to complex128 storage with a plain right-hand side.
-// Fused RK4 applies to complex128 fields and a right-hand side whose only
-// optional term is the trap V.
+// Fused RK4 applies to complex128 and complex64 fields and a right-hand side
+// whose only optional term is the trap V.
 static bool fused_rk4(const gpe_ctx* c, int scheme) {
     const int f = make_rhs_p<double>(c->p, nullptr).flags;
-    return c->fused && scheme == GPE_RK4 && c->p.storage == GPE_C128 && (f == 0 || f == RHS_V);
+    return c->fused && scheme == GPE_RK4 && (c->p.storage == GPE_C128 || c->p.storage == GPE_C64) &&
+           (f == 0 || f == RHS_V);
 }
 
 static int need_rk_buffers(gpe_ctx* c, int scheme) {
@@ -449,30 +450,30 @@ static int ck_attempt_mixed(gpe_ctx* c, double dre, double dim, double d0re, dou
 
 // Launch-accounting tags of the fused kernels (run_launch reads these).
 struct FusedTag12 {
-    static constexpr int kKind = 25, kBytesPerCell = 32, kNRed = 0, kMode = RED_NONE;
+    static constexpr int kKind = 25, kAccess = 2, kNRed = 0, kMode = RED_NONE;
 };
 struct FusedTag34 {
-    static constexpr int kKind = 26, kBytesPerCell = 48, kNRed = 0, kMode = RED_NONE;
+    static constexpr int kKind = 26, kAccess = 3, kNRed = 0, kMode = RED_NONE;
 };
 
 // One classic RK4 step as two fused kernels (gpe_fused.h): psi -> u2 (uA),
 // then (u2, psi) -> psi_new (pnew, swapped in).
-template <int ORD> static int step_rk4_fused(gpe_ctx* c, double dre, double dim) {
-    const RhsC<double> k = make_rhs<double>(c);
-    const StW<double> w = make_w<double>(c->p);
-    const double2 dt = cdt<double>(dre, dim, 1.0);
-    Geom g = make_geom<double>(c);
+template <typename R, int ORD> static int step_rk4_fused(gpe_ctx* c, double dre, double dim) {
+    const RhsC<R> k = make_rhs<R>(c);
+    const StW<R> w = make_w<R>(c->p);
+    const C<R> dt = cdt<R>(dre, dim, 1.0);
+    Geom g = make_geom<R>(c);
     fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
-    const double2* psi = (const double2*)c->psi;
-    double2* u2 = (double2*)c->uA;
-    double2* out = (double2*)c->pnew;
+    const C<R>* psi = (const C<R>*)c->psi;
+    C<R>* u2 = (C<R>*)c->uA;
+    C<R>* out = (C<R>*)c->pnew;
     int rc;
     if ((rc = run_launch<FusedTag12>(c, g, [&](Geom gg) {
-             return launch_rk4_fused<ORD>(gg, w, k, psi, u2, nullptr, dt, 12, c->stream);
+             return launch_rk4_fused<R, ORD>(gg, w, k, psi, u2, nullptr, dt, 12, c->stream);
          })))
         return rc;
     if ((rc = run_launch<FusedTag34>(c, g, [&](Geom gg) {
-             return launch_rk4_fused<ORD>(gg, w, k, psi, u2, out, dt, 34, c->stream);
+             return launch_rk4_fused<R, ORD>(gg, w, k, psi, u2, out, dt, 34, c->stream);
          })))
         return rc;
     std::swap(c->psi, c->pnew);
@@ -822,8 +823,11 @@ int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
     const int nrhs = scheme == GPE_EULER ? 1 : scheme == GPE_RK2 ? 2 : 4;
     for (int s = 0; s < nsteps; ++s) {
         if (fused_rk4(c, scheme))
-            rc = c->p.order == 2 ? step_rk4_fused<2>(c, io->dt_re, io->dt_im)
-                                 : step_rk4_fused<4>(c, io->dt_re, io->dt_im);
+            rc = c->p.storage == GPE_C64
+                     ? (c->p.order == 2 ? step_rk4_fused<float, 2>(c, io->dt_re, io->dt_im)
+                                        : step_rk4_fused<float, 4>(c, io->dt_re, io->dt_im))
+                     : (c->p.order == 2 ? step_rk4_fused<double, 2>(c, io->dt_re, io->dt_im)
+                                        : step_rk4_fused<double, 4>(c, io->dt_re, io->dt_im));
         else if (c->p.storage == GPE_MIXED && scheme == GPE_RK4)
             rc = c->p.order == 2 ? step_rk4_mixed<2>(c, io->dt_re, io->dt_im)
                                  : step_rk4_mixed<4>(c, io->dt_re, io->dt_im);

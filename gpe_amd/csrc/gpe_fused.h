@@ -1,6 +1,7 @@
 // Fused RK4: the four stages of a classic RK4 step in two kernels that each
 // evaluate two stages (temporal blocking in the z-march), for complex128
-// fields and a right-hand side whose only optional term is the trap V.
+// (FP64) or complex64 (FP32) fields and a right-hand side whose only optional
+// term is the trap V.
 //
 //   F12: u1 = psi0 + dt/2 f(psi0) on the whole block, then
 //        u2 = psi0 + dt/2 f(u1) on the block's inner tile      reads psi0        writes u2
@@ -174,33 +175,33 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
 // Stages 1+2 of the fused block `bid` of a grid (the body of k_rk4_f12 and
 // of its batched form).
 // TERMS: 0 (no optional right-hand-side term) or RHS_V (the trap only).
-template <int ORD, int LW, int TERMS>
-using FusedSten = std::conditional_t<TERMS == 0, Sten<double, ORD, LW, false>, StenV<double, ORD, LW>>;
+template <typename R, int ORD, int LW, int TERMS>
+using FusedSten = std::conditional_t<TERMS == 0, Sten<R, ORD, LW, false>, StenV<R, ORD, LW>>;
 
-template <int ORD, int TERMS>
-__device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
-                                             const double2* __restrict__ psi, double2* __restrict__ u2out, double2 hdt,
+template <typename R, int ORD, int TERMS>
+__device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
+                                             const C<R>* __restrict__ psi, C<R>* __restrict__ u2out, C<R> hdt,
                                              int bid) {
     using S = FusedShape<ORD>;
     constexpr int H = S::H, D = GPE_FUSED_DEPTH;
-    __shared__ double2 ldsA[2][S::LHA * S::LWA];
-    __shared__ double2 ldsB[H + 1][S::NTB];
+    __shared__ C<R> ldsA[2][S::LHA * S::LWA];
+    __shared__ C<R> ldsB[H + 1][S::NTB];
     const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;  // uniform per block
     const int start = c.z0 - 2 * H;
 
-    double2 qp[2 * H + 1];  // psi0, planes z..z+2H
-    double2 qv[2 * H + 1];  // u1, planes z-H..z+H
+    C<R> qp[2 * H + 1];  // psi0, planes z..z+2H
+    C<R> qv[2 * H + 1];  // u1, planes z-H..z+H
 #pragma unroll
-    for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<double>(0.0, 0.0);
+    for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<R>(0.0, 0.0);
     if (c.active) {
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) qp[d] = *(psi + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
     }
     // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H
     struct Pf {
-        double2 qn;
-        double2 hr[S::NSLOT];
+        C<R> qn;
+        C<R> hr[S::NSLOT];
     };
     auto fetch = [&](Pf& F, int zz) {
         if (c.active) F.qn = *(psi + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
@@ -211,7 +212,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w
     auto step = [&](int z, Pf& F) {
         const int p = z + H;
         qp[2 * H] = F.qn;
-        double2* A = ldsA[(z - start) & 1];
+        C<R>* A = ldsA[(z - start) & 1];
         if (c.active) A[c.la] = qp[H];
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s)
@@ -220,26 +221,26 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w
         if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // first stage at plane p, whole block
-            FusedSten<ORD, S::LWA, TERMS> sa;
+            FusedSten<R, ORD, S::LWA, TERMS> sa;
             sa.L = A;
             sa.lc = c.la;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qp[d];
             sa.w = w;
-            const double2 u1 = cfma<double>(hdt, rhs_eval<double>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs)), qp[H]);
+            const C<R> u1 = cfma<R>(hdt, rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs)), qp[H]);
             qv[2 * H] = u1;
             ldsB[(p - start) % (H + 1)][c.lb] = u1;
         }
         if (z >= c.z0 && c.inner) {
             // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
-            FusedSten<ORD, S::TXB, TERMS> sb;
+            FusedSten<R, ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[(z - start) % (H + 1)];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
             sb.w = w;
-            const double2 u2 = cfma<double>(hdt, rhs_eval<double>(k, sb, c.mx, c.my, z), qp[0]);
-            if (c.store) st_stream<double>(u2out + Cell{g.plane * z, c.own}, u2);
+            const C<R> u2 = cfma<R>(hdt, rhs_eval<R>(k, sb, c.mx, c.my, z), qp[0]);
+            if (c.store) st_stream<R>(u2out + Cell{g.plane * z, c.own}, u2);
         }
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) {
@@ -250,35 +251,35 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<double>& w
     fused_march<D, Pf>(start, c.z1, fetch, step);
 }
 
-template <int ORD, int TERMS>
-__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ psi,
-                                                  double2* __restrict__ u2out, double2 hdt) {
-    rk4_f12_body<ORD, TERMS>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+template <typename R, int ORD, int TERMS>
+__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ psi,
+                                                  C<R>* __restrict__ u2out, C<R> hdt) {
+    rk4_f12_body<R, ORD, TERMS>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
-template <int ORD, int TERMS>
-__device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
-                                             const double2* __restrict__ u2, const double2* __restrict__ psi,
-                                             double2* __restrict__ out, double2 dt, double2 dt3, double2 dt6, int bid) {
+template <typename R, int ORD, int TERMS>
+__device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
+                                             const C<R>* __restrict__ u2, const C<R>* __restrict__ psi,
+                                             C<R>* __restrict__ out, C<R> dt, C<R> dt3, C<R> dt6, int bid) {
     using S = FusedShape<ORD>;
     // one plane of loads in flight: a second set would not fit beside the
     // three register queues without spilling
     constexpr int H = S::H, D = 1, NB = H + 1, Q = 2 * H + 1;
-    __shared__ double2 ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
-    __shared__ double2 ldsP[2][S::NTB];           // psi0 plane p
-    __shared__ double2 ldsB[NB][S::NTB];          // u3 planes z..z+H
+    __shared__ C<R> ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
+    __shared__ C<R> ldsP[2][S::NTB];           // psi0 plane p
+    __shared__ C<R> ldsB[NB][S::NTB];          // u3 planes z..z+H
     const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
 
     // Register queues, slot (plane - start) % Q: u2 and psi0 for planes
     // z..z+2H, u3 for planes z-H..z+H, acc for the inner planes z..z+H.
-    double2 qu[Q], qp[Q], qv[Q], qa[H + 1];
+    C<R> qu[Q], qp[Q], qv[Q], qa[H + 1];
 #pragma unroll
-    for (int d = 0; d < Q; ++d) qv[d] = mkc<double>(0.0, 0.0);
+    for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
 #pragma unroll
-    for (int d = 0; d <= H; ++d) qa[d] = mkc<double>(0.0, 0.0);
+    for (int d = 0; d <= H; ++d) qa[d] = mkc<R>(0.0, 0.0);
     if (c.active) {
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) {
@@ -287,10 +288,10 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
             qp[d] = *(psi + Cell{zb, c.col});
         }
     }
-    constexpr double two3 = 2.0 / 3.0;
+    constexpr R two3 = R(2.0 / 3.0);
     struct Pf {
-        double2 qnu, qnp;
-        double2 hr[S::NSLOT];
+        C<R> qnu, qnp;
+        C<R> hr[S::NSLOT];
     };
     auto fetch = [&](Pf& F, int zz) {
         if (c.active) {
@@ -309,8 +310,8 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
         const int p = z + H;
         qu[(PH + 2 * H) % Q] = F.qnu;
         qp[(PH + 2 * H) % Q] = F.qnp;
-        double2* A = ldsA[(z - start) & 1];
-        double2* P = ldsP[(z - start) & 1];
+        C<R>* A = ldsA[(z - start) & 1];
+        C<R>* P = ldsP[(z - start) & 1];
         if (c.active) {
             A[c.la] = qu[(PH + H) % Q];
             P[c.lb] = qp[(PH + H) % Q];
@@ -322,41 +323,41 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
         if (z + D < c.z1) fetch(F, z + D);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
-            FusedSten<ORD, S::LWA, TERMS> sa;
+            FusedSten<R, ORD, S::LWA, TERMS> sa;
             sa.L = A;
             sa.lc = c.la;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
             sa.w = w;
-            const double2 f2 = rhs_eval<double>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
-            const double2 p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
-            const double2 u3 = cfma<double>(dt, f2, p0);
+            const C<R> f2 = rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+            const C<R> p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
+            const C<R> u3 = cfma<R>(dt, f2, p0);
             qv[(PH + 2 * H) % Q] = u3;
             ldsB[(p - start) % NB][c.lb] = u3;
             if (c.inner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
-                FusedSten<ORD, S::TXB, TERMS> sp;
+                FusedSten<R, ORD, S::TXB, TERMS> sp;
                 sp.L = P;
                 sp.lc = c.lb;
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[(PH + d) % Q];
                 sp.w = w;
-                const double2 f1 = rhs_eval<double>(k, sp, c.mx, c.my, bcmap(p, g.nz, g.bcs));
-                const double2 base = mkc<double>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
-                qa[H] = cfma<double>(dt3, f2, cfma<double>(dt6, f1, base));
+                const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+                const C<R> base = mkc<R>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
+                qa[H] = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
             }
         }
         if (z >= c.z0 && c.inner) {
             // fourth stage at plane z, inner tile (ldsB of plane z was written
             // H iterations ago)
-            FusedSten<ORD, S::TXB, TERMS> sb;
+            FusedSten<R, ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[(z - start) % NB];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
-            const double2 nw = cfma<double>(dt6, rhs_eval<double>(k, sb, c.mx, c.my, z), qa[0]);
-            if (c.store) st_stream<double>(out + Cell{g.plane * z, c.own}, nw);
+            const C<R> nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z), qa[0]);
+            if (c.store) st_stream<R>(out + Cell{g.plane * z, c.own}, nw);
         }
 #pragma unroll
         for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
@@ -364,11 +365,11 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<double>& w
     march_rotating<Q>(start, c.z1, step);
 }
 
-template <int ORD, int TERMS>
-__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<double> w, RhsC<double> k, const double2* __restrict__ u2,
-                                                  const double2* __restrict__ psi, double2* __restrict__ out,
-                                                  double2 dt, double2 dt3, double2 dt6) {
-    rk4_f34_body<ORD, TERMS>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
+template <typename R, int ORD, int TERMS>
+__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ u2,
+                                                  const C<R>* __restrict__ psi, C<R>* __restrict__ out,
+                                                  C<R> dt, C<R> dt3, C<R> dt6) {
+    rk4_f34_body<R, ORD, TERMS>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Batched form for an ensemble of grids: persistent blocks walk a work list
@@ -400,7 +401,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f12_batched(const FusedMember* __r
                                                           double2 hdt, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f12_body<ORD, 0>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
+        rk4_f12_body<double, ORD, 0>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
         __syncthreads();  // the next item reuses the LDS planes
     }
 }
@@ -410,7 +411,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __r
                                                           double2 dt, double2 dt3, double2 dt6, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f34_body<ORD, 0>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
+        rk4_f34_body<double, ORD, 0>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
                           item - M.begin);
         __syncthreads();
     }
@@ -418,9 +419,9 @@ __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __r
 
 // Host side (gpe_k_fused.hip).
 template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);
-template <int ORD>
-hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
-                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s);  // g.general: GEN
+template <typename R, int ORD>
+hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
+                            C<R> dt, int stage_pair, hipStream_t s);  // g.general: the trap term
 template <int ORD>
 hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
                                     int flip, hipStream_t s);

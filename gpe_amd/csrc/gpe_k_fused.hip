@@ -30,24 +30,24 @@ template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
     return g.nblocks;
 }
 
-template <int ORD>
-hipError_t launch_rk4_fused(const Geom& g, const StW<double>& w, const RhsC<double>& k, const double2* psi,
-                            double2* u2, double2* out, double2 dt, int stage_pair, hipStream_t s) {
+template <typename R, int ORD>
+hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
+                            C<R> dt, int stage_pair, hipStream_t s) {
     if (g.nblocks <= 0) return hipSuccess;
     if (stage_pair == 12) {
-        const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
+        const C<R> hdt = mkc<R>(R(0.5) * dt.x, R(0.5) * dt.y);
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f12<ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
         else
-            hipLaunchKernelGGL((k_rk4_f12<ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
     } else {
-        const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
+        const C<R> dt3 = mkc<R>(dt.x / R(3), dt.y / R(3)), dt6 = mkc<R>(dt.x / R(6), dt.y / R(6));
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f34<ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const double2*)u2,
-                               psi, out, dt, dt3, dt6);
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k,
+                               (const C<R>*)u2, psi, out, dt, dt3, dt6);
         else
-            hipLaunchKernelGGL((k_rk4_f34<ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k,
-                               (const double2*)u2, psi, out, dt, dt3, dt6);
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const C<R>*)u2,
+                               psi, out, dt, dt3, dt6);
     }
     return hipGetLastError();
 }
@@ -69,10 +69,13 @@ hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total,
 
 template int fused_tiles<2>(int, int, int, int, Geom&);
 template int fused_tiles<4>(int, int, int, int, Geom&);
-template hipError_t launch_rk4_fused<2>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
-                                        double2*, double2, int, hipStream_t);
-template hipError_t launch_rk4_fused<4>(const Geom&, const StW<double>&, const RhsC<double>&, const double2*, double2*,
-                                        double2*, double2, int, hipStream_t);
+#define GPE_FUSED_INST(R, ORD)                                                                                     \
+    template hipError_t launch_rk4_fused<R, ORD>(const Geom&, const StW<R>&, const RhsC<R>&, const C<R>*, C<R>*, C<R>*, \
+                                                 C<R>, int, hipStream_t);
+GPE_FUSED_INST(double, 2)
+GPE_FUSED_INST(double, 4)
+GPE_FUSED_INST(float, 2)
+GPE_FUSED_INST(float, 4)
 template hipError_t launch_rk4_fused_batched<2>(const FusedMember*, int, int, int, double2, int, int, hipStream_t);
 template hipError_t launch_rk4_fused_batched<4>(const FusedMember*, int, int, int, double2, int, int, hipStream_t);
 

@@ -39,14 +39,14 @@ ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3,
 MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
                "mx_rkck_stage1": 24, "mx_rkck_stage2": 40, "mx_rkck_stage3": 56, "mx_rkck_stage4": 64,
                "mx_rkck_stage5": 72, "mx_rkck_stage6_error": 72}
-FUSED_BYTES = {"rk4_fused12": 32, "rk4_fused34": 48}  # complex128 only
+FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3}  # field accesses per cell
 # kernels whose calibrated traffic bench.py reads back (traffic_<kind>_<prec>_<n>.json)
 TRAFFIC_KINDS = {"rk4_fused12", "rk4_fused34", "rk4_stage3", "mx_rk4_stage4"}
 
 
 def cell_bytes(kind: str, prec: str) -> int:
-    if kind in FUSED_BYTES:
-        return FUSED_BYTES[kind]
+    if kind in FUSED_ACCESS:
+        return FUSED_ACCESS[kind] * (16 if prec == "c128" else 8)
     if prec == "mixed":
         return MIXED_BYTES.get(kind, 0)
     return ACCESS.get(kind, 0) * (16 if prec == "c128" else 8)
@@ -72,10 +72,12 @@ def kernel_label(name: str) -> str:
         if kind.startswith("mx_"):
             prec = "mixed"
         return f"{kind} {prec} order{order} tile{tx}x{ty}"
-    mf = re.search(r"k_rk4_f(12|34)<(\d)>", name)
+    mf = re.search(r"k_rk4_f(12|34)<(double|float), (\d), (\d+)>", name)
     if mf:
-        h = int(mf.group(2)) // 2
-        return f"rk4_fused{mf.group(1)} c128 order{mf.group(2)} tile{64 + 2 * h}x15"
+        prec = "c128" if mf.group(2) == "double" else "c64"
+        h = int(mf.group(3)) // 2
+        trap = " with-trap" if mf.group(4) != "0" else ""
+        return f"rk4_fused{mf.group(1)} {prec} order{mf.group(3)} tile{64 + 2 * h}x15{trap}"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
@@ -138,8 +140,9 @@ def main(argv=None):
         if "k_stencil" not in name and "k_rk4_f" not in name:
             continue
         label = kernel_label(name)
-        kind = label.split()[0]
-        prec = label.split()[1]
+        if len(label.split()) < 2:  # not a kernel of a known form (e.g. the batched ensemble's)
+            continue
+        kind, prec = label.split()[:2]
         alg = cell_bytes(kind, prec) * cells
         avg_ms = float(r["AverageNs"]) / 1e6
         fk = next((v for k, v in fetch.items() if k.startswith(name[:120])), None)

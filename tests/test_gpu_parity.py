@@ -365,3 +365,29 @@ def test_imprint_line_matches_host_generator(storage):
             ctx.imprint_line(ics.LineParam(amp1=0.5, ll=0.0))
         with pytest.raises(GpeError):
             ctx.peek(20, 0, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,bcs,eqn", [(2, 1, 1), (4, 2, 1), (4, 2, 4), (2, 2, 2)])
+def test_fused_rk4_complex64(orc, order, bcs, eqn):
+    """Complex64 storage through the fused kernels (FP32 stencil and stages):
+    against the complex64 stage kernels and the FP64 oracle at FP32 accuracy
+    (6 steps: 2e-6 of the field)."""
+    p = RunParams(nx=37, ny=21, nz=19, xr=0.35 * 37, yr=0.35 * 21, zr=0.35 * 19, order=order, bcs=bcs,
+                  eqn_to_solve=eqn, storage="c64")
+    if eqn == 4:
+        p = p.replace(g=1.0, mu=2.0, omx=0.3, omy=0.2, omz=0.4)
+    psi = smooth_field(p, seed=12)
+    exp, _ = orc.rk4_n(p, psi, 0.01 + 0j, 6)
+    outs = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi.astype(np.complex64))
+            ctx.step("rk4", 6, StepInfo(0.01 + 0j))
+            outs[fused] = ctx.download()
+            prof = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(0.01 + 0j)))
+            assert ("rk4_fused12" in prof) == fused
+    print(f"c64 fused vs oracle {rel(outs[True], exp):.2e}, vs stage kernels {rel(outs[True], outs[False]):.2e}")
+    assert rel(outs[True], exp) < 2e-6
+    assert rel(outs[True], outs[False]) < 2e-6
