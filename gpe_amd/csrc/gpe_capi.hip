@@ -95,7 +95,7 @@ static const char* const kKindNames[GPE_NKINDS] = {
     "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
     "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
     "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error",
-    "rk4_fused12", "rk4_fused34", "momentum", "velocity"};
+    "rk4_fused12", "rk4_fused34", "momentum", "velocity", "rkck_fused12", "rkck_fused34", "rkck_fused56_error"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -324,6 +324,12 @@ static bool fused_rk4(const gpe_ctx* c, int scheme) {
            (f == 0 || f == RHS_V);
 }
 
+// The fused Cash-Karp attempt has the same conditions.
+static bool fused_ck(const gpe_ctx* c) {
+    const int f = make_rhs_p<double>(c->p, nullptr).flags;
+    return c->fused && (c->p.storage == GPE_C128 || c->p.storage == GPE_C64) && (f == 0 || f == RHS_V);
+}
+
 static int need_rk_buffers(gpe_ctx* c, int scheme) {
     int rc;
     const size_t fb = field_bytes(c);
@@ -346,7 +352,7 @@ static int need_rk_buffers(gpe_ctx* c, int scheme) {
         if ((rc = dalloc(c, &c->k2, fb, "k2"))) return rc;
         if ((rc = dalloc(c, &c->k3, fb, "k3"))) return rc;
         if ((rc = dalloc(c, &c->k4, fb, "k4"))) return rc;
-        if ((rc = dalloc(c, &c->k5, fb, "k5"))) return rc;
+        if (!fused_ck(c) && (rc = dalloc(c, &c->k5, fb, "k5"))) return rc;
         if ((rc = dalloc(c, &c->pnew, fb, "pnew"))) return rc;
     }
     return 0;
@@ -480,11 +486,80 @@ template <typename R, int ORD> static int step_rk4_fused(gpe_ctx* c, double dre,
     return 0;
 }
 
+struct FusedCkTag1 {
+    static constexpr int kKind = 29, kAccess = 4, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedCkTag2 {
+    static constexpr int kKind = 30, kAccess = 7, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedCkTag3 {
+    static constexpr int kKind = 31, kAccess = 7, kNRed = 2, kMode = RED_MAX;
+};
+
+// One Cash-Karp attempt as three fused kernels (gpe_fused.h): psi -> f0, k2,
+// u3 (uA); u3 -> k3, k4, u5 (uB); u5 -> psi_new (pnew) with the error
+// max-norm, as ck_attempt.  A retry evaluates f(psi0) again inside the first
+// kernel (bit for bit the stored f0), so the retry costs one field read more
+// than reusing f0 would and no extra pass.
+template <typename R, int ORD>
+static int ck_attempt_fused(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    const RhsC<R> k = make_rhs<R>(c);
+    const StW<R> w = make_w<R>(c->p);
+    Geom g = make_geom<R>(c);
+    fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
+    CkPairArgs<R> a{};
+    a.p0 = (const C<R>*)c->psi;
+    a.f0 = (const C<R>*)c->f0;
+    a.k2 = (const C<R>*)c->k2;
+    a.k3 = (const C<R>*)c->k3;
+    a.k4 = (const C<R>*)c->k4;
+    a.dt = cdt<R>(dre, dim, 1.0);
+    a.t = make_tab<R>();
+    a.dt0_re = d0re;
+    a.dt0_im = d0im;
+    int rc;
+    CkPairArgs<R> a1 = a;
+    a1.u = (const C<R>*)c->psi;
+    a1.f0out = (C<R>*)c->f0;
+    a1.kb = (C<R>*)c->k2;
+    a1.unext = (C<R>*)c->uA;
+    if ((rc = run_launch<FusedCkTag1>(c, g, [&](Geom gg) {
+             return launch_ck_fused<R, ORD>(gg, w, k, a1, 1, c->partial, c->stream);
+         })))
+        return rc;
+    CkPairArgs<R> a2 = a;
+    a2.u = (const C<R>*)c->uA;
+    a2.ka = (C<R>*)c->k3;
+    a2.kb = (C<R>*)c->k4;
+    a2.unext = (C<R>*)c->uB;
+    if ((rc = run_launch<FusedCkTag2>(c, g, [&](Geom gg) {
+             return launch_ck_fused<R, ORD>(gg, w, k, a2, 2, c->partial, c->stream);
+         })))
+        return rc;
+    CkPairArgs<R> a3 = a;
+    a3.u = (const C<R>*)c->uB;
+    a3.out = (C<R>*)c->pnew;
+    if ((rc = run_launch<FusedCkTag3>(c, g, [&](Geom gg) {
+             return launch_ck_fused<R, ORD>(gg, w, k, a3, 3, c->partial, c->stream);
+         })))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = c->hres[0] / c->p.eps;
+    out[1] = c->hres[1];
+    c->f0_valid = true;
+    c->cand_valid = true;
+    return 0;
+}
+
 template <typename R> static int dispatch_fixed(gpe_ctx* c, int scheme, double dre, double dim) {
     return c->p.order == 2 ? step_fixed<R, 2>(c, scheme, dre, dim) : step_fixed<R, 4>(c, scheme, dre, dim);
 }
 template <typename R>
 static int dispatch_attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    if (fused_ck(c))
+        return c->p.order == 2 ? ck_attempt_fused<R, 2>(c, dre, dim, d0re, d0im, out)
+                               : ck_attempt_fused<R, 4>(c, dre, dim, d0re, d0im, out);
     return c->p.order == 2 ? ck_attempt<R, 2>(c, dre, dim, d0re, d0im, out)
                            : ck_attempt<R, 4>(c, dre, dim, d0re, d0im, out);
 }

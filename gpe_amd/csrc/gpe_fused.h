@@ -372,6 +372,227 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<R> w, RhsC<R> k, c
     rk4_f34_body<R, ORD, TERMS>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
 }
 
+// Fused Cash-Karp attempt (rkck, src/solve.f90:248-378, with the error
+// max-norm of rkqs, src/solve.f90:176-185): the six stages in three kernels
+// of two stages each, on the same overlapped tiles, register queues and LDS
+// ring as the RK4 pairs.  k_j = dt f(u_j), u_1 = psi0.
+//   pair 1: f0 = f(psi0), u2 = psi0 + b21 k1 on the whole block;
+//           k2 = dt f(u2), u3 = psi0 + b31 k1 + b32 k2 on the inner tile
+//                                          reads psi0                 writes f0, k2, u3
+//   pair 2: k3 = dt f(u3), u4 = psi0 + b41 k1 + b42 k2 + b43 k3;
+//           k4 = dt f(u4), u5 = psi0 + b51 k1 + .. + b54 k4
+//                                          reads u3, psi0, f0, k2     writes k3, k4, u5
+//   pair 3: k5 = dt f(u5), u6 = psi0 + b61 k1 + .. + b65 k5;
+//           k6 = dt f(u6), psi_new = psi0 + c1 k1 + c3 k3 + c4 k4 + c6 k6,
+//           err = dc1 k1 + dc3 k3 + dc4 k4 + dc5 k5 + dc6 k6 and the block
+//           maxima of |err| / |psi_scal| and |psi_new|^2
+//                                          reads u5, psi0, f0, k2, k3, k4   writes psi_new
+// 18 field accesses per attempt instead of 36.  Every sum adds its terms in
+// the order of the stage kernels (OpCkMid, OpCkLast).  Pairs 1 and 2 form
+// the part of the second stage's sum that does not depend on its own k at
+// the first stage and carry it in a register queue for the H planes in
+// between.  Pair 3 would need three such queues and spill, so it carries
+// only k5, in an LDS ring, and reads psi0, f0, k3 and k4 of its output cell
+// again at the second stage (two planes after the first read of the same
+// column, mostly from cache).
+template <typename R> struct CkPairArgs {
+    const C<R>* u;  // stencil input: psi0 (pair 1), u3 (pair 2), u5 (pair 3)
+    const C<R>* p0;
+    const C<R>* f0;
+    const C<R>* k2;
+    const C<R>* k3;
+    const C<R>* k4;
+    C<R>* f0out;  // pair 1
+    C<R>* ka;     // pair 2: k3
+    C<R>* kb;     // pair 1: k2, pair 2: k4
+    C<R>* unext;  // pair 1: u3, pair 2: u5
+    C<R>* out;    // pair 3: psi_new
+    C<R> dt;
+    CkTab<R> t;
+    double dt0_re, dt0_im;  // the step's dt on entry to rkqs (psi_scal)
+};
+
+template <typename R, int ORD, int TERMS, int PAIR>
+__device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, const RhsC<R>& k, const CkPairArgs<R>& a,
+                                             double* __restrict__ partial, int bid, int pidx) {
+    using S = FusedShape<ORD>;
+    constexpr int H = S::H, NB = H + 1, Q = 2 * H + 1;
+    constexpr bool P3 = PAIR == 3;
+    // FP64 pair 3 loads its pointwise inputs after the first stage's stencil:
+    // loaded at the top of the iteration they are live across the stencil and
+    // spill 12 registers (4.8 ms against 3.9 ms at 512^3).  In FP32 there is
+    // room, and the early loads hide their latency (2.2 ms against 2.6 ms).
+    constexpr bool kLate = sizeof(R) == 8;
+    __shared__ C<R> ldsA[2][S::LHA * S::LWA];         // stencil input plane p with halo
+    __shared__ C<R> ldsB[NB][S::NTB];                 // first-stage u_next, planes z..z+H
+    __shared__ C<R> ldsK[P3 ? NB : 1][P3 ? S::NTB : 1];  // pair 3: k5 of the own cell, planes z..z+H
+    __shared__ double red_sh[16];
+    const FusedCtx<ORD> c(g, bid);
+    double red0 = 0.0, red1 = 0.0;
+    if (c.z0 < c.z1) {  // uniform per block
+        const int start = c.z0 - 2 * H;
+        // Queues, slot (plane - start) % Q: the stencil input for planes
+        // z..z+2H and the first stage's u for planes z-H..z+H; qs (pairs 1, 2)
+        // the inner tile's partial sum of u_next for planes z..z+H.
+        C<R> qu[Q], qv[Q], qs[NB];
+#pragma unroll
+        for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
+#pragma unroll
+        for (int d = 0; d < NB; ++d) qs[d] = mkc<R>(0.0, 0.0);
+        if (c.active) {
+#pragma unroll
+            for (int d = 0; d < 2 * H; ++d) qu[d] = *(a.u + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
+        }
+        struct Pw {
+            C<R> p0, f0, k2, k3, k4;
+        };
+        auto load_pw = [&](Pw& v, long long pb) {
+            const Cell ci{pb, c.col};
+            if constexpr (PAIR >= 2) {
+                v.p0 = *(a.p0 + ci);
+                v.f0 = *(a.f0 + ci);
+                v.k2 = *(a.k2 + ci);
+            }
+            if constexpr (P3) {
+                v.k3 = *(a.k3 + ci);
+                v.k4 = *(a.k4 + ci);
+            }
+        };
+        // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H
+        // and (pair 2) the pointwise inputs of plane zz + H
+        struct Pf {
+            C<R> qn;
+            C<R> hr[S::NSLOT];
+            Pw pw;
+        };
+        auto fetch = [&](Pf& F, int zz) {
+            const long long pb = g.plane * bcmap(zz + H, g.nz, g.bcs);
+            if (c.active) {
+                F.qn = *(a.u + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
+                if constexpr (!P3) load_pw(F.pw, pb);
+            }
+#pragma unroll
+            for (int s = 0; s < S::NSLOT; ++s)
+                if (c.hv[s]) F.hr[s] = *(a.u + Cell{pb, c.hoff[s]});
+        };
+        Pf F{};
+        fetch(F, start);
+        const CkTab<R>& t = a.t;
+        auto step = [&](int z, auto phase) {
+            constexpr int PH = decltype(phase)::value;
+            const int p = z + H;
+            qu[(PH + 2 * H) % Q] = F.qn;
+            Pw pw = F.pw;
+            if constexpr (P3 && !kLate) {
+                // pair 3 loads its pointwise inputs at the top of the plane's
+                // iteration: a second set in flight would not fit
+                if (c.active) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+            }
+            C<R>* A = ldsA[(z - start) & 1];
+            if (c.active) A[c.la] = qu[(PH + H) % Q];
+#pragma unroll
+            for (int s = 0; s < S::NSLOT; ++s)
+                if (c.hv[s]) A[c.hl[s]] = F.hr[s];
+            __syncthreads();
+            if (z + 1 < c.z1) fetch(F, z + 1);
+            if (c.active) {
+                // first stage at plane p, whole block
+                FusedSten<R, ORD, S::LWA, TERMS> sa;
+                sa.L = A;
+                sa.lc = c.la;
+#pragma unroll
+                for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
+                sa.w = w;
+                const C<R> fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+                C<R> ub;
+                if constexpr (PAIR == 1) {
+                    const C<R> p0 = qu[(PH + H) % Q];
+                    const C<R> k1 = cmul<R>(a.dt, fa);
+                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.f0out + Cell{g.plane * p, c.own}, fa);
+                    ub = caxpy<R>(t.b21, k1, p0);
+                    qs[H] = caxpy<R>(t.b31, k1, p0);
+                } else if constexpr (PAIR == 2) {
+                    const C<R> k3 = cmul<R>(a.dt, fa);
+                    const C<R> k1 = cmul<R>(a.dt, pw.f0);
+                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.ka + Cell{g.plane * p, c.own}, k3);
+                    ub = caxpy<R>(t.b43, k3, caxpy<R>(t.b42, pw.k2, caxpy<R>(t.b41, k1, pw.p0)));
+                    qs[H] = caxpy<R>(t.b53, k3, caxpy<R>(t.b52, pw.k2, caxpy<R>(t.b51, k1, pw.p0)));
+                } else {
+                    if constexpr (kLate) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+                    const C<R> k5 = cmul<R>(a.dt, fa);
+                    const C<R> k1 = cmul<R>(a.dt, pw.f0);
+                    ub = caxpy<R>(t.b65, k5,
+                                  caxpy<R>(t.b64, pw.k4,
+                                           caxpy<R>(t.b63, pw.k3, caxpy<R>(t.b62, pw.k2, caxpy<R>(t.b61, k1, pw.p0)))));
+                    if (c.inner) ldsK[(p - start) % NB][c.lb] = k5;
+                }
+                qv[(PH + 2 * H) % Q] = ub;
+                ldsB[(p - start) % NB][c.lb] = ub;
+            }
+            if (z >= c.z0 && c.inner) {
+                // second stage at plane z, inner tile (ldsB of plane z was
+                // written H iterations ago)
+                FusedSten<R, ORD, S::TXB, TERMS> sb;
+                sb.L = ldsB[(z - start) % NB];
+                sb.lc = c.lb;
+#pragma unroll
+                for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
+                sb.w = w;
+                const C<R> kb = cmul<R>(a.dt, rhs_eval<R>(k, sb, c.mx, c.my, z));
+                const Cell oz{g.plane * z, c.own};
+                if constexpr (PAIR == 1) {
+                    if (c.store) {
+                        st_stream<R>(a.kb + oz, kb);
+                        st_stream<R>(a.unext + oz, caxpy<R>(t.b32, kb, qs[0]));
+                    }
+                } else if constexpr (PAIR == 2) {
+                    if (c.store) {
+                        st_stream<R>(a.kb + oz, kb);
+                        st_stream<R>(a.unext + oz, caxpy<R>(t.b54, kb, qs[0]));
+                    }
+                } else {
+                    if (c.store) {
+                        // OpCkLast at the own cell, k5 from the ring
+                        const C<R> p0 = *(a.p0 + oz), f0 = *(a.f0 + oz), k3 = *(a.k3 + oz), k4 = *(a.k4 + oz);
+                        const C<R> k5 = ldsK[(z - start) % NB][c.lb];
+                        const C<R> k1 = cmul<R>(a.dt, f0);
+                        const C<R> nw = caxpy<R>(t.c6, kb, caxpy<R>(t.c4, k4, caxpy<R>(t.c3, k3, caxpy<R>(t.c1, k1, p0))));
+                        st_stream<R>(a.out + oz, nw);
+                        const double er = t.dc1 * (double)k1.x + t.dc3 * (double)k3.x + t.dc4 * (double)k4.x +
+                                          t.dc5 * (double)k5.x + t.dc6 * (double)kb.x;
+                        const double ei = t.dc1 * (double)k1.y + t.dc3 * (double)k3.y + t.dc4 * (double)k4.y +
+                                          t.dc5 * (double)k5.y + t.dc6 * (double)kb.y;
+                        const double ap = sqrt((double)p0.x * p0.x + (double)p0.y * p0.y);
+                        const double af = sqrt((double)f0.x * f0.x + (double)f0.y * f0.y);
+                        const double sr = ap + a.dt0_re * af + 1e-30, si = a.dt0_im * af;
+                        const double e = sqrt((er * er + ei * ei) / (sr * sr + si * si));
+                        red0 = e > red0 ? e : red0;
+                        const double a2 = (double)nw.x * nw.x + (double)nw.y * nw.y;
+                        red1 = a2 > red1 ? a2 : red1;
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < H; ++d) qs[d] = qs[d + 1];
+        };
+        march_rotating<Q>(start, c.z1, step);
+    }
+    if constexpr (P3) {
+        const double v0 = block_reduce<1024, RED_MAX>(red0, red_sh);
+        const double v1 = block_reduce<1024, RED_MAX>(red1, red_sh);
+        if (threadIdx.x == 0) {
+            partial[pidx] = v0;
+            partial[(long long)g.nblocks + pidx] = v1;
+        }
+    }
+}
+
+template <typename R, int ORD, int TERMS, int PAIR>
+__global__ __launch_bounds__(1024) void k_ck_pair(Geom g, StW<R> w, RhsC<R> k, CkPairArgs<R> a,
+                                                  double* __restrict__ partial) {
+    ck_pair_body<R, ORD, TERMS, PAIR>(g, w, k, a, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
+}
+
 // Batched form for an ensemble of grids: persistent blocks walk a work list
 // of (member, fused block) items, members in descriptor order.  A member's
 // state alternates between psi and alt from step to step (flip = 1: the
@@ -425,5 +646,8 @@ hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, co
 template <int ORD>
 hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
                                     int flip, hipStream_t s);
+template <typename R, int ORD>
+hipError_t launch_ck_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const CkPairArgs<R>& a, int pair,
+                           double* partial, hipStream_t s);  // g.general: the trap term
 
 }  // namespace gpe

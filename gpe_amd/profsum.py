@@ -39,7 +39,9 @@ ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3,
 MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
                "mx_rkck_stage1": 24, "mx_rkck_stage2": 40, "mx_rkck_stage3": 56, "mx_rkck_stage4": 64,
                "mx_rkck_stage5": 72, "mx_rkck_stage6_error": 72}
-FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3}  # field accesses per cell
+FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3,  # field accesses per cell
+                "rkck_fused12": 4, "rkck_fused34": 7, "rkck_fused56_error": 7}
+CK_PAIR = {"1": "rkck_fused12", "2": "rkck_fused34", "3": "rkck_fused56_error"}
 # kernels whose calibrated traffic bench.py reads back (traffic_<kind>_<prec>_<n>.json)
 TRAFFIC_KINDS = {"rk4_fused12", "rk4_fused34", "rk4_stage3", "mx_rk4_stage4"}
 
@@ -78,6 +80,12 @@ def kernel_label(name: str) -> str:
         h = int(mf.group(3)) // 2
         trap = " with-trap" if mf.group(4) != "0" else ""
         return f"rk4_fused{mf.group(1)} {prec} order{mf.group(3)} tile{64 + 2 * h}x15{trap}"
+    mc = re.search(r"k_ck_pair<(double|float), (\d), (\d+), (\d)>", name)
+    if mc:
+        prec = "c128" if mc.group(1) == "double" else "c64"
+        h = int(mc.group(2)) // 2
+        trap = " with-trap" if mc.group(3) != "0" else ""
+        return f"{CK_PAIR[mc.group(4)]} {prec} order{mc.group(2)} tile{64 + 2 * h}x15{trap}"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
@@ -116,8 +124,8 @@ def main(argv=None):
         run_pmc = f"python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}"
     else:
         title = f"# RKF45 (Cash-Karp) stage kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
-        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128,mixed,c64"
-        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128,mixed,c64 "
+        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed,c64"
+        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128/stages,c128,mixed,c64 "
                    f"--copy-probe-mib {a.copy_mib}")
     lines = [title, "",
              "Commands (run from the repository root; TMPDIR=/tmp):", "",

@@ -6,7 +6,8 @@
              (gpe_hamiltonian) for each, and the c64 field's distance from the
              FP64 (c128) run.
   time_to_t  RKF45 to a fixed final time on the random-phase workload:
-             an FP64 reference at a tight tolerance, then plain c128, c64 and
+             an FP64 reference at a tight tolerance, then plain c128 (six
+             stage kernels per attempt, and fused: three kernels), c64 and
              mixed storage (c128 state, c64 increments) at 1e-6, and the
              learned controller on mixed storage when a model is given;
              wall time, steps, rejections, relative L2 error at T and the
@@ -22,7 +23,7 @@
 Each prints one JSON object per line.  Usage:
     python -m gpe_amd.studies drift [--n 256] [--steps 2000]
     python -m gpe_amd.studies time_to_t [--n 512] [--T 20]
-    python -m gpe_amd.studies attempts [--n 512] [--attempts 6] [--storages c128,mixed]
+    python -m gpe_amd.studies attempts [--n 512] [--attempts 6] [--storages c128/stages,c128,mixed]
     python -m gpe_amd.studies diag [--n 512] [--reps 5]
 """
 from __future__ import annotations
@@ -82,19 +83,31 @@ def drift(n: int, steps: int, every: int):
               / abs(s128["energy0"] * (1 + s128["energy_drift"]))))
 
 
-def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=None):
+def variant(spec: str):
+    """'c128', 'c64', 'mixed': that storage with the fused kernels where they
+    apply (the library default); a '/stages' suffix forces the stage kernels."""
+    storage, _, how = spec.partition("/")
+    if how not in ("", "stages"):
+        raise ValueError(f"unknown variant {spec!r}")
+    return storage, how != "stages"
+
+
+def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=None, fused=True):
     pp = p.replace(storage=storage, eps=eps)
     ctrl = controller or PlainController()
     with Context(pp) as ctx:
+        ctx.set_fused(fused)
         ctx.upload(psi)
         st = integrate(ctx, T, dt0, ctrl, clamp=True, energy=energy)
         final = ctx.download()
-    st.method = name or f"{ctrl.name}-{storage}-eps{eps:g}"
+    st.method = name or f"{ctrl.name}-{storage}{'' if fused else '/stages'}-eps{eps:g}"
     return st, final
 
 
 def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None,
-              storages=("c128", "c64", "mixed")):
+              storages=("c128/stages", "c128", "c64", "mixed")):
+    """The baseline of the speed-ups is plain c128 with the six stage kernels
+    per attempt ('c128/stages') when it is among the variants."""
     p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=dt0)
     ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
     emit(dict(study="time_to_t", n=n, T=T, method=ref.method, wall_s=ref.wall_s, steps=ref.n_accept,
@@ -107,16 +120,18 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
         if learned_model is not None:
             from .learned import LearnedController
             runs.append(("mixed", eps, LearnedController(learned_model, p.replace(eps=eps)), True))
-    for storage, eps, ctrl, energy in runs:
-        st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy)
+    for spec, eps, ctrl, energy in runs:
+        storage, fused = variant(spec)
+        st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy, fused=fused)
         err = rel_l2(fin.astype(np.complex128), ref_final)
-        row = dict(study="time_to_t", n=n, T=T, method=st.method, storage=storage, eps=eps, wall_s=st.wall_s,
+        row = dict(study="time_to_t", n=n, T=T, method=st.method, storage=spec, eps=eps, wall_s=st.wall_s,
                    steps=st.n_accept, rejects=st.n_reject, rhs=st.n_rhs, rel_l2_vs_oracle=err)
         rows.append(row)
         emit(row)
     for eps in eps_list:
-        base = next((r for r in rows if r["eps"] == eps and r["storage"] == "c128" and r["method"].startswith("plain")),
-                    None)
+        plain = [r for r in rows if r["eps"] == eps and r["method"].startswith("plain")]
+        base = next((r for r in plain if r["storage"] == "c128/stages"), None) or \
+            next((r for r in plain if r["storage"] == "c128"), None)
         if base is None:
             continue
         emit(dict(study="time_to_t_speedup", n=n, T=T, eps=eps, baseline=base["method"],
@@ -137,8 +152,10 @@ def attempts(n: int, count: int, storages, copy_mib: int = 0):
     if copy_mib:
         copy_probe(Hip(), copy_mib << 20)
     p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=0.01)
-    for storage in storages:
+    for spec in storages:
+        storage, fused = variant(spec)
         with Context(p.replace(storage=storage, eps=1e-6)) as ctx:
+            ctx.set_fused(fused)
             ctx.upload(psi)
             info = StepInfo(p.dt0())
             ctx.step("rk45", 2, info)
@@ -149,7 +166,7 @@ def attempts(n: int, count: int, storages, copy_mib: int = 0):
             na = info.n_accept + info.n_reject - a0
             ker = {k: dict(avg_ms=v["avg_ms"], launches=v["launches"],
                            GB_s=v["bytes"] / (v["avg_ms"] * 1e-3) / 1e9) for k, v in prof.items()}
-            emit(dict(study="attempts", n=n, storage=storage, steps=count, attempts=na,
+            emit(dict(study="attempts", n=n, storage=spec, steps=count, attempts=na,
                       ms_per_attempt=sum(v["avg_ms"] * v["launches"] for v in prof.values()) / max(na, 1),
                       wall_ms_per_attempt=1e3 * wall / max(na, 1), kernels=ker))
 
@@ -197,7 +214,9 @@ def main(argv=None):
     ap.add_argument("--oracle-eps", type=float, default=1e-10)
     ap.add_argument("--eps", default="1e-6")
     ap.add_argument("--model", default=None, help="learned controller weights (.npz)")
-    ap.add_argument("--storages", default="c128,c64,mixed")
+    ap.add_argument("--storages", default=None,
+                    help="comma-separated variants: c128, c64, mixed, each optionally /stages "
+                         "(default: time_to_t c128/stages,c128,c64,mixed; attempts c128/stages,c128,mixed,c64)")
     ap.add_argument("--attempts", type=int, default=6)
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -206,14 +225,15 @@ def main(argv=None):
     elif a.study == "diag":
         diag(a.n or 512, a.reps)
     elif a.study == "attempts":
-        attempts(a.n or 512, a.attempts, tuple(a.storages.split(",")), a.copy_probe_mib)
+        attempts(a.n or 512, a.attempts, tuple((a.storages or "c128/stages,c128,mixed,c64").split(",")),
+                 a.copy_probe_mib)
     else:
         model = None
         if a.model:
             from .learned import load_model
             model = load_model(a.model)
         time_to_t(a.n or 512, a.T, a.dt0, a.oracle_eps, [float(v) for v in a.eps.split(",")], model,
-                  tuple(a.storages.split(",")))
+                  tuple((a.storages or "c128/stages,c128,c64,mixed").split(",")))
     return 0
 
 

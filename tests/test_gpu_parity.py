@@ -126,11 +126,15 @@ def test_ring64_rk4_200_steps(orc):
     assert abs(info.t - t[0]) < 1e-12
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
-def test_rkck_attempt_and_rkqs_match_oracle(orc, order, bcs):
+def test_rkck_attempt_and_rkqs_match_oracle(orc, order, bcs, fused):
+    """One Cash-Karp attempt and four rkqs steps from a too-large dt, fused
+    (three kernels) and as six stage kernels, against the oracle."""
     p = RunParams(nx=24, ny=22, nz=20, xr=8.0, yr=8.0, zr=8.0, order=order, bcs=bcs, eqn_to_solve=1)
     psi = smooth_field(p, seed=4)
     with Context(p) as ctx:
+        ctx.set_fused(fused)
         ctx.upload(psi)
         # single attempt: errmax and the candidate
         new_o, err_o = orc.rkck(p, psi, 0.05 + 0j)
@@ -282,6 +286,92 @@ def test_fused_rk4_matches_oracle_and_stage_kernels(orc, shape, order, bcs, eqn)
             assert abs(info.t - t[0]) < 1e-15
     assert rel(outs[True], exp) < 1e-12
     assert rel(outs[True], outs[False]) < 1e-13
+
+
+CK_CASES = [((24, 20, 18), 2, 1, 1), ((37, 21, 19), 4, 2, 1), ((70, 33, 12), 4, 1, 2), ((6, 6, 6), 2, 2, 2),
+            ((31, 17, 23), 4, 2, 4), ((40, 35, 9), 2, 1, 4)]
+
+
+@pytest.mark.parametrize("shape,order,bcs,eqn", CK_CASES, ids=[f"{s[0]}x{s[1]}x{s[2]}-o{o}-b{b}-e{e}"
+                                                                for s, o, b, e in CK_CASES])
+def test_fused_rkck_attempt_matches_oracle_and_stage_kernels(orc, shape, order, bcs, eqn):
+    """Fused Cash-Karp attempt (three kernels of two stages, overlapped
+    tiles, unfolded boundary cells; eqn 4 with the trap) against the FP64
+    oracle's rkck (candidate 1e-12, error max-norm 1e-9) and the six stage
+    kernels (candidate 1e-13, error max-norm 1e-10).  A second attempt at a
+    smaller dt (a retry) re-evaluates f(psi0) and must agree too."""
+    nx, ny, nz = shape
+    kw = dict(g=1.0, mu=3.0, omx=0.2, omy=0.25, omz=0.3) if eqn == 4 else {}
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=0.35 * nx, yr=0.35 * ny, zr=0.35 * nz, order=order, bcs=bcs,
+                  eqn_to_solve=eqn, **kw)
+    psi = smooth_field(p, seed=11)
+    dt0 = 0.1 + 0j
+    res = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            em1, a21 = ctx.rk45_attempt(dt0, dt0)
+            em2, a22 = ctx.rk45_attempt(0.5 * dt0, dt0)  # retry at a smaller dt
+            ctx.rk45_accept()
+            res[fused] = (em1, a21, em2, a22, ctx.download())
+    print(f"errmax: first attempt {res[True][0]:.4e}, retry {res[True][2]:.4e}")
+    new_o, err_o = orc.rkck(p, psi, 0.5 * dt0)
+    f0 = orc.rhs(p, psi)
+    scal = np.abs(psi) + dt0 * np.abs(f0) + 1e-30
+    em_o = np.max(np.abs(err_o) / np.abs(scal)) / p.eps
+    ef, es = res[True], res[False]
+    # the error norm is a cancellation of O(dt f) increments: besides the
+    # relative bound, allow 1e-9 in errmax units (1e-15 of |psi_scal|)
+    assert rel(ef[4], new_o) < 1e-12
+    assert abs(ef[2] - em_o) <= 1e-9 * em_o + 1e-9
+    assert rel(ef[4], es[4]) < 1e-13
+    for i in (0, 2):
+        assert abs(ef[i] - es[i]) <= 1e-9 * es[i] + 1e-9, (i, ef[i], es[i])
+    for i in (1, 3):
+        assert abs(ef[i] - es[i]) <= 1e-13 * es[i]
+
+
+def test_fused_rkck_rkqs_run_and_complex64():
+    """rkqs with the fused attempt reproduces the stage kernels' step
+    sequence (accepts, rejects, dt) over 12 steps from a too-large dt; with
+    complex64 fields the fused and stage attempts agree to FP32 accuracy.
+    The tolerance eps = 1e-8 keeps dt (~0.03) inside the Cash-Karp stability
+    limit of the grid's highest modes (~0.065 at dx 0.38): at 1e-6 this
+    smooth field lets the controller take dt ~0.09, where those modes grow
+    ~6x per step from rounding level and any two correct implementations
+    part.  The error estimate is ~eps/|dt f| ~1e-7 of the increments it is
+    formed from, so its rounding (~1e-9 relative) moves each dt by ~2e-10
+    (dt ~ errmax^-0.2): t, dt and the field are compared at 1e-8."""
+    p = RunParams(nx=48, ny=40, nz=36, xr=9.0, yr=8.0, zr=7.0, order=4, bcs=1, eqn_to_solve=1, scheme="rk45",
+                  eps=1e-8)
+    psi = smooth_field(p, seed=12)
+    runs = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            info = StepInfo(0.2 + 0j)
+            ctx.step("rk45", 12, info)
+            runs[fused] = (info.n_accept, info.n_reject, info.n_rhs, info.t, info.dt, ctx.download())
+    f, s = runs[True], runs[False]
+    print(f"rkqs: {f[0]} accepted, {f[1]} rejected, t = {f[3]:.12f} / {s[3]:.12f}, field rel {rel(f[5], s[5]):.2e}")
+    assert f[:3] == s[:3] and s[1] > 0
+    assert abs(f[3] - s[3]) <= 1e-8 * s[3] and abs(f[4] - s[4]) <= 1e-8 * abs(s[4])
+    assert rel(f[5], s[5]) < 1e-8
+    out = {}
+    for fused in (True, False):
+        with Context(p.replace(storage="c64")) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            em, a2 = ctx.rk45_attempt(0.05 + 0j, 0.05 + 0j)
+            ctx.rk45_accept()
+            out[fused] = (em, a2, ctx.download())
+    print("complex64 errmax fused/stage:", out[True][0], out[False][0])
+    assert rel(out[True][2], out[False][2]) < 1e-6
+    # the error norm is a difference of FP32 increments: its rounding noise is ~1e-3 of it here
+    assert abs(out[True][0] - out[False][0]) <= 5e-2 * out[False][0]
+    assert abs(out[True][1] - out[False][1]) <= 1e-6 * out[False][1]
 
 
 def test_fused_rk4_imaginary_time_and_fallback(orc):
