@@ -123,9 +123,9 @@ def main(argv=None):
         run_kt = "python3 bench.py --steps 20 --warmup 5 --cpu-steps 0"
         run_pmc = f"python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}"
     else:
-        title = f"# RKF45 (Cash-Karp) stage kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
-        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed,c64"
-        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128/stages,c128,mixed,c64 "
+        title = f"# RKF45 (Cash-Karp) stage and fused kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
+        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed,c64/stages,c64"
+        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128/stages,c128,mixed,c64/stages,c64 "
                    f"--copy-probe-mib {a.copy_mib}")
     lines = [title, "",
              "Commands (run from the repository root; TMPDIR=/tmp):", "",
@@ -145,7 +145,7 @@ def main(argv=None):
     traffic = {}
     for r in stats:
         name = r["Name"]
-        if "k_stencil" not in name and "k_rk4_f" not in name:
+        if "k_stencil" not in name and "k_rk4_f" not in name and "k_ck_pair" not in name:
             continue
         label = kernel_label(name)
         if len(label.split()) < 2:  # not a kernel of a known form (e.g. the batched ensemble's)
