@@ -87,7 +87,8 @@ class Context:
         self._chk(self.lib.gpe_set_tiles(self.h, tx, ty, tz), "gpe_set_tiles")
 
     def set_fused(self, on: bool) -> None:
-        """Fused RK4 (two kernels per step) where it applies; on by default."""
+        """Fused stages where they apply (RK4: two kernels per step; RKF45:
+        three kernels per Cash-Karp attempt); on by default."""
         self._chk(self.lib.gpe_set_fused(self.h, int(bool(on))), "gpe_set_fused")
 
     def tiles(self):

@@ -114,12 +114,15 @@ void gpe_ctx_destroy(gpe_ctx* ctx);
 int gpe_set_tiles(gpe_ctx* ctx, int tx, int ty, int tz);
 int gpe_get_tiles(const gpe_ctx* ctx, int* tx, int* ty, int* tz);
 
-/* Fused RK4 (on by default): for complex128 storage and a right-hand side
- * without optional terms (no V, Urhs, dissipation or eqn-3 mask), gpe_step
- * with GPE_RK4 runs the four stages as two kernels that each evaluate two
- * stages (5 field accesses per cell instead of 13).  on = 0 selects the
- * four stage kernels.  No counterpart in the reference: a performance switch
- * of the same rk4 (src/solve.f90:109-154). */
+/* Fused stages (on by default) for complex128 or complex64 storage and a
+ * right-hand side whose only optional term is the trap V (no Urhs,
+ * dissipation or eqn-3 mask): gpe_step with GPE_RK4 runs the four stages as
+ * two kernels that each evaluate two stages (5 field accesses per cell
+ * instead of 13), and each Cash-Karp attempt of GPE_RK45 (and
+ * gpe_rk45_attempt) runs its six stages as three such kernels (18 accesses
+ * instead of 36).  on = 0 selects the stage-by-stage kernels.  No
+ * counterpart in the reference: a performance switch of the same rk4 and
+ * rkck (src/solve.f90:109-154, 248-378). */
 int gpe_set_fused(gpe_ctx* ctx, int on);
 
 /* Copy a host field into the device state.  The host array has extent
