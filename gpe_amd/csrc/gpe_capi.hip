@@ -14,7 +14,7 @@
 #include <hipfft/hipfft.h>
 
 #include "../../include/gpe_hip.h"
-#include "gpe_fused.h"
+#include "gpe_fused_mx.h"
 #include "gpe_host.h"
 
 using namespace gpe;
@@ -95,7 +95,8 @@ static const char* const kKindNames[GPE_NKINDS] = {
     "rkck_stage2", "rkck_stage3", "rkck_stage4", "rkck_stage5", "rkck_stage6_error", "deriv", "get_rhs", "energy",
     "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
     "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error",
-    "rk4_fused12", "rk4_fused34", "momentum", "velocity", "rkck_fused12", "rkck_fused34", "rkck_fused56_error"};
+    "rk4_fused12", "rk4_fused34", "momentum", "velocity", "rkck_fused12", "rkck_fused34", "rkck_fused56_error",
+    "mx_rkck_fused12", "mx_rkck_fused34", "mx_rkck_fused56_error"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -324,10 +325,11 @@ static bool fused_rk4(const gpe_ctx* c, int scheme) {
            (f == 0 || f == RHS_V);
 }
 
-// The fused Cash-Karp attempt has the same conditions.
+// The fused Cash-Karp attempt has the same conditions, and also applies to
+// mixed storage.
 static bool fused_ck(const gpe_ctx* c) {
     const int f = make_rhs_p<double>(c->p, nullptr).flags;
-    return c->fused && (c->p.storage == GPE_C128 || c->p.storage == GPE_C64) && (f == 0 || f == RHS_V);
+    return c->fused && (f == 0 || f == RHS_V);
 }
 
 static int need_rk_buffers(gpe_ctx* c, int scheme) {
@@ -495,6 +497,71 @@ struct FusedCkTag2 {
 struct FusedCkTag3 {
     static constexpr int kKind = 31, kAccess = 7, kNRed = 2, kMode = RED_MAX;
 };
+struct FusedMxCkTag1 {
+    static constexpr int kKind = 32, kBytesPerCell = 40, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedMxCkTag2 {
+    static constexpr int kKind = 33, kBytesPerCell = 64, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedMxCkTag3 {
+    static constexpr int kKind = 34, kBytesPerCell = 72, kNRed = 2, kMode = RED_MAX;
+};
+
+// One mixed-storage Cash-Karp attempt as three fused kernels
+// (gpe_fused_mx.h): psi -> F0, K2, D3 (DA); psi + D3 -> K3, K4, D5 (DB);
+// psi + D5 -> psi_new (pnew) with the error max-norm, as ck_attempt_mixed.
+template <int ORD>
+static int ck_attempt_mixed_fused(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
+    const RhsC<double> k = make_rhs<double>(c);
+    const StW<double> w = make_w<double>(c->p);
+    Geom g = make_geom<double>(c);
+    fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
+    CkMxArgs a{};
+    a.p0 = (const CD*)c->psi;
+    a.f0 = (const CF*)c->mx[0];
+    a.k2 = (const CF*)c->mx[1];
+    a.k3 = (const CF*)c->mx[2];
+    a.k4 = (const CF*)c->mx[3];
+    a.dt = cdt<double>(dre, dim, 1.0);
+    a.t = make_tab<double>();
+    a.s21 = mkc<double>(a.t.b21 * a.dt.x, a.t.b21 * a.dt.y);
+    a.dt0_re = d0re;
+    a.dt0_im = d0im;
+    CF* DA = (CF*)c->mx[5];
+    CF* DB = (CF*)c->mx[6];
+    int rc;
+    CkMxArgs a1 = a;
+    a1.f0out = (CF*)c->mx[0];
+    a1.kb = (CF*)c->mx[1];
+    a1.dnext = DA;
+    if ((rc = run_launch<FusedMxCkTag1>(c, g, [&](Geom gg) {
+             return launch_ckmx_fused<ORD>(gg, w, k, a1, 1, c->partial, c->stream);
+         })))
+        return rc;
+    CkMxArgs a2 = a;
+    a2.d = DA;
+    a2.ka = (CF*)c->mx[2];
+    a2.kb = (CF*)c->mx[3];
+    a2.dnext = DB;
+    if ((rc = run_launch<FusedMxCkTag2>(c, g, [&](Geom gg) {
+             return launch_ckmx_fused<ORD>(gg, w, k, a2, 2, c->partial, c->stream);
+         })))
+        return rc;
+    CkMxArgs a3 = a;
+    a3.d = DB;
+    a3.out = (CD*)c->pnew;
+    if ((rc = run_launch<FusedMxCkTag3>(c, g, [&](Geom gg) {
+             return launch_ckmx_fused<ORD>(gg, w, k, a3, 3, c->partial, c->stream);
+         })))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out[0] = c->hres[0] / c->p.eps;
+    out[1] = c->hres[1];
+    c->f0_valid = true;
+    c->cand_valid = true;
+    return 0;
+}
 
 // One Cash-Karp attempt as three fused kernels (gpe_fused.h): psi -> f0, k2,
 // u3 (uA); u3 -> k3, k4, u5 (uB); u5 -> psi_new (pnew) with the error
@@ -567,9 +634,13 @@ static int dispatch_attempt(gpe_ctx* c, double dre, double dim, double d0re, dou
 static int attempt(gpe_ctx* c, double dre, double dim, double d0re, double d0im, double out[2]) {
     int rc = need_rk_buffers(c, GPE_RK45);
     if (rc) return rc;
-    if (c->p.storage == GPE_MIXED)
+    if (c->p.storage == GPE_MIXED) {
+        if (fused_ck(c))
+            return c->p.order == 2 ? ck_attempt_mixed_fused<2>(c, dre, dim, d0re, d0im, out)
+                                   : ck_attempt_mixed_fused<4>(c, dre, dim, d0re, d0im, out);
         return c->p.order == 2 ? ck_attempt_mixed<2>(c, dre, dim, d0re, d0im, out)
                                : ck_attempt_mixed<4>(c, dre, dim, d0re, d0im, out);
+    }
     return c->p.storage == GPE_C128 ? dispatch_attempt<double>(c, dre, dim, d0re, d0im, out)
                                     : dispatch_attempt<float>(c, dre, dim, d0re, d0im, out);
 }

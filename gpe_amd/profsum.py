@@ -42,6 +42,7 @@ MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
 FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3,  # field accesses per cell
                 "rkck_fused12": 4, "rkck_fused34": 7, "rkck_fused56_error": 7}
 CK_PAIR = {"1": "rkck_fused12", "2": "rkck_fused34", "3": "rkck_fused56_error"}
+MIXED_BYTES.update({"mx_rkck_fused12": 40, "mx_rkck_fused34": 64, "mx_rkck_fused56_error": 72})
 # kernels whose calibrated traffic bench.py reads back (traffic_<kind>_<prec>_<n>.json)
 TRAFFIC_KINDS = {"rk4_fused12", "rk4_fused34", "rk4_stage3", "mx_rk4_stage4"}
 
@@ -86,6 +87,11 @@ def kernel_label(name: str) -> str:
         h = int(mc.group(2)) // 2
         trap = " with-trap" if mc.group(3) != "0" else ""
         return f"{CK_PAIR[mc.group(4)]} {prec} order{mc.group(2)} tile{64 + 2 * h}x15{trap}"
+    mm = re.search(r"k_ckmx_pair<(\d), (\d+), (\d)>", name)
+    if mm:
+        h = int(mm.group(1)) // 2
+        trap = " with-trap" if mm.group(2) != "0" else ""
+        return f"mx_{CK_PAIR[mm.group(3)]} mixed order{mm.group(1)} tile{64 + 2 * h}x15{trap}"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
@@ -124,8 +130,8 @@ def main(argv=None):
         run_pmc = f"python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --copy-probe-mib {a.copy_mib}"
     else:
         title = f"# RKF45 (Cash-Karp) stage and fused kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
-        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed,c64/stages,c64"
-        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128/stages,c128,mixed,c64/stages,c64 "
+        run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed/stages,mixed,c64/stages,c64"
+        run_pmc = (f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 1 --storages c128/stages,c128,mixed/stages,mixed,c64/stages,c64 "
                    f"--copy-probe-mib {a.copy_mib}")
     lines = [title, "",
              "Commands (run from the repository root; TMPDIR=/tmp):", "",
@@ -145,7 +151,7 @@ def main(argv=None):
     traffic = {}
     for r in stats:
         name = r["Name"]
-        if "k_stencil" not in name and "k_rk4_f" not in name and "k_ck_pair" not in name:
+        if not any(f in name for f in ("k_stencil", "k_rk4_f", "k_ck_pair", "k_ckmx_pair")):
             continue
         label = kernel_label(name)
         if len(label.split()) < 2:  # not a kernel of a known form (e.g. the batched ensemble's)

@@ -105,7 +105,7 @@ def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=
 
 
 def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None,
-              storages=("c128/stages", "c128", "c64", "mixed")):
+              storages=("c128/stages", "c128", "c64", "mixed/stages", "mixed")):
     """The baseline of the speed-ups is plain c128 with the six stage kernels
     per attempt ('c128/stages') when it is among the variants."""
     p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=dt0)
@@ -216,7 +216,8 @@ def main(argv=None):
     ap.add_argument("--model", default=None, help="learned controller weights (.npz)")
     ap.add_argument("--storages", default=None,
                     help="comma-separated variants: c128, c64, mixed, each optionally /stages "
-                         "(default: time_to_t c128/stages,c128,c64,mixed; attempts c128/stages,c128,mixed,c64)")
+                         "(default: time_to_t c128/stages,c128,c64,mixed/stages,mixed; "
+                         "attempts c128/stages,c128,mixed/stages,mixed,c64)")
     ap.add_argument("--attempts", type=int, default=6)
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -225,7 +226,7 @@ def main(argv=None):
     elif a.study == "diag":
         diag(a.n or 512, a.reps)
     elif a.study == "attempts":
-        attempts(a.n or 512, a.attempts, tuple((a.storages or "c128/stages,c128,mixed,c64").split(",")),
+        attempts(a.n or 512, a.attempts, tuple((a.storages or "c128/stages,c128,mixed/stages,mixed,c64").split(",")),
                  a.copy_probe_mib)
     else:
         model = None
@@ -233,7 +234,7 @@ def main(argv=None):
             from .learned import load_model
             model = load_model(a.model)
         time_to_t(a.n or 512, a.T, a.dt0, a.oracle_eps, [float(v) for v in a.eps.split(",")], model,
-                  tuple((a.storages or "c128/stages,c128,c64,mixed").split(",")))
+                  tuple((a.storages or "c128/stages,c128,c64,mixed/stages,mixed").split(",")))
     return 0
 
 

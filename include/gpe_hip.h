@@ -247,7 +247,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 32
+#define GPE_NKINDS 35
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

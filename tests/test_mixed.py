@@ -55,8 +55,11 @@ def test_mixed_ring64_rk4_far_below_c64(orc):
     assert errs["mixed"] < errs["c64"] / 30
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
-def test_mixed_rkck_attempt_matches_oracle(orc, order, bcs):
+def test_mixed_rkck_attempt_matches_oracle(orc, order, bcs, fused):
+    """Mixed-storage Cash-Karp attempt, fused (three kernels) and as six
+    stage kernels, against the oracle; a retry equals a fresh attempt."""
     p = RunParams(nx=24, ny=22, nz=20, xr=8.0, yr=8.0, zr=8.0, order=order, bcs=bcs, eqn_to_solve=1,
                   storage="mixed")
     psi = smooth_field(p, seed=4)
@@ -65,6 +68,7 @@ def test_mixed_rkck_attempt_matches_oracle(orc, order, bcs):
     scal = np.abs(psi) + 0.05 * np.abs(f0) + 1e-30
     em_o = np.max(np.abs(err_o) / np.abs(scal)) / 1e-6
     with Context(p) as ctx:
+        ctx.set_fused(fused)
         ctx.upload(psi)
         em, a2 = ctx.rk45_attempt(0.05 + 0j, 0.05 + 0j)
         assert abs(em - em_o) <= 1e-2 * em_o
@@ -85,7 +89,8 @@ def test_mixed_rkck_attempt_matches_oracle(orc, order, bcs):
         assert np.array_equal(got_retry, ctx.download())
 
 
-def test_mixed_rkqs_steps_match_oracle(orc):
+@pytest.mark.parametrize("fused", [True, False])
+def test_mixed_rkqs_steps_match_oracle(orc, fused):
     p = RunParams(nx=24, ny=22, nz=20, xr=8.0, yr=8.0, zr=8.0, order=4, bcs=1, eqn_to_solve=1, storage="mixed")
     psi = smooth_field(p, seed=5)
     cur, dt, t = psi, 0.4 + 0j, (0.0, 0.0)
@@ -94,6 +99,7 @@ def test_mixed_rkqs_steps_match_oracle(orc):
         cur, dt, t, _, nrej, _ = orc.rkqs(p, cur, dt, t)
         nrej_o += nrej
     with Context(p) as ctx:
+        ctx.set_fused(fused)
         ctx.upload(psi)
         info = StepInfo(0.4 + 0j)
         ctx.step("rk45", 4, info)
@@ -102,6 +108,42 @@ def test_mixed_rkqs_steps_match_oracle(orc):
     assert abs(info.dt - dt) <= 1e-3 * abs(dt)
     assert abs(info.t - t[0]) <= 1e-3 * t[0]
     assert rel(got, cur) < 1e-5
+
+
+MX_CASES = [((24, 20, 18), 2, 1, 1), ((37, 21, 19), 4, 2, 1), ((70, 33, 12), 4, 1, 2), ((31, 17, 23), 4, 2, 4),
+            ((6, 6, 6), 2, 2, 4)]
+
+
+@pytest.mark.parametrize("shape,order,bcs,eqn", MX_CASES, ids=[f"{s[0]}x{s[1]}x{s[2]}-o{o}-b{b}-e{e}"
+                                                                for s, o, b, e in MX_CASES])
+def test_mixed_fused_attempt_matches_stage_kernels(shape, order, bcs, eqn):
+    """The fused mixed-storage attempt rounds its complex64 increments at the
+    same points as the six mixed stage kernels: candidate within 1e-9
+    relative L2 (an FP64 contraction difference can move one FP32 rounding
+    by an ulp, ~6e-8 of an O(dt f) increment at that cell), the error
+    max-norm within 1e-3, for a first attempt and a retry."""
+    nx, ny, nz = shape
+    kw = dict(g=1.0, mu=3.0, omx=0.2, omy=0.25, omz=0.3) if eqn == 4 else {}
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=0.35 * nx, yr=0.35 * ny, zr=0.35 * nz, order=order, bcs=bcs,
+                  eqn_to_solve=eqn, storage="mixed", **kw)
+    psi = smooth_field(p, seed=13)
+    dt0 = 0.1 + 0j
+    res = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            em1, a21 = ctx.rk45_attempt(dt0, dt0)
+            em2, a22 = ctx.rk45_attempt(0.5 * dt0, dt0)
+            ctx.rk45_accept()
+            res[fused] = (em1, a21, em2, a22, ctx.download())
+    f, s = res[True], res[False]
+    print(f"mixed fused/stage errmax {f[0]:.6e}/{s[0]:.6e}, {f[2]:.6e}/{s[2]:.6e}; cand rel {rel(f[4], s[4]):.2e}")
+    assert rel(f[4], s[4]) < 1e-9
+    for i in (0, 2):
+        assert abs(f[i] - s[i]) <= 1e-3 * s[i] + 1e-6
+    for i in (1, 3):
+        assert abs(f[i] - s[i]) <= 1e-9 * s[i]
 
 
 def test_mixed_euler_rk2_use_the_fp64_path(orc):
