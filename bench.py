@@ -13,7 +13,11 @@ Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
 dominant kernel (the one with the largest share of the step's time), timed
 live with HIP events around each launch on the context's stream, the
 four-stage-kernel run (13 field accesses per cell, the bandwidth-bound
-formulation), the complex64- and mixed-storage runs, and a CPU
+formulation), the complex64- and mixed-storage runs, the time per RKF45
+(Cash-Karp) attempt of the stage and fused kernels in complex128 and mixed
+storage on the same grid (rkf45_attempt), the last recorded run of
+`python -m gpe_amd.studies time_to_t` (time_to_t_recorded, read from
+profiles/, not timed by this run), and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC), and the
 reference's own Fortran build timed on its 64^3 vortex-ring case
@@ -74,6 +78,33 @@ def run_storage(p, psi, args, hip, fused=True):
         prof = kernel_profile(ctx, "rk4", args.steps, info)
         tiles = ctx.tiles()
     return ms, prof, tiles
+
+
+def rkf45_attempts(p, psi, count):
+    """RKF45 (rkqs, tolerance 1e-6, dt0 = 0.01) on the same 512^3 workload for
+    the Cash-Karp variants: ms per attempt from the per-launch event timing of
+    `count` accepted steps after two untimed ones, and per-kernel figures."""
+    from gpe_amd import Context, StepInfo
+    from gpe_amd.studies import variant
+
+    out = {}
+    for spec in ("c128/stages", "c128", "mixed/stages", "mixed"):
+        storage, fused = variant(spec)
+        with Context(p.replace(storage=storage, scheme="rk45", eps=1e-6)) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            info = StepInfo(0.01 + 0j)
+            ctx.step("rk45", 2, info)
+            a0 = info.n_accept + info.n_reject
+            prof = ctx.kernel_profile(lambda: ctx.step("rk45", count, info))
+            na = info.n_accept + info.n_reject - a0
+        ms = sum(v["avg_ms"] * v["launches"] for v in prof.values()) / max(na, 1)
+        attempt_bytes = sum(v["bytes"] * v["launches"] for v in prof.values()) / max(na, 1)
+        out[spec] = {"ms_per_attempt": round(ms, 4), "attempts": na,
+                     "hbm_frac": round(attempt_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "kernels": kernel_table(prof)}
+        print(f"[bench] rkf45 {spec}: {ms:.3f} ms/attempt", file=sys.stderr, flush=True)
+    return out
 
 
 def dominant(prof):
@@ -152,6 +183,8 @@ def main(argv=None) -> int:
                     help="RK4 steps of the reference build's 64^3 sample (0: skip)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
+    ap.add_argument("--rk45-steps", type=int, default=3,
+                    help="timed RKF45 steps per Cash-Karp variant (c128 and mixed, stage and fused; 0: skip)")
     ap.add_argument("--unfused", action="store_true", help="four stage kernels per RK4 step instead of two fused")
     ap.add_argument("--no-unfused", action="store_true", help="skip the four-stage-kernel comparison run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -234,6 +267,12 @@ def main(argv=None) -> int:
                            "roofline": roofline(profm, dominant(profm),
                                                 load_traffic(f"traffic_{dominant(profm)}_mixed_{args.n}.json")),
                            "kernels": kernel_table(profm)}
+    if args.rk45_steps > 0:
+        result["rkf45_attempt"] = rkf45_attempts(p, psi, args.rk45_steps)
+        tt = os.path.join(ROOT, "profiles", f"time_to_t_{args.n}.json")
+        if os.path.exists(tt):
+            with open(tt) as f:
+                result["time_to_t_recorded"] = json.load(f)
     if args.cpu_steps > 0:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:
