@@ -551,3 +551,30 @@ def test_full_size_fused_matches_stage_kernels():
     print(f"512^3 fused vs stage kernels: rel L2 {r:.2e}; norm drift {norms[True]:.2e}")
     assert r <= 1e-13
     assert abs(norms[True]) < 1e-6 and abs(norms[True] - norms[False]) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["c128", "mixed"])
+def test_full_size_fused_rkck_matches_stage_kernels(storage):
+    """512^3 random-phase workload, one Cash-Karp attempt and a retry at half
+    the dt: the fused attempt (three kernels) and the six stage kernels agree
+    on the candidate (1e-13 relative L2) and the error max-norm (1e-9)."""
+    p, psi = ics.config_turb(n=512, scheme="rk45", tau=0.01)
+    p = p.replace(storage=storage, eps=1e-6)
+    res = {}
+    for fused in (True, False):
+        with Context(p) as ctx:
+            ctx.set_fused(fused)
+            ctx.upload(psi)
+            em1, _ = ctx.rk45_attempt(0.02 + 0j, 0.02 + 0j)
+            em2, a2 = ctx.rk45_attempt(0.01 + 0j, 0.02 + 0j)
+            ctx.rk45_accept()
+            res[fused] = (em1, em2, a2, ctx.download())
+    f, s = res[True], res[False]
+    r = rel(f[3], s[3])
+    print(f"512^3 {storage} rkck fused vs stage kernels: errmax {f[0]:.6e}/{s[0]:.6e}, "
+          f"{f[1]:.6e}/{s[1]:.6e}; candidate rel L2 {r:.2e}")
+    assert r <= 1e-13
+    for i in (0, 1):
+        assert abs(f[i] - s[i]) <= 1e-9 * s[i] + 1e-9
+    assert abs(f[2] - s[2]) <= 1e-13 * s[2]
