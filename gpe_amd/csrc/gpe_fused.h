@@ -59,8 +59,11 @@ template <int ORD> struct FusedShape {
     static constexpr int NSLOT = (NHALO + 1023) / 1024;
 };
 
-// Per-thread geometry of a fused block.
-template <int ORD> struct FusedCtx {
+// Per-thread geometry of a fused block.  The thread-to-column map is a
+// per-kernel choice (measured): inner-first saves the rim lanes' share of
+// the second stage, but moves register allocation, and k_rk4_f34 in FP64
+// then spills.
+template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
     using S = FusedShape<ORD>;
     int z0, z1;
     bool active, inner, store;
@@ -79,13 +82,38 @@ template <int ORD> struct FusedCtx {
         const int ox0 = (txy % g.ntx) * S::OX, oy0 = (txy / g.ntx) * S::OY;
         z0 = zc * g.tz;
         z1 = min(z0 + g.tz, g.nz);
+        // INNER_FIRST: threads 0 .. OX*OY-1 own the inner tile, OX = 64
+        // columns per row, so whole wavefronts are inner and the second stage
+        // runs wave-uniform; the others own the H-wide rim: the H rows above
+        // and below, then the H columns left and right of each inner row.
+        // Otherwise thread t owns column (t % TXB, t / TXB) of the block.
         active = tid < S::NTB;
-        lx = tid % S::TXB;
-        ly = tid / S::TXB;
-        if (!active) ly = 0;
+        constexpr int NIN = S::OX * S::OY;
+        if (!INNER_FIRST) {
+            lx = tid % S::TXB;
+            ly = active ? tid / S::TXB : 0;
+        } else if (tid < NIN) {
+            lx = S::H + tid % S::OX;
+            ly = S::H + tid / S::OX;
+        } else {
+            int r = tid - NIN;
+            if (r < 2 * S::H * S::TXB) {
+                const int row = r / S::TXB;
+                lx = r % S::TXB;
+                ly = row < S::H ? row : S::OY + row;
+            } else if (r < 2 * S::H * S::TXB + 2 * S::H * S::OY) {
+                r -= 2 * S::H * S::TXB;
+                const int cc = r % (2 * S::H);
+                ly = S::H + r / (2 * S::H);
+                lx = cc < S::H ? cc : S::OX + cc;
+            } else {
+                lx = 0;
+                ly = 0;
+            }
+        }
         gx = ox0 - S::H + lx;
         gy = oy0 - S::H + ly;
-        inner = active && lx >= S::H && lx < S::TXB - S::H && ly >= S::H && ly < S::TYB - S::H;
+        inner = INNER_FIRST ? tid < NIN : active && lx >= S::H && lx < S::TXB - S::H && ly >= S::H && ly < S::TYB - S::H;
         store = inner && gx < g.nx && gy < g.ny;
         mx = bcmap(gx, g.nx, g.bcs);
         my = bcmap(gy, g.ny, g.bcs);
@@ -269,7 +297,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ C<R> ldsP[2][S::NTB];           // psi0 plane p
     __shared__ C<R> ldsB[NB][S::NTB];          // u3 planes z..z+H
-    const FusedCtx<ORD> c(g, bid);
+    const FusedCtx<ORD, sizeof(R) == 4> c(g, bid);  // inner-first spills in FP64 here
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
 
@@ -436,7 +464,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
     // pair 3: k5 (FP64) or the psi_new partial (FP32) of the own cell, planes z..z+H
     __shared__ C<R> ldsK[P3 ? NB : 1][P3 ? S::NTB : 1];
     __shared__ double red_sh[16];
-    const FusedCtx<ORD> c(g, bid);
+    const FusedCtx<ORD, !(P3 && sizeof(R) == 4)> c(g, bid);  // the map measured faster per kernel
     double red0 = 0.0, red1 = 0.0;
     if (c.z0 < c.z1) {  // uniform per block
         const int start = c.z0 - 2 * H;

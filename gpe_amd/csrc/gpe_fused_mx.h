@@ -54,7 +54,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     __shared__ CD ldsB[NB][S::NTB];                   // first-stage u_next, planes z..z+H
     __shared__ CD ldsS[P3 ? NB : 1][P3 ? S::NTB : 1];  // pair 3: the psi_new partial, planes z..z+H
     __shared__ double red_sh[16];
-    const FusedCtx<ORD> c(g, bid);
+    const FusedCtx<ORD, PAIR == 2> c(g, bid);  // the map measured faster per pair
     const SumSrc sum1{a.p0, a.d, mkc<double>(1.0, 0.0)};
     const SumSrc sum21{a.p0, a.f0, a.s21};
     const CkTab<double>& t = a.t;
