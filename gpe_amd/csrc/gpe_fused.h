@@ -61,8 +61,8 @@ template <int ORD> struct FusedShape {
 
 // Per-thread geometry of a fused block.  The thread-to-column map is a
 // per-kernel choice (measured): inner-first saves the rim lanes' share of
-// the second stage, but moves register allocation, and k_rk4_f34 in FP64
-// then spills.
+// the second stage, but it moves register allocation, and some kernels then
+// spill or run slower.
 template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
     using S = FusedShape<ORD>;
     int z0, z1;
@@ -297,12 +297,20 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ C<R> ldsP[2][S::NTB];           // psi0 plane p
     __shared__ C<R> ldsB[NB][S::NTB];          // u3 planes z..z+H
-    const FusedCtx<ORD, sizeof(R) == 4> c(g, bid);  // inner-first spills in FP64 here
+    // FP64: acc of the inner tile for planes z..z+H lives in an LDS ring
+    // indexed by the inner thread (with the inner-first map, threads
+    // 0 .. OX*OY-1), which leaves room in registers for that map (159 KB of
+    // LDS at order 4; 1.53 ms against 1.61 at 512^3).  FP32 keeps it in
+    // registers.
+    constexpr bool kAccLds = sizeof(R) == 8;
+    __shared__ C<R> ldsQ[kAccLds ? NB : 1][kAccLds ? S::OX * S::OY : 1];
+    const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
 
     // Register queues, slot (plane - start) % Q: u2 and psi0 for planes
-    // z..z+2H, u3 for planes z-H..z+H, acc for the inner planes z..z+H.
+    // z..z+2H, u3 for planes z-H..z+H, and (FP32) acc for the inner planes
+    // z..z+H.
     C<R> qu[Q], qp[Q], qv[Q], qa[H + 1];
 #pragma unroll
     for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
@@ -372,7 +380,9 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
                 sp.w = w;
                 const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, bcmap(p, g.nz, g.bcs));
                 const C<R> base = mkc<R>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
-                qa[H] = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
+                const C<R> acc = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
+                if constexpr (kAccLds) ldsQ[(p - start) % NB][threadIdx.x] = acc;
+                else qa[H] = acc;
             }
         }
         if (z >= c.z0 && c.inner) {
@@ -384,7 +394,8 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
-            const C<R> nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z), qa[0]);
+            const C<R> nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z),
+                                    kAccLds ? ldsQ[(z - start) % NB][threadIdx.x] : qa[0]);
             if (c.store) st_stream<R>(out + Cell{g.plane * z, c.own}, nw);
         }
 #pragma unroll
