@@ -431,12 +431,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<R> w, RhsC<R> k, c
 // the part of the second stage's sum that does not depend on its own k at
 // the first stage and carry it in a register queue for the H planes in
 // between.  Pair 3 needs three such partials (psi_new, the error, the
-// scale): in FP32 it carries them (the psi_new partial in an LDS ring); in
-// FP64 they would spill, so it carries only k5, in an LDS ring, and reads
-// psi0, f0, k3 and k4 of its output cell again at the second stage.  Those
-// second reads come two planes after the first and mostly miss L2 (4 fields
-// x 3 planes of 32 blocks exceed an XCD's 4 MB), so that kernel moves ~1.9x
-// its algorithmic bytes and is bandwidth-bound on them.
+// scale) and keeps part of them in LDS rings (see ck_pair_body).
 template <typename R> struct CkPairArgs {
     const C<R>* u;  // stencil input: psi0 (pair 1), u3 (pair 2), u5 (pair 3)
     const C<R>* p0;
@@ -465,15 +460,20 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
     // spill 12 registers (4.8 ms against 3.9 ms at 512^3).  In FP32 there is
     // room, and the early loads hide their latency (2.2 ms against 2.6 ms).
     constexpr bool kLate = sizeof(R) == 8;
-    // FP32 pair 3 carries the psi_new partial (LDS ring), the error partial
-    // and the scale (registers) from the first stage, as the other pairs do,
-    // instead of reading psi0, f0, k3 and k4 again: 2.0 ms against 2.2 ms.
-    // In FP64 the carried state spills (5.3 ms) and the second reads win.
-    constexpr bool kCarry = sizeof(R) == 4;
+    // Pair 3 carries its second stage's partials from the first stage: the
+    // psi_new partial in an LDS ring (ldsK), and the error partial and the
+    // scale |psi_scal|^2.  FP32 keeps those two in registers; FP64 keeps the
+    // error partial in a second LDS ring (ldsE), both rings indexed by the
+    // inner thread (inner-first map), 160 KB of LDS in all at order 4.
+    // Carried in registers the FP64 state spills (5.3 ms at 512^3); read
+    // again at the second stage, psi0, f0, k3 and k4 miss L2 (1.9x the
+    // algorithmic bytes, 3.9 ms).
+    constexpr bool kErrLds = sizeof(R) == 8;
+    constexpr int NIN = S::OX * S::OY;
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];         // stencil input plane p with halo
     __shared__ C<R> ldsB[NB][S::NTB];                 // first-stage u_next, planes z..z+H
-    // pair 3: k5 (FP64) or the psi_new partial (FP32) of the own cell, planes z..z+H
-    __shared__ C<R> ldsK[P3 ? NB : 1][P3 ? S::NTB : 1];
+    __shared__ C<R> ldsK[P3 ? NB : 1][P3 ? (kErrLds ? NIN : S::NTB) : 1];
+    __shared__ double2 ldsE[P3 && kErrLds ? NB : 1][P3 && kErrLds ? NIN : 1];
     __shared__ double red_sh[16];
     const FusedCtx<ORD, !(P3 && sizeof(R) == 4)> c(g, bid);  // the map measured faster per kernel
     double red0 = 0.0, red1 = 0.0;
@@ -580,20 +580,18 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                                   caxpy<R>(t.b64, pw.k4,
                                            caxpy<R>(t.b63, pw.k3, caxpy<R>(t.b62, pw.k2, caxpy<R>(t.b61, k1, pw.p0)))));
                     if (c.inner) {
-                        if constexpr (kCarry) {
-                            ldsK[(p - start) % NB][c.lb] =
-                                caxpy<R>(t.c4, pw.k4, caxpy<R>(t.c3, pw.k3, caxpy<R>(t.c1, k1, pw.p0)));
-                            qe[H] = mkc<double>(t.dc1 * (double)k1.x + t.dc3 * (double)pw.k3.x +
-                                                    t.dc4 * (double)pw.k4.x + t.dc5 * (double)k5.x,
-                                                t.dc1 * (double)k1.y + t.dc3 * (double)pw.k3.y +
-                                                    t.dc4 * (double)pw.k4.y + t.dc5 * (double)k5.y);
-                            const double ap = sqrt((double)pw.p0.x * pw.p0.x + (double)pw.p0.y * pw.p0.y);
-                            const double af = sqrt((double)pw.f0.x * pw.f0.x + (double)pw.f0.y * pw.f0.y);
-                            const double sr = ap + a.dt0_re * af + 1e-30, si = a.dt0_im * af;
-                            qd[H] = sr * sr + si * si;
-                        } else {
-                            ldsK[(p - start) % NB][c.lb] = k5;
-                        }
+                        const int slot = (p - start) % NB, li = kErrLds ? (int)threadIdx.x : c.lb;
+                        ldsK[slot][li] = caxpy<R>(t.c4, pw.k4, caxpy<R>(t.c3, pw.k3, caxpy<R>(t.c1, k1, pw.p0)));
+                        const double2 se = mkc<double>(t.dc1 * (double)k1.x + t.dc3 * (double)pw.k3.x +
+                                                           t.dc4 * (double)pw.k4.x + t.dc5 * (double)k5.x,
+                                                       t.dc1 * (double)k1.y + t.dc3 * (double)pw.k3.y +
+                                                           t.dc4 * (double)pw.k4.y + t.dc5 * (double)k5.y);
+                        if constexpr (kErrLds) ldsE[slot][li] = se;
+                        else qe[H] = se;
+                        const double ap = sqrt((double)pw.p0.x * pw.p0.x + (double)pw.p0.y * pw.p0.y);
+                        const double af = sqrt((double)pw.f0.x * pw.f0.x + (double)pw.f0.y * pw.f0.y);
+                        const double sr = ap + a.dt0_re * af + 1e-30, si = a.dt0_im * af;
+                        qd[H] = sr * sr + si * si;
                     }
                 }
                 qv[(PH + 2 * H) % Q] = ub;
@@ -622,28 +620,12 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                     }
                 } else {
                     if (c.store) {
-                        C<R> nw;
-                        double er, ei, den;
-                        if constexpr (kCarry) {
-                            nw = caxpy<R>(t.c6, kb, ldsK[(z - start) % NB][c.lb]);
-                            er = qe[0].x + t.dc6 * (double)kb.x;
-                            ei = qe[0].y + t.dc6 * (double)kb.y;
-                            den = qd[0];
-                        } else {
-                            // OpCkLast at the own cell, k5 from the ring
-                            const C<R> p0 = *(a.p0 + oz), f0 = *(a.f0 + oz), k3 = *(a.k3 + oz), k4 = *(a.k4 + oz);
-                            const C<R> k5 = ldsK[(z - start) % NB][c.lb];
-                            const C<R> k1 = cmul<R>(a.dt, f0);
-                            nw = caxpy<R>(t.c6, kb, caxpy<R>(t.c4, k4, caxpy<R>(t.c3, k3, caxpy<R>(t.c1, k1, p0))));
-                            er = t.dc1 * (double)k1.x + t.dc3 * (double)k3.x + t.dc4 * (double)k4.x +
-                                 t.dc5 * (double)k5.x + t.dc6 * (double)kb.x;
-                            ei = t.dc1 * (double)k1.y + t.dc3 * (double)k3.y + t.dc4 * (double)k4.y +
-                                 t.dc5 * (double)k5.y + t.dc6 * (double)kb.y;
-                            const double ap = sqrt((double)p0.x * p0.x + (double)p0.y * p0.y);
-                            const double af = sqrt((double)f0.x * f0.x + (double)f0.y * f0.y);
-                            const double sr = ap + a.dt0_re * af + 1e-30, si = a.dt0_im * af;
-                            den = sr * sr + si * si;
-                        }
+                        const int slot = (z - start) % NB, li = kErrLds ? (int)threadIdx.x : c.lb;
+                        const C<R> nw = caxpy<R>(t.c6, kb, ldsK[slot][li]);
+                        const double2 se = kErrLds ? ldsE[slot][li] : qe[0];
+                        const double er = se.x + t.dc6 * (double)kb.x;
+                        const double ei = se.y + t.dc6 * (double)kb.y;
+                        const double den = qd[0];
                         st_stream<R>(a.out + oz, nw);
                         const double e = sqrt((er * er + ei * ei) / den);
                         red0 = e > red0 ? e : red0;
