@@ -52,9 +52,14 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     constexpr bool P3 = PAIR == 3;
     __shared__ CD ldsA[2][S::LHA * S::LWA];           // stencil source plane p with halo
     __shared__ CD ldsB[NB][S::NTB];                   // first-stage u_next, planes z..z+H
-    __shared__ CD ldsS[P3 ? NB : 1][P3 ? S::NTB : 1];  // pair 3: the psi_new partial, planes z..z+H
+    // pair 3: the psi_new and error partials of the inner tile, planes
+    // z..z+H, in LDS rings indexed by the inner thread (inner-first map);
+    // in registers they spill
+    constexpr int NIN = S::OX * S::OY;
+    __shared__ CD ldsS[P3 ? NB : 1][P3 ? NIN : 1];
+    __shared__ CD ldsE[P3 ? NB : 1][P3 ? NIN : 1];
     __shared__ double red_sh[16];
-    const FusedCtx<ORD, PAIR == 2> c(g, bid);  // the map measured faster per pair
+    const FusedCtx<ORD, PAIR != 1> c(g, bid);  // the map measured faster per pair
     const SumSrc sum1{a.p0, a.d, mkc<double>(1.0, 0.0)};
     const SumSrc sum21{a.p0, a.f0, a.s21};
     const CkTab<double>& t = a.t;
@@ -68,15 +73,14 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         // Queues, slot (plane - start) % Q: the stencil source for planes
         // z..z+2H and the first stage's u for planes z-H..z+H; for planes
         // z..z+H: qs the inner tile's partial sum of D_next (pairs 1, 2), and
-        // qe, qd (pair 3) the error partial and |psi_scal|^2.
-        CD qu[Q], qv[Q], qs[NB], qe[NB];
+        // qd (pair 3) |psi_scal|^2.
+        CD qu[Q], qv[Q], qs[NB];
         double qd[NB];
 #pragma unroll
         for (int d = 0; d < Q; ++d) qv[d] = mkc<double>(0.0, 0.0);
 #pragma unroll
         for (int d = 0; d < NB; ++d) {
             qs[d] = mkc<double>(0.0, 0.0);
-            qe[d] = mkc<double>(0.0, 0.0);
             qd[d] = 1.0;
         }
         if (c.active) {
@@ -173,10 +177,11 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     ub = sum1.comb(pw.p0, d6);
                     if (c.inner) {
                         const CD p0 = csub<double>(ub, tod(d6));  // as OpMxCkLast: u6 - D6
-                        ldsS[(p - start) % NB][c.lb] =
+                        ldsS[(p - start) % NB][threadIdx.x] =
                             caxpy<double>(t.c4, k4v, caxpy<double>(t.c3, k3v, caxpy<double>(t.c1, k1, p0)));
-                        qe[H] = mkc<double>(t.dc1 * k1.x + t.dc3 * k3v.x + t.dc4 * k4v.x + t.dc5 * k5v.x,
-                                            t.dc1 * k1.y + t.dc3 * k3v.y + t.dc4 * k4v.y + t.dc5 * k5v.y);
+                        ldsE[(p - start) % NB][threadIdx.x] =
+                            mkc<double>(t.dc1 * k1.x + t.dc3 * k3v.x + t.dc4 * k4v.x + t.dc5 * k5v.x,
+                                        t.dc1 * k1.y + t.dc3 * k3v.y + t.dc4 * k4v.y + t.dc5 * k5v.y);
                         const CD f0v = tod(pw.f0);
                         const double ap = sqrt(p0.x * p0.x + p0.y * p0.y);
                         const double af = sqrt(f0v.x * f0v.x + f0v.y * f0v.y);
@@ -206,10 +211,11 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     }
                 } else {
                     if (c.store) {
-                        const CD nw = caxpy<double>(t.c6, kb, ldsS[(z - start) % NB][c.lb]);
+                        const CD nw = caxpy<double>(t.c6, kb, ldsS[(z - start) % NB][threadIdx.x]);
                         st_stream<double>(a.out + oz, nw);
-                        const double er = qe[0].x + t.dc6 * kb.x;
-                        const double ei = qe[0].y + t.dc6 * kb.y;
+                        const CD se = ldsE[(z - start) % NB][threadIdx.x];
+                        const double er = se.x + t.dc6 * kb.x;
+                        const double ei = se.y + t.dc6 * kb.y;
                         const double e = sqrt((er * er + ei * ei) / qd[0]);
                         red0 = e > red0 ? e : red0;
                         const double a2 = nw.x * nw.x + nw.y * nw.y;
@@ -220,7 +226,6 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
 #pragma unroll
             for (int d = 0; d < H; ++d) {
                 qs[d] = qs[d + 1];
-                qe[d] = qe[d + 1];
                 qd[d] = qd[d + 1];
             }
         };
