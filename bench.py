@@ -20,8 +20,9 @@ storage on the same grid (rkf45_attempt), the last recorded run of
 profiles/, not timed by this run), and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC), and the
-reference's own Fortran build timed on its 64^3 vortex-ring case
-(cpu_reference, --ref-steps steps, where oracle/_ref is present).
+reference's own Fortran build timed on the host cores (cpu_reference:
+--ref-steps RK4 steps of the same grid over 16 MPI ranks with the
+reference's y-z decomposition, where oracle/_ref holds that build).
 """
 from __future__ import annotations
 
@@ -138,24 +139,50 @@ def cpu_baseline(p, psi, steps):
             "kind": "port", "sample": f"{steps} RK4 step(s) of the same {p.nx}^3 workload ({s:.1f} s)"}
 
 
-def cpu_reference(steps):
+def cpu_reference(steps, n=512, ranks=(4, 4)):
     """The reference's own Fortran time-stepping path (oracle/_ref, compiled
-    from its unmodified sources; one MPI rank, one core) on `steps` RK4 steps
-    of the 64^3 vortex-ring workload, the only grid of the benchmark
-    configurations it is built for.  None where that build is absent."""
+    from its unmodified sources), timed on the host cores:
+      * where oracle/_ref/bench_<n>_<py>x<pz>/ref_bench is built, `steps` RK4
+        steps of the n^3 workload (periodic, dx 0.3, 4th order, eqn 1,
+        dt 0.005) over py * pz MPI ranks with the reference's own y-z
+        decomposition and ghost exchange, from an analytic initial field
+        (oracle/ref/ref_bench.f90; the time per step does not depend on it);
+      * else 40 RK4 steps of the 64^3 vortex-ring workload on one rank
+        (oracle/_ref/64x64x64/ref_driver).
+    None where neither build is present."""
     import shutil
     import subprocess
     import tempfile
 
-    from gpe_amd import ics
+    from gpe_amd import RunParams, ics
     from gpe_amd.io import run_in_text, write_end_state
 
-    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "64x64x64", "ref_driver")
-    if not os.access(exe, os.X_OK):
-        return None
-    p, psi = ics.config_ring64(seed=0)
+    here = os.path.dirname(os.path.abspath(__file__))
+    np_ = ranks[0] * ranks[1]
+    big = os.path.join(here, "oracle", "_ref", f"bench_{n}_{ranks[0]}x{ranks[1]}", "ref_bench")
+    mpiexec = shutil.which("mpiexec", path="/opt/conda/bin") or shutil.which("mpiexec")
     run = tempfile.mkdtemp(prefix="gpe_ref_")
     try:
+        if os.access(big, os.X_OK) and mpiexec:
+            xr = 0.3 * (n - 1) / 2
+            p = RunParams(nx=n, ny=n, nz=n, xr=xr, yr=xr, zr=xr, order=4, bcs=1, eqn_to_solve=1, scheme="rk4",
+                          tau=0.005)
+            with open(os.path.join(run, "run.in"), "w") as f:
+                f.write(run_in_text(p, 1.0))
+            r = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {mpiexec} -n {np_} {big} {steps}"],
+                               cwd=run, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            if r.returncode == 0:
+                f = r.stdout.split()
+                cells, nst, s = int(f[1]), int(f[2]), float(f[3])
+                return {"value": round(cells * nst / s / 1e9, 6), "unit": "Gcell-steps/s", "cores": np_,
+                        "kind": "reference",
+                        "sample": f"{nst} RK4 steps of the {n}^3 workload (analytic initial field), reference "
+                                  f"build, {np_} MPI ranks ({ranks[0]}x{ranks[1]} y-z decomposition), {s:.1f} s"}
+        exe = os.path.join(here, "oracle", "_ref", "64x64x64", "ref_driver")
+        if not os.access(exe, os.X_OK):
+            return None
+        steps = 40
+        p, psi = ics.config_ring64(seed=0)
         with open(os.path.join(run, "run.in"), "w") as f:
             f.write(run_in_text(p, (steps - 0.5) * p.tau))
         write_end_state(os.path.join(run, "in.dat"), psi, 0, 0.0, 0.0, p.dt0())
@@ -179,8 +206,9 @@ def main(argv=None) -> int:
     ap.add_argument("--n", type=int, default=512, help="grid points per axis (512 = the benchmark workload)")
     ap.add_argument("--tiles", default="", help="tx,ty,tz override of the stencil tiling")
     ap.add_argument("--cpu-steps", type=int, default=4, help="RK4 steps of the CPU baseline sample (0: skip)")
-    ap.add_argument("--ref-steps", type=int, default=40,
-                    help="RK4 steps of the reference build's 64^3 sample (0: skip)")
+    ap.add_argument("--ref-steps", type=int, default=2,
+                    help="RK4 steps of the reference build's 512^3 multi-rank sample (0: skip; the 64^3 "
+                         "one-rank fallback runs 40)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
     ap.add_argument("--rk45-steps", type=int, default=3,
@@ -277,7 +305,7 @@ def main(argv=None) -> int:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:
         result["cpu_baseline"] = None
-    result["cpu_reference"] = cpu_reference(args.ref_steps) if args.ref_steps > 0 else None
+    result["cpu_reference"] = cpu_reference(args.ref_steps, n=args.n) if args.ref_steps > 0 else None
     print(json.dumps(result), flush=True)
     return 0
 

@@ -23,11 +23,11 @@ MPI=${MPI_HOME:-/opt/conda}
 [ -d "$REF/examples/ring" ] || { echo "build_ref.sh: $REF not present, skipping"; exit 0; }
 
 write_pin() {
-  local dir=$1 nx=$2 ny=$3 nz=$4
+  local dir=$1 nx=$2 ny=$3 nz=$4 npy=${5:-1} npz=${6:-1}
   cat > "$dir/parameters.in" <<PIN
   integer, parameter :: pr = selected_real_kind(15,307)
-  integer, parameter :: nyprocs = 1
-  integer, parameter :: nzprocs = 1
+  integer, parameter :: nyprocs = $npy
+  integer, parameter :: nzprocs = $npz
   integer, parameter :: nx      = $nx
   integer, parameter :: ny      = $ny
   integer, parameter :: nz      = $nz
@@ -125,6 +125,28 @@ build_ics() {
   done < "$HERE/ref/ic_cases.txt"
 }
 
+# Multi-rank timing build: the reference's path at the benchmark grid with
+# its own y-z decomposition (nyprocs x nzprocs MPI ranks) and the harness
+# ref/ref_bench.f90.  Output: _ref/bench_<nx>_<npy>x<npz>/ref_bench.
+build_bench() {
+  local nx=$1 ny=$2 nz=$3 npy=$4 npz=$5
+  local dir=$OUT/bench_${nx}_${npy}x${npz}
+  mkdir -p "$dir"
+  write_pin "$dir" $nx $ny $nz $npy $npz
+  echo "      init_cond = cmplx(1.0_pr, 0.0_pr, pr)" > "$dir/ic.in"
+  local flags="-O2 -flto -I$dir -I$MPI/include -module-dir $dir"
+  for m in parameters constants error derivs ic variables solve; do
+    $FC $flags -c "$REF/examples/ring/$m.f90" -o "$dir/$m.o"
+  done
+  $FC $flags -c "$HERE/ref/ref_bench.f90" -o "$dir/ref_bench.o"
+  $FC -O2 -flto -o "$dir/ref_bench" "$dir"/{parameters,constants,error,derivs,ic,variables,solve,ref_bench}.o \
+      -L$MPI/lib -Wl,-rpath,$MPI/lib -lmpifort -lmpi -flto -fuse-ld=lld
+  echo "built $dir/ref_bench"
+}
+
 build_grid 64 64 64
 build_grid 24 20 18
 build_ics
+build_bench 512 512 512 4 4
+build_bench 32 32 32 2 2
+build_bench 32 32 32 1 1

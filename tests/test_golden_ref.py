@@ -99,3 +99,44 @@ def test_gpu_matches_reference_build(path):
     assert abs(info.dt - complex(z["dt"])) <= 1e-9 * abs(info.dt)
     assert abs(nrm - float(z["norm"])) <= 1e-12 * float(z["norm"])
     assert abs(e - float(z["energy"])) <= 1e-11 * abs(float(z["energy"]))
+
+
+def bench_field(p):
+    """The analytic field of oracle/ref/ref_bench.f90, array (z, y, x)."""
+    k, j, i = np.meshgrid(np.arange(p.nz), np.arange(p.ny), np.arange(p.nx), indexing="ij")
+    two_pi = 6.283185307179586
+    ph = 0.3 * np.sin(two_pi * i / p.nx) + 0.2 * np.cos(two_pi * (2 * j + k) / p.ny)
+    return np.ascontiguousarray((1.0 + 0.1 * np.cos(ph)) * np.exp(1j * ph))
+
+
+def test_reference_decomposition_matches_one_rank_and_oracle(orc, tmp_path):
+    """The multi-rank timing build of the reference (its own y-z
+    decomposition and ghost exchange over 2 x 2 MPI ranks, oracle/build_ref.sh)
+    gives the one-rank build's final norm bit for bit, and the oracle's to
+    1e-13, after 5 RK4 steps of a 32^3 periodic 4th-order run."""
+    import shutil
+    import subprocess
+
+    from gpe_amd.io import run_in_text
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exes = {n: os.path.join(root, "oracle", "_ref", f"bench_32_{n}", "ref_bench") for n in ("1x1", "2x2")}
+    mpiexec = shutil.which("mpiexec", path="/opt/conda/bin") or shutil.which("mpiexec")
+    if not mpiexec or not all(os.access(e, os.X_OK) for e in exes.values()):
+        pytest.skip("built by oracle/build_ref.sh where the reference sources and MPICH are present")
+    n = 32
+    xr = 0.3 * (n - 1) / 2
+    p = RunParams(nx=n, ny=n, nz=n, xr=xr, yr=xr, zr=xr, order=4, bcs=1, eqn_to_solve=1, scheme="rk4", tau=0.005)
+    (tmp_path / "run.in").write_text(run_in_text(p, 1.0))
+    norms = {}
+    for name, ranks in (("1x1", 1), ("2x2", 4)):
+        r = subprocess.run(["bash", "-c", f"ulimit -s unlimited && exec {mpiexec} -n {ranks} {exes[name]} 5"],
+                           cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr[-2000:]
+        fields = r.stdout.split()
+        assert int(fields[0]) == ranks and int(fields[1]) == n ** 3 and int(fields[2]) == 5
+        norms[name] = float(fields[4])
+    assert norms["1x1"] == norms["2x2"]
+    exp, _ = orc.rk4_n(p, bench_field(p), 0.005 + 0j, 5)
+    n_orc = orc.norm(p, exp)
+    assert abs(norms["1x1"] - n_orc) <= 1e-13 * n_orc, (norms, n_orc)
