@@ -12,7 +12,7 @@ log(dt_next / dt) from the recent history of the solve:
 Acceptance is still decided by the embedded Cash-Karp estimate
 (errmax <= 1); after a rejection the reference's shrink rule is used, and a
 non-finite proposal falls back to the reference's growth rule (the plain
-controller is the fallback and the judge).
+controller is the fallback and the arbiter).
 
 Training data are RKF45 traces the solver generates itself on random-phase
 runs (the 512^3 turbulence workload's IC on smaller grids, 2nd and 4th
@@ -196,7 +196,7 @@ class LearnedController(PlainController):
     without growth right after a rejected attempt; the reference's rules
     shrink after a rejection and replace a non-finite proposal.
 
-    The plain controller is the fallback and the judge: when more than
+    The plain controller is the fallback and the arbiter: when more than
     `max_reject` of the last `window` steps needed a retry, the plain rule
     proposes for the next `cooldown` steps (a problem unlike the training
     data cannot cost more than a few rejections).  With explore > 0 the
@@ -219,7 +219,7 @@ class LearnedController(PlainController):
 
     def reset(self):
         self.n_fallback = 0        # non-finite proposals
-        self.n_plain_steps = 0     # steps proposed by the plain rule after the judge intervened
+        self.n_plain_steps = 0     # steps proposed by the plain rule after the arbiter intervened
         self.rejected = False
         self.recent = []
         self.hold = 0
@@ -298,8 +298,8 @@ def generate_traces(runs=TRACE_RUNS, log=print, model=None):
 
 
 def train_pipeline(runs=TRACE_RUNS, rounds: int = 2, steps: int = 4000, log=print):
-    """Round 1: traces of the plain/exploring controller, train.  Later
-    rounds: traces of the learned controller itself (small exploration),
+    """Pass 1: traces of the plain/exploring controller, train.  Later
+    passes: traces of the learned controller itself (small exploration),
     added to the data set, retrain (on-policy aggregation)."""
     data = generate_traces(runs, log=log)
     model = None

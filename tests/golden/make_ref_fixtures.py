@@ -10,7 +10,7 @@ The ref_ll_* fixtures hold inputs and the reference's line length for the
 branches smooth fields do not reach.  The ref_ic_* fixtures hold the reference's initial conditions (its ics with
 the expressions of oracle/ref/ic_cases.txt as ic.in) on a 16 x 12 x 10 grid.
 
-Run in the build container (the reference is not present on the GPU box):
+Run where the reference sources and their build (oracle/build_ref.sh) are present:
     python tests/golden/make_ref_fixtures.py
 """
 from __future__ import annotations
