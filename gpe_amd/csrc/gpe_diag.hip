@@ -116,6 +116,64 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_partial(const double* __re
     }
 }
 
+// The same sums, lag-blocked (nx <= 64 K): the block's four wavefronts each
+// take one line of a group of four; lane t owns the K lags r = tK .. tK+K-1
+// and runs i = 0, K, 2K, .. with the K x K products of a step unrolled, a
+// broadcast read of v(i + ii) and K + K reads of v(i + r) per step instead of
+// two LDS reads per product.  Lines are zero-padded in LDS to nx + 2K, so
+// the products past the line's end add exact zeros and need no masks.  The
+// four wavefronts' sums are added in a fixed order.
+template <int K>
+__global__ __launch_bounds__(kDiagThreads) void k_vcf_lags(const double* __restrict__ vx, int nx, long long nlines,
+                                                          double* __restrict__ partial) {
+    extern __shared__ double lines[];  // 4 lines of nx + 2K, reused for the wave sums
+    constexpr int NW = kDiagThreads / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lw = nx + 2 * K;
+    double* L = lines + wv * lw;
+    const long long per = (nlines + gridDim.x - 1) / gridDim.x;
+    const long long l0 = (long long)blockIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
+    const int r0 = lane * K;
+    double acc[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc[q] = 0.0;
+    for (long long lg = l0; lg < l1; lg += NW) {
+        const long long l = lg + wv;
+        __syncthreads();
+        for (int i = lane; i < lw; i += 64) L[i] = (l < l1 && i < nx) ? vx[l * nx + i] : 0.0;
+        __syncthreads();
+        if (l < l1 && r0 < nx) {
+            double b[2 * K];
+#pragma unroll
+            for (int j = 0; j < K; ++j) b[j] = L[r0 + j];
+            for (int i0 = 0; i0 < nx - r0; i0 += K) {
+#pragma unroll
+                for (int j = 0; j < K; ++j) b[K + j] = L[i0 + r0 + K + j];
+#pragma unroll
+                for (int ii = 0; ii < K; ++ii) {
+                    const double a = L[i0 + ii];
+#pragma unroll
+                    for (int q = 0; q < K; ++q) acc[q] += a * b[ii + q];
+                }
+#pragma unroll
+                for (int j = 0; j < K; ++j) b[j] = b[K + j];
+            }
+        }
+    }
+    __syncthreads();
+    double* red = lines;  // NW x nx
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+        if (r0 + q < nx) red[wv * nx + r0 + q] = acc[q];
+    __syncthreads();
+    for (int r = threadIdx.x; r < nx; r += kDiagThreads) {
+        double sum = red[r];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) sum += red[w * nx + r];
+        partial[(long long)blockIdx.x * nx + r] = sum;
+    }
+}
+
 // out[r] = sum over blocks (fixed order) of partial[b * nx + r].
 __global__ __launch_bounds__(kDiagThreads) void k_vcf_final(const double* __restrict__ partial, int nx, int nb,
                                                            double* __restrict__ out) {
@@ -138,10 +196,22 @@ hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const doub
     return hipGetLastError();
 }
 
+template <int K>
+static void launch_vcf_lags(const double* vx, int nx, long long nlines, double* partial, int nblocks, hipStream_t s) {
+    const size_t lds = (size_t)(kDiagThreads / 64) * (nx + 2 * K) * sizeof(double);
+    hipLaunchKernelGGL((k_vcf_lags<K>), dim3(nblocks), dim3(kDiagThreads), lds, s, vx, nx, nlines, partial);
+}
+
 hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,
                       hipStream_t s) {
-    hipLaunchKernelGGL(k_vcf_partial, dim3(nblocks), dim3(kDiagThreads), (size_t)nx * sizeof(double), s, vx, nx, nlines,
-                       partial);
+    if (nx <= 64) launch_vcf_lags<1>(vx, nx, nlines, partial, nblocks, s);
+    else if (nx <= 128) launch_vcf_lags<2>(vx, nx, nlines, partial, nblocks, s);
+    else if (nx <= 256) launch_vcf_lags<4>(vx, nx, nlines, partial, nblocks, s);
+    else if (nx <= 512) launch_vcf_lags<8>(vx, nx, nlines, partial, nblocks, s);
+    else if (nx <= 1024) launch_vcf_lags<16>(vx, nx, nlines, partial, nblocks, s);
+    else
+        hipLaunchKernelGGL(k_vcf_partial, dim3(nblocks), dim3(kDiagThreads), (size_t)nx * sizeof(double), s, vx, nx,
+                           nlines, partial);
     hipLaunchKernelGGL(k_vcf_final, dim3((nx + kDiagThreads - 1) / kDiagThreads), dim3(kDiagThreads), 0, s, partial, nx,
                        nblocks, out);
     return hipGetLastError();

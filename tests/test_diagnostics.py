@@ -150,6 +150,28 @@ def test_gpu_diagnostics_c64_and_larger_grid(orc):
     assert np.allclose(st64["vstdev"], exp_st["vstdev"], rtol=1e-4)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx", [40, 100, 256, 300, 512, 1000, 1500])
+def test_gpu_vcf_line_lengths(nx):
+    """The velocity correlation over every lag-block width of the kernel
+    (K = 1..16 lags per lane, nx up to 1024) and the direct-sum kernel beyond
+    (nx 1500), against the oracle's get_vcf to 1e-10."""
+    from oracle import diag_oracle as dg
+
+    ny, nz = 7, 6
+    p = RunParams(nx=nx, ny=ny, nz=nz, xr=0.3 * (nx - 1) / 2, yr=0.3 * (ny - 1) / 2, zr=0.3 * (nz - 1) / 2, order=4,
+                  bcs=1)
+    rng = np.random.default_rng(nx)
+    x = np.arange(nx)
+    psi = np.ascontiguousarray(np.exp(1j * (2 * np.pi * 3 * x / nx)[None, None, :]) *
+                               (1.0 + 0.2 * rng.standard_normal((nz, ny, nx))))
+    exp_f = dg.vcf(p, psi)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        f = ctx.vcf()
+    assert np.allclose(f, exp_f, rtol=1e-10, atol=1e-12)
+
+
 LL_FIXTURES = sorted(glob.glob(os.path.join(HERE, "golden", "ref_ll_*.npz")))
 
 
