@@ -79,7 +79,15 @@ template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
         const int tid = threadIdx.x;
         const int ntxy = g.ntx * g.nty;
         const int txy = bid % ntxy, zc = bid / ntxy;
-        const int ox0 = (txy % g.ntx) * S::OX, oy0 = (txy / g.ntx) * S::OY;
+        // Tiles run in column groups 4 tiles wide (all rows of a group, then
+        // the next group), so the blocks one XCD runs at a time form a patch
+        // taller than wide: the y rims, 4 of every 15 rows, are re-read from
+        // that XCD's L2 more often, at the price of the x rims (4 of 68);
+        // 1-2% faster at 512^3 than whole rows.
+        constexpr int kGroupW = 4;
+        const int gw = g.ntx % kGroupW == 0 ? kGroupW : g.ntx;
+        const int grp = txy / (gw * g.nty), rem = txy % (gw * g.nty);
+        const int ox0 = (grp * gw + rem % gw) * S::OX, oy0 = (rem / gw) * S::OY;
         z0 = zc * g.tz;
         z1 = min(z0 + g.tz, g.nz);
         // INNER_FIRST: threads 0 .. OX*OY-1 own the inner tile, OX = 64
