@@ -24,12 +24,13 @@
 // z-loop unrolled 2H + 1 times (march_rotating), so they rotate by renaming
 // instead of being copied down one slot per plane.
 //
-// Per thread (own column): register queues of the inputs for planes z..z+2H,
-// of the first stage's result for planes z-H..z+H, and (F34) of acc for the
-// inner planes z..z+H.  LDS: the first stage's input plane p with an H halo (double
-// buffered), F34's psi0 plane p (double buffered), and the first stage's
-// results in a ring of planes z..z+H, written when computed and read H
-// iterations later, so one barrier per iteration orders every LDS reuse.
+// Per thread (own column): register queues of the inputs for planes z..z+2H
+// and of the first stage's result for planes z-H..z+H; F34's acc for the
+// inner planes z..z+H is a register queue in FP32 and an LDS ring indexed by
+// the inner thread in FP64.  LDS: the first stage's input plane p with an H
+// halo (double buffered), F34's psi0 plane p (double buffered), and the first
+// stage's results in a ring of planes z..z+H, written when computed and read
+// H iterations later, so one barrier per iteration orders every LDS reuse.
 // F12 keeps the loads of the next GPE_FUSED_DEPTH (3) planes in flight while a
 // plane is evaluated; F34 one plane.
 //
