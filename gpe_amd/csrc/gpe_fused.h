@@ -80,12 +80,14 @@ template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
         const int tid = threadIdx.x;
         const int ntxy = g.ntx * g.nty;
         const int txy = bid % ntxy, zc = bid / ntxy;
-        // Tiles run in column groups 4 tiles wide (all rows of a group, then
+        // Tiles run in column groups 2 tiles wide (all rows of a group, then
         // the next group), so the blocks one XCD runs at a time form a patch
         // taller than wide: the y rims, 4 of every 15 rows, are re-read from
-        // that XCD's L2 more often, at the price of the x rims (4 of 68);
-        // 1-2% faster at 512^3 than whole rows.
-        constexpr int kGroupW = 4;
+        // that XCD's L2 more often, at the price of the x rims (4 of 68).
+        // Measured at 512^3 against whole rows / 4-wide / 1-wide groups: the
+        // Cash-Karp attempt 7.8 against 8.5 / 8.4 / 8.8 ms, RK4 2.64 against
+        // 2.70 / 2.69 / 2.71 ms.
+        constexpr int kGroupW = 2;
         const int gw = g.ntx % kGroupW == 0 ? kGroupW : g.ntx;
         const int grp = txy / (gw * g.nty), rem = txy % (gw * g.nty);
         const int ox0 = (grp * gw + rem % gw) * S::OX, oy0 = (rem / gw) * S::OY;
