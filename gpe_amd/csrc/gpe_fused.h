@@ -466,10 +466,11 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
     using S = FusedShape<ORD>;
     constexpr int H = S::H, NB = H + 1, Q = 2 * H + 1;
     constexpr bool P3 = PAIR == 3;
-    // FP64 pair 3 loads its pointwise inputs after the first stage's stencil:
-    // loaded at the top of the iteration they are live across the stencil and
-    // spill 12 registers (4.8 ms against 3.9 ms at 512^3).  In FP32 there is
-    // room, and the early loads hide their latency (2.2 ms against 2.6 ms).
+    // FP64 pairs 2 and 3 load their pointwise inputs after the first stage's
+    // stencil: loaded earlier they are live across the stencil, and pair 3
+    // spilled 12 registers (4.8 ms against 3.9 ms at 512^3); pair 2 takes
+    // 2.91 against 2.95 ms.  In FP32 there is room, and earlier loads hide
+    // their latency (pair 3: 2.2 ms against 2.6 ms).
     constexpr bool kLate = sizeof(R) == 8;
     // Pair 3 carries its second stage's partials from the first stage: the
     // psi_new partial in an LDS ring (ldsK), and the error partial and the
@@ -525,7 +526,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
             }
         };
         // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H
-        // and (pair 2) the pointwise inputs of plane zz + H
+        // and (FP32 pair 2) the pointwise inputs of plane zz + H
         struct Pf {
             C<R> qn;
             C<R> hr[S::NSLOT];
@@ -535,7 +536,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
             const long long pb = g.plane * bcmap(zz + H, g.nz, g.bcs);
             if (c.active) {
                 F.qn = *(a.u + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
-                if constexpr (!P3) load_pw(F.pw, pb);
+                if constexpr (PAIR == 2 && !kLate) load_pw(F.pw, pb);
             }
 #pragma unroll
             for (int s = 0; s < S::NSLOT; ++s)
@@ -578,6 +579,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                     ub = caxpy<R>(t.b21, k1, p0);
                     qs[H] = caxpy<R>(t.b31, k1, p0);
                 } else if constexpr (PAIR == 2) {
+                    if constexpr (kLate) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
                     const C<R> k3 = cmul<R>(a.dt, fa);
                     const C<R> k1 = cmul<R>(a.dt, pw.f0);
                     if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.ka + Cell{g.plane * p, c.own}, k3);
