@@ -15,9 +15,10 @@ live with HIP events around each launch on the context's stream, the
 four-stage-kernel run (13 field accesses per cell, the bandwidth-bound
 formulation), the complex64- and mixed-storage runs, the time per RKF45
 (Cash-Karp) attempt of the stage and fused kernels in complex128 and mixed
-storage on the same grid (rkf45_attempt), the last recorded run of
-`python -m gpe_amd.studies time_to_t` (time_to_t_recorded, read from
-profiles/, not timed by this run), and a CPU
+storage on the same grid (rkf45_attempt), the
+time to T = 2 of plain complex128, mixed-storage, learned and complex64
+RKF45 with each tolerance tuned to the same 1e-6 error (time_to_t), the
+48-member ensemble in every mode (ensemble), and a CPU
 baseline (the FP64 oracle, OpenMP, on a bounded sample of the same
 workload: --cpu-steps RK4 steps of the same grid and IC), and the
 reference's own Fortran build timed on the host cores (cpu_reference:
@@ -105,6 +106,55 @@ def rkf45_attempts(p, psi, count):
                      "hbm_frac": round(attempt_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "kernels": kernel_table(prof)}
         print(f"[bench] rkf45 {spec}: {ms:.3f} ms/attempt", file=sys.stderr, flush=True)
+    return out
+
+
+def ensemble_leg(steps):
+    """The 48-member ensemble workload (members from {64..192}^3, seeds
+    0-47, 2nd order, RK4 dt 0.005): ms per step and total cell-steps/s of
+    every mode over `steps` steps, and the batching gain (one persistent
+    launch for all members against one launch per member, same kernels)
+    apart from the fusion gain."""
+    from gpe_amd.ensemble import MODES, Ensemble, ensemble_config, gains, time_modes
+
+    members, fields = ensemble_config(48)
+    with Ensemble(members) as ens:
+        rows = time_modes(ens, fields, list(MODES), steps)
+        cells = ens.cells
+    out = {"workload": "48 random-phase members, seeds 0-47, n from {64,96,128,160,192}^3, dx 0.3, 2nd-order FD, "
+                       "RK4 dt 0.005, complex128", "cells": cells, "steps": steps,
+           "modes": {r["ensemble"]: {"ms_per_step": round(r["ms_per_step"], 4),
+                                     "gcell_steps_s": round(r["gcell_steps_s"], 3),
+                                     "field_accesses": r["accesses"], "hbm_frac": round(r["hbm_frac"], 4)}
+                     for r in rows},
+           "gains": {k: round(v, 4) for k, v in gains(rows).items()}}
+    print(f"[bench] ensemble: " + ", ".join(f"{k} {v['gcell_steps_s']:.1f}" for k, v in out["modes"].items()),
+          file=sys.stderr, flush=True)
+    return out
+
+
+def time_to_t_leg(p, psi, T):
+    """RKF45 from the same 512^3 IC to T (last step clamped onto T): each
+    method's tolerance tuned so its relative L2 error against an FP64 run at
+    tolerance 1e-10 lands at or just under 1e-6; wall time (attempts,
+    controller, syncs) and the speed-up over the fastest plain complex128
+    RKF45 (the reference's controller, fused Cash-Karp attempt)."""
+    from gpe_amd.learned import DEFAULT_MODEL, LearnedController, load_model
+    from gpe_amd.studies import time_to_t_tuned
+
+    pr = p.replace(scheme="rk45", tau=0.01)
+    model = load_model(DEFAULT_MODEL)
+    methods = [("plain-c128", "c128", None), ("plain-mixed", "mixed", None),
+               ("learned-mixed", "mixed", lambda eps: LearnedController(model, pr.replace(eps=eps))),
+               ("plain-c64", "c64", None)]
+    t0 = time.perf_counter()
+    runs = time_to_t_tuned(pr, psi, T, 0.01, methods)
+    out = {"workload": f"random-phase turbulence {p.nx}^3, dx 0.3, 4th-order FD, eqn 1, RKF45 to T={T:g}",
+           "error_target": 1e-6, "baseline": "plain-c128", "runs": runs,
+           "leg_wall_s": round(time.perf_counter() - t0, 2)}
+    print("[bench] time to T: " + ", ".join(f"{k} {v['wall_s']:.2f} s (eps {v['eps']:.2g}, err "
+                                           f"{v.get('rel_l2_vs_oracle', 0):.2e})" for k, v in runs.items()),
+          file=sys.stderr, flush=True)
     return out
 
 
@@ -213,6 +263,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
     ap.add_argument("--rk45-steps", type=int, default=3,
                     help="timed RKF45 steps per Cash-Karp variant (c128 and mixed, stage and fused; 0: skip)")
+    ap.add_argument("--ttt-T", type=float, default=2.0,
+                    help="final time of the matched-error time-to-T leg (0: skip)")
+    ap.add_argument("--ens-steps", type=int, default=5, help="timed steps per mode of the 48-member ensemble leg "
+                                                            "(0: skip)")
     ap.add_argument("--unfused", action="store_true", help="four stage kernels per RK4 step instead of two fused")
     ap.add_argument("--no-unfused", action="store_true", help="skip the four-stage-kernel comparison run")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -297,10 +351,10 @@ def main(argv=None) -> int:
                            "kernels": kernel_table(profm)}
     if args.rk45_steps > 0:
         result["rkf45_attempt"] = rkf45_attempts(p, psi, args.rk45_steps)
-        tt = os.path.join(ROOT, "profiles", f"time_to_t_{args.n}.json")
-        if os.path.exists(tt):
-            with open(tt) as f:
-                result["time_to_t_recorded"] = json.load(f)
+    if args.ttt_T > 0:
+        result["time_to_t"] = time_to_t_leg(p, psi, args.ttt_T)
+    if args.ens_steps > 0:
+        result["ensemble"] = ensemble_leg(args.ens_steps)
     if args.cpu_steps > 0:
         result["cpu_baseline"] = cpu_baseline(p, psi, args.cpu_steps)
     else:

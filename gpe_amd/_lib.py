@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libgpehip.so")
 GPE_OK = 0
 GPE_C128, GPE_C64, GPE_MIXED = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
+GPE_FIELD_STATE, GPE_FIELD_AUX = 0, 1
+GPE_ABI_VERSION = 2
 GPE_NKINDS = 35  # kernel kinds counted by gpe_prof_read
 SCHEMES = {"euler": 0, "rk2": 1, "rk4": 2, "rk45": 3}
 DERIV_OPS = {"x": 0, "y": 1, "z": 2, "xx": 3, "yy": 4, "zz": 5, "lap": 6}
@@ -58,6 +60,9 @@ PROTOTYPES = [
     ("gpe_ctx_destroy", None, [_VP]),
     ("gpe_set_tiles", _I, [_VP, _I, _I, _I]),
     ("gpe_set_fused", _I, [_VP, _I]),
+    ("gpe_field_upload", _I, [_VP, _I, _VP, _I, _I64, _I64, _I64]),
+    ("gpe_select_field", _I, [_VP, _I]),
+    ("gpe_set_real_time", _I, [_VP, _I]),
     ("gpe_get_tiles", _I, [_VP, ct.POINTER(_I), ct.POINTER(_I), ct.POINTER(_I)]),
     ("gpe_upload", _I, [_VP, _VP, _I, _I64, _I64, _I64]),
     ("gpe_download", _I, [_VP, _VP, _I]),

@@ -111,6 +111,23 @@ class Context:
         nya, yo, zo = self._layout(a)
         self._chk(self.lib.gpe_upload(self.h, a.ctypes.data, _host_dtype(a), nya, yo, zo), "gpe_upload")
 
+    def upload_aux(self, a: np.ndarray) -> None:
+        """Upload into the context's auxiliary field (never the state)."""
+        nya, yo, zo = self._layout(a)
+        self._chk(self.lib.gpe_field_upload(self.h, _lib.GPE_FIELD_AUX, a.ctypes.data, _host_dtype(a), nya, yo, zo),
+                  "gpe_field_upload")
+
+    def select(self, aux: bool) -> None:
+        """Make the reductions, diagnostics, renormalise, imprint_line, peek and
+        download act on the auxiliary field (True) or the state (False)."""
+        f = _lib.GPE_FIELD_AUX if aux else _lib.GPE_FIELD_STATE
+        self._chk(self.lib.gpe_select_field(self.h, f), "gpe_select_field")
+
+    def set_real_time(self, real_time: bool) -> None:
+        """The module variable real_time that rkqs reads for the direction
+        of a retried step (src/solve.f90:195-201)."""
+        self._chk(self.lib.gpe_set_real_time(self.h, int(bool(real_time))), "gpe_set_real_time")
+
     def download(self, dtype=np.complex128) -> np.ndarray:
         out = np.empty(self.params.shape, dtype=dtype)
         self._chk(self.lib.gpe_download(self.h, out.ctypes.data, _host_dtype(out)), "gpe_download")

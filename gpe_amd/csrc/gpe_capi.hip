@@ -64,6 +64,10 @@ struct gpe_ctx {
     size_t hist_cap = 0;
     bool f0_valid = false;
     bool cand_valid = false;
+    // auxiliary field (gpe_field_upload): host arrays handed to the
+    // diagnostics drop-ins land here, never in the state
+    void* aux = nullptr;
+    bool use_aux = false;
     // drop-in scratch
     void* sin = nullptr;
     void* sout = nullptr;
@@ -142,12 +146,18 @@ static void dfree(gpe_ctx* c, void* p) {
 
 static size_t field_bytes(const gpe_ctx* c) { return (size_t)c->n * c->esz; }
 
+// The field the reductions, diagnostics, gpe_renormalise, gpe_imprint_line,
+// gpe_peek and gpe_download act on (gpe_select_field); stepping always acts
+// on the state.
+static void* sel_field(const gpe_ctx* c) { return c->use_aux ? c->aux : c->psi; }
+
 static int validate(const gpe_params* p, std::string& why) {
     if (!p) { why = "params is NULL"; return GPE_EINVAL; }
     const int mn = p->order == 4 ? 6 : 6;
     if (p->nx < mn || p->ny < mn || p->nz < mn) { why = "nx, ny, nz must be >= 6"; return GPE_EINVAL; }
     // the kernels address a plane with 32-bit byte offsets (16 B per value)
-    if ((long long)p->nx * p->ny > (1LL << 28)) { why = "nx*ny must be <= 2^28"; return GPE_EINVAL; }
+    // and mark lanes without access with the out-of-range offset 0xFFFFFFF0
+    if ((long long)p->nx * p->ny >= (1LL << 28)) { why = "nx*ny must be < 2^28"; return GPE_EINVAL; }
     if (p->order != 2 && p->order != 4) { why = "order must be 2 or 4"; return GPE_EINVAL; }
     if (p->bcs != 1 && p->bcs != 2) { why = "bcs must be 1 (periodic) or 2 (reflective)"; return GPE_EINVAL; }
     if (p->eqn_to_solve < 1 || p->eqn_to_solve > 4) { why = "eqn_to_solve must be 1..4"; return GPE_EINVAL; }
@@ -829,10 +839,10 @@ static int norm_mass(gpe_ctx* c, double* v, int weighted) {
     int rc = ensure_partial(c, kNormBlocks);
     if (rc) return rc;
     if (c->p.storage != GPE_C64) {
-        HIPCHK(c, launch_norm_partial<double>((const double2*)c->psi, c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
+        HIPCHK(c, launch_norm_partial<double>((const double2*)sel_field(c), c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
                                               kNormBlocks, c->stream));
     } else {
-        HIPCHK(c, launch_norm_partial<float>((const float2*)c->psi, c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
+        HIPCHK(c, launch_norm_partial<float>((const float2*)sel_field(c), c->p.nx, c->p.ny, c->p.nz, weighted, c->partial,
                                              kNormBlocks, c->stream));
     }
     HIPCHK(c, launch_reduce_final(RED_SUM, c->partial, kNormBlocks, 1, c->dres, c->stream));
@@ -903,7 +913,7 @@ void gpe_ctx_destroy(gpe_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     prof_clear(c);
-    void* bufs[] = {c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
+    void* bufs[] = {c->aux,   c->psi,   c->uA,   c->uB,   c->acc,   c->f0,   c->k2,   c->k3,   c->k4,    c->k5,
                     c->pnew,  c->sin,  c->sout, c->stage, c->partial, c->dres, c->dtab, c->mx[0], c->mx[1],
                     c->mx[2], c->mx[3], c->mx[4], c->mx[5], c->mx[6], c->vel, c->hist, c->fbuf, c->sres, c->llbuf, c->llpsi};
     for (void* b : bufs) dfree(c, b);
@@ -948,9 +958,35 @@ int gpe_upload(gpe_ctx* c, const void* host, int host_dtype, int64_t ny_alloc, i
     return rc;
 }
 
+int gpe_field_upload(gpe_ctx* c, int field, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off,
+                     int64_t z_off) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_field_upload: ctx is NULL");
+    if (field == GPE_FIELD_STATE) return gpe_upload(c, host, host_dtype, ny_alloc, y_off, z_off);
+    if (field != GPE_FIELD_AUX) return fail(c, GPE_EINVAL, "gpe_field_upload: field must be GPE_FIELD_STATE or GPE_FIELD_AUX");
+    int rc = dalloc(c, &c->aux, field_bytes(c), "auxiliary field");
+    if (rc) return rc;
+    return upload_to(c, c->aux, host, host_dtype, ny_alloc, y_off, z_off);
+}
+
+int gpe_select_field(gpe_ctx* c, int field) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_select_field: ctx is NULL");
+    if (field != GPE_FIELD_STATE && field != GPE_FIELD_AUX)
+        return fail(c, GPE_EINVAL, "gpe_select_field: field must be GPE_FIELD_STATE or GPE_FIELD_AUX");
+    if (field == GPE_FIELD_AUX && !c->aux)
+        return fail(c, GPE_ESTATE, "gpe_select_field: no auxiliary field has been uploaded");
+    c->use_aux = field == GPE_FIELD_AUX;
+    return 0;
+}
+
+int gpe_set_real_time(gpe_ctx* c, int real_time) {
+    if (!c) return fail(nullptr, GPE_EINVAL, "gpe_set_real_time: ctx is NULL");
+    c->p.real_time = real_time != 0;
+    return 0;
+}
+
 int gpe_download(gpe_ctx* c, void* host, int host_dtype) {
     if (!c) return fail(nullptr, GPE_EINVAL, "gpe_download: ctx is NULL");
-    return download_from(c, c->psi, host, host_dtype);
+    return download_from(c, sel_field(c), host, host_dtype);
 }
 
 int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
@@ -1037,10 +1073,10 @@ int gpe_energy(gpe_ctx* c, double* energy) {
     int rc;
     if (c->p.storage != GPE_C64) {
         OpEnergy<double> o;
-        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<double, 2>(c, sel_field(c), o) : run_op<double, 4>(c, sel_field(c), o);
     } else {
         OpEnergy<float> o;
-        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<float, 2>(c, sel_field(c), o) : run_op<float, 4>(c, sel_field(c), o);
     }
     if (rc) return rc;
     double s;
@@ -1057,10 +1093,10 @@ int gpe_momentum(gpe_ctx* c, double mom[3]) {
     int rc;
     if (c->p.storage != GPE_C64) {
         OpMomentum<double> o{c->p.nx, c->p.ny, c->p.nz};
-        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<double, 2>(c, sel_field(c), o) : run_op<double, 4>(c, sel_field(c), o);
     } else {
         OpMomentum<float> o{c->p.nx, c->p.ny, c->p.nz};
-        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<float, 2>(c, sel_field(c), o) : run_op<float, 4>(c, sel_field(c), o);
     }
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1080,10 +1116,10 @@ static int velocity_field(gpe_ctx* c, int masked, int ncomp) {
     const double f = c->p.order == 2 ? 2.0 : 12.0;  // (2.0_pr*dx) or (12.0_pr*dx), src/derivs.f90:41-56
     if (c->p.storage != GPE_C64) {
         OpVelocity<double> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
-        return c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+        return c->p.order == 2 ? run_op<double, 2>(c, sel_field(c), o) : run_op<double, 4>(c, sel_field(c), o);
     }
     OpVelocity<float> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
-    return c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+    return c->p.order == 2 ? run_op<float, 2>(c, sel_field(c), o) : run_op<float, 4>(c, sel_field(c), o);
 }
 
 int gpe_velocity_pdf(gpe_ctx* c, int nbins, int64_t count[3], double mean[3], double stdev[3], double vmax[3],
@@ -1170,11 +1206,11 @@ int gpe_hamiltonian(gpe_ctx* c, double* h) {
     if (c->p.storage != GPE_C64) {
         RhsC<double> k = make_rhs<double>(c);
         OpHamiltonian<double> o{k, k.pim >= 0 ? 1.0 : -1.0};
-        rc = c->p.order == 2 ? run_op<double, 2>(c, c->psi, o) : run_op<double, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<double, 2>(c, sel_field(c), o) : run_op<double, 4>(c, sel_field(c), o);
     } else {
         RhsC<float> k = make_rhs<float>(c);
         OpHamiltonian<float> o{k, k.pim >= 0 ? 1.0 : -1.0};
-        rc = c->p.order == 2 ? run_op<float, 2>(c, c->psi, o) : run_op<float, 4>(c, c->psi, o);
+        rc = c->p.order == 2 ? run_op<float, 2>(c, sel_field(c), o) : run_op<float, 4>(c, sel_field(c), o);
     }
     if (rc) return rc;
     double s;
@@ -1190,12 +1226,14 @@ int gpe_renormalise(gpe_ctx* c, double norm) {
     if (!(norm > 0.0) || !isfinite(norm)) return fail(c, GPE_EINVAL, "gpe_renormalise: norm must be positive");
     const double snn = sqrt(c->p.nn), sn = sqrt(norm);
     if (c->p.storage != GPE_C64) {
-        HIPCHK(c, launch_scale<double>((double2*)c->psi, c->n, snn, sn, c->stream));
+        HIPCHK(c, launch_scale<double>((double2*)sel_field(c), c->n, snn, sn, c->stream));
     } else {
-        HIPCHK(c, launch_scale<float>((float2*)c->psi, c->n, snn, sn, c->stream));
+        HIPCHK(c, launch_scale<float>((float2*)sel_field(c), c->n, snn, sn, c->stream));
     }
-    c->f0_valid = false;
-    c->cand_valid = false;
+    if (!c->use_aux) {
+        c->f0_valid = false;
+        c->cand_valid = false;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1217,12 +1255,14 @@ int gpe_imprint_line(gpe_ctx* c, const gpe_line* vl, double scal) {
     spacing(c->p, d);
     const double lo[3] = {-c->p.xr, -c->p.yr, -c->p.zr};
     if (c->p.storage != GPE_C64) {
-        HIPCHK(c, launch_imprint_line<double>((double2*)c->psi, c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
+        HIPCHK(c, launch_imprint_line<double>((double2*)sel_field(c), c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
     } else {
-        HIPCHK(c, launch_imprint_line<float>((float2*)c->psi, c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
+        HIPCHK(c, launch_imprint_line<float>((float2*)sel_field(c), c->p.nx, c->p.ny, c->n, lo, d, lp, c->stream));
     }
-    c->f0_valid = false;
-    c->cand_valid = false;
+    if (!c->use_aux) {
+        c->f0_valid = false;
+        c->cand_valid = false;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1235,10 +1275,10 @@ int gpe_peek(gpe_ctx* c, int32_t i, int32_t j, int32_t k, double out[2]) {
     const long long q = ((long long)k * c->p.ny + j) * c->p.nx + i;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->esz == 16) {
-        HIPCHK(c, hipMemcpy(out, (const char*)c->psi + q * 16, 16, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(out, (const char*)sel_field(c) + q * 16, 16, hipMemcpyDeviceToHost));
     } else {
         float v[2];
-        HIPCHK(c, hipMemcpy(v, (const char*)c->psi + q * 8, 8, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(v, (const char*)sel_field(c) + q * 8, 8, hipMemcpyDeviceToHost));
         out[0] = v[0];
         out[1] = v[1];
     }
@@ -1252,10 +1292,10 @@ int gpe_linelength(gpe_ctx* c, double* length) {
     const long long nl = linelength_lines(ny, nz), nb = linelength_blocks(ny, nz);
     int rc = dalloc(c, (void**)&c->llbuf, (size_t)(7 * nl + nb + 1) * sizeof(double), "line length");
     if (rc) return rc;
-    const double2* src = (const double2*)c->psi;
+    const double2* src = (const double2*)sel_field(c);
     if (c->p.storage == GPE_C64) {
         if ((rc = dalloc(c, &c->llpsi, (size_t)c->n * 16, "line length c128 copy"))) return rc;
-        HIPCHK(c, launch_to_c128<float>((const float2*)c->psi, (double2*)c->llpsi, c->n, c->stream));
+        HIPCHK(c, launch_to_c128<float>((const float2*)sel_field(c), (double2*)c->llpsi, c->n, c->stream));
         src = (const double2*)c->llpsi;
     }
     double* last = c->llbuf;
@@ -1289,7 +1329,7 @@ int gpe_condensed_particles(gpe_ctx* c, double row[7], double eta[7], int64_t nh
     if (hipfftSetStream(c->fplan, c->stream) != HIPFFT_SUCCESS)
         return fail(c, GPE_EHIP, "gpe_condensed_particles: hipfftSetStream failed");
     // a = FFT+(psi) (FFTW_BACKWARD), normalised by 1/sqrt(N) in the sums below
-    HIPCHK(c, hipMemcpyAsync(c->fbuf, c->psi, (size_t)c->n * c->esz, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fbuf, sel_field(c), (size_t)c->n * c->esz, hipMemcpyDeviceToDevice, c->stream));
     hipfftResult fr = dbl ? hipfftExecZ2Z(c->fplan, (hipfftDoubleComplex*)c->fbuf, (hipfftDoubleComplex*)c->fbuf,
                                           HIPFFT_BACKWARD)
                           : hipfftExecC2C(c->fplan, (hipfftComplex*)c->fbuf, (hipfftComplex*)c->fbuf, HIPFFT_BACKWARD);

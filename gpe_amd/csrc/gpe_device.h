@@ -94,6 +94,15 @@ __device__ __forceinline__ int bcmap(int i, int n, int bcs) {
     return i >= n ? p - i : i;
 }
 
+// bcmap for an index at most n - 2 outside [0, n) (one fold), without the
+// integer division: the z-marching kernels map a few planes around their
+// range every plane, in scalar registers (n >= 6 and H <= 2 keep them in
+// that band).
+__device__ __forceinline__ int bcmap1(int i, int n, int bcs) {
+    if (bcs == 1) return i < 0 ? i + n : (i >= n ? i - n : i);
+    return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
 // XCD receives a contiguous range of tiles and neighbouring tiles share that
 // XCD's L2.

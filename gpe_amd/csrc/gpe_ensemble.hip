@@ -22,6 +22,9 @@
 //                       two buffers from step to step
 //   GPE_ENS_FUSED_GRAPH GPE_ENS_FUSED captured in a hipGraph (an even number of
 //                       steps) and replayed
+//   GPE_ENS_PER_MEMBER_FUSED  the single-context fused RK4 launched once per
+//                       member (2 x members launches): the baseline of the
+//                       fused batched modes
 // The fused modes need complex128 members whose right-hand side has no
 // optional terms, as the single-context fused path does.
 // Each member's arithmetic is the single-context kernel's (same operators,
@@ -146,6 +149,7 @@ struct gpe_ens {
     void* fdesc = nullptr;
     int ftotal = 0, fgrid = 0;
     bool fused_ok = false;
+    int ncu = 256;  // compute units of the device (fused tiling)
     int fflip = 0;
     hipGraph_t fgraph = nullptr;
     hipGraphExec_t fgexec = nullptr;
@@ -198,6 +202,7 @@ int build_fused_desc(gpe_ens* e) {
     int dev = 0, ncu = 0;
     EHIP(e, hipGetDevice(&dev));
     EHIP(e, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    e->ncu = ncu;
     std::vector<int> idx(e->n);
     for (int m = 0; m < e->n; ++m) idx[m] = m;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return e->cells[a] > e->cells[b]; });
@@ -429,12 +434,47 @@ template <typename R, int ORD> int step_member(gpe_ens* e, int m, double dre, do
     return 0;
 }
 
+// One fused RK4 step of member m with the single-context launch (the two
+// fused kernels on the member's own grid, its default z-chunks); the new
+// state lands in ub, which becomes psi.
+template <int ORD> int step_member_fused(gpe_ens* e, int m, double dre, double dim) {
+    const gpe_params& p = e->mp[m];
+    Geom g{};
+    g.nx = p.nx;
+    g.ny = p.ny;
+    g.nz = p.nz;
+    g.bcs = p.bcs;
+    g.plane = (long long)p.nx * p.ny;
+    fused_tiles<ORD>(p.nx, p.ny, p.nz, e->ncu, g);
+    const StW<double> w = make_w<double>(p);
+    const RhsC<double> k = make_rhs_p<double>(p, nullptr);
+    const double2 dt = mkc<double>(dre, dim);
+    const double2* psi = (const double2*)e->psi[m];
+    double2* u2 = (double2*)e->ua[m];
+    EHIP(e, (launch_rk4_fused<double, ORD>(g, w, k, psi, u2, nullptr, dt, 12, e->stream)));
+    EHIP(e, (launch_rk4_fused<double, ORD>(g, w, k, psi, u2, (double2*)e->ub[m], dt, 34, e->stream)));
+    std::swap(e->psi[m], e->ub[m]);
+    return 0;
+}
+
 template <typename R> int ens_step(gpe_ens* e, int mode, double dre, double dim, int nsteps) {
     if (e->desc_dirty) {
         int rc = build_desc<R>(e);
         if (rc) return rc;
     }
     if (mode == GPE_ENS_FUSED || mode == GPE_ENS_FUSED_GRAPH) return ens_step_fused<R>(e, mode, dre, dim, nsteps);
+    if (mode == GPE_ENS_PER_MEMBER_FUSED) {
+        if (!e->fused_ok)
+            return efail(e, GPE_EINVAL, "gpe_ens_step: the fused modes need complex128 members whose right-hand "
+                                        "side has no optional terms (trap, Urhs, dissipation, eqn 3)");
+        for (int s = 0; s < nsteps; ++s)
+            for (int m = 0; m < e->n; ++m) {
+                int rc = e->mp[0].order == 2 ? step_member_fused<2>(e, m, dre, dim) : step_member_fused<4>(e, m, dre, dim);
+                if (rc) return rc;
+            }
+        if (nsteps > 0) e->desc_dirty = true;  // the state buffers changed roles
+        return 0;
+    }
     if (mode == GPE_ENS_PER_MEMBER) {
         for (int s = 0; s < nsteps; ++s)
             for (int m = 0; m < e->n; ++m) {
@@ -510,7 +550,7 @@ int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out) {
         if (p.storage != GPE_C128 && p.storage != GPE_C64)
             return efail(nullptr, GPE_EINVAL, "gpe_ens_create: storage must be GPE_C128 or GPE_C64");
         if (p.nx < 6 || p.ny < 6 || p.nz < 6) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: nx, ny, nz must be >= 6");
-        if ((long long)p.nx * p.ny > (1LL << 28)) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: nx*ny must be <= 2^28");
+        if ((long long)p.nx * p.ny >= (1LL << 28)) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: nx*ny must be < 2^28");
         if (!(p.xr > 0) || !(p.yr > 0) || !(p.zr > 0)) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: xr, yr, zr must be > 0");
         if (p.diss_amp != 0.0) return efail(nullptr, GPE_EINVAL, "gpe_ens_create: diss_amp is not supported in an ensemble");
     }
@@ -598,7 +638,7 @@ int gpe_ens_download(gpe_ens* e, int m, void* host, int host_dtype) {
 
 int gpe_ens_step(gpe_ens* e, int mode, double dt_re, double dt_im, int nsteps) {
     if (!e) return efail(nullptr, GPE_EINVAL, "gpe_ens_step: ens is NULL");
-    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_FUSED_GRAPH || nsteps < 0)
+    if (mode < GPE_ENS_PER_MEMBER || mode > GPE_ENS_PER_MEMBER_FUSED || nsteps < 0)
         return efail(e, GPE_EINVAL, "gpe_ens_step: bad mode or nsteps");
     return e->mp[0].storage == GPE_C128 ? ens_step<double>(e, mode, dt_re, dt_im, nsteps)
                                         : ens_step<float>(e, mode, dt_re, dt_im, nsteps);

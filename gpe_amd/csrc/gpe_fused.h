@@ -166,30 +166,48 @@ template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
 // Runs step(z, F) for z = start .. z1-1 with DEPTH sets of in-flight loads:
 // fetch(F, z) issues the loads iteration z consumes, step(z, F) consumes
 // them and refills F for iteration z + DEPTH.
-template <int DEPTH, class Pf, class Fetch, class Step>
-__device__ __forceinline__ void fused_march(int start, int z1, Fetch fetch, Step step) {
+// pad() issues a store with no lanes in range after each fetch of the
+// preamble, so the preamble leaves the same memory operations in flight as
+// the loop's iterations (a fetch then a store) and the loop's first wait
+// covers one plane's loads, as every later one does.
+template <int DEPTH, class Pf, class Fetch, class Step, class Pad>
+__device__ __forceinline__ void fused_march(int start, int z1, Fetch fetch, Step step, Pad pad) {
     if constexpr (DEPTH == 1) {
         Pf F0{};
         fetch(F0, start);
+        pad();
         for (int z = start; z < z1; ++z) step(z, F0);
     } else if constexpr (DEPTH == 3) {
+        // whole triples in the loop, the remainder after it: no step inside
+        // the loop is conditional, so the loop's memory operations are the
+        // same on every path
         Pf F0{}, F1{}, F2{};
         fetch(F0, start);
-        if (start + 1 < z1) fetch(F1, start + 1);
-        if (start + 2 < z1) fetch(F2, start + 2);
-        for (int z = start; z < z1; z += 3) {
+        pad();
+        fetch(F1, start + 1);
+        pad();
+        fetch(F2, start + 2);
+        pad();
+        int z = start;
+        for (; z + 2 < z1; z += 3) {
             step(z, F0);
-            if (z + 1 < z1) step(z + 1, F1);
-            if (z + 2 < z1) step(z + 2, F2);
+            step(z + 1, F1);
+            step(z + 2, F2);
         }
+        if (z < z1) step(z, F0);
+        if (z + 1 < z1) step(z + 1, F1);
     } else {
         Pf F0{}, F1{};
         fetch(F0, start);
-        if (start + 1 < z1) fetch(F1, start + 1);
-        for (int z = start; z < z1; z += 2) {
+        pad();
+        fetch(F1, start + 1);
+        pad();
+        int z = start;
+        for (; z + 1 < z1; z += 2) {
             step(z, F0);
-            if (z + 1 < z1) step(z + 1, F1);
+            step(z + 1, F1);
         }
+        if (z < z1) step(z, F0);
     }
 }
 
@@ -206,9 +224,21 @@ __device__ __forceinline__ void march_phases(int z, int z1, Step& step) {
     }
 }
 
+template <int U, int PH = 0, class Step>
+__device__ __forceinline__ void march_group(int z, Step& step) {
+    if constexpr (PH < U) {
+        step(z, std::integral_constant<int, PH>{});
+        march_group<U, PH + 1>(z + 1, step);
+    }
+}
+
+// Whole groups of U planes in the loop (no step inside it is conditional,
+// so its memory operations are the same on every path), the remainder after.
 template <int U, class Step>
 __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
-    for (int z = start; z < z1; z += U) march_phases<U>(z, z1, step);
+    int z = start;
+    for (; z + U <= z1; z += U) march_group<U>(z, step);
+    march_phases<U>(z, z1, step);
 }
 
 // Stages 1+2 of the fused block `bid` of a grid (the body of k_rk4_f12 and
@@ -223,30 +253,41 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
                                              int bid) {
     using S = FusedShape<ORD>;
     constexpr int H = S::H, D = GPE_FUSED_DEPTH;
+    constexpr unsigned E = sizeof(C<R>);
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];
     __shared__ C<R> ldsB[H + 1][S::NTB];
     const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;  // uniform per block
     const int start = c.z0 - 2 * H;
+    // byte offsets within a plane; kNoOff where the lane has no access
+    const unsigned pbytes = (unsigned)g.plane * E;
+    const unsigned ocol = c.active ? c.col * E : kNoOff;
+    const unsigned oown = c.store ? c.own * E : kNoOff;
+    unsigned ohal[S::NSLOT];
+#pragma unroll
+    for (int s = 0; s < S::NSLOT; ++s) ohal[s] = c.hv[s] ? c.hoff[s] * E : kNoOff;
+    auto pl = [&](const C<R>* f, int zz) { return plane_res<R>(f, g.plane * bcmap1(zz, g.nz, g.bcs), pbytes); };
 
     C<R> qp[2 * H + 1];  // psi0, planes z..z+2H
     C<R> qv[2 * H + 1];  // u1, planes z-H..z+H
 #pragma unroll
     for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<R>(0.0, 0.0);
-    if (c.active) {
 #pragma unroll
-        for (int d = 0; d < 2 * H; ++d) qp[d] = *(psi + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
-    }
-    // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H
+    for (int d = 0; d < 2 * H; ++d) qp[d] = bld<R>(pl(psi, start + d), ocol);
+    // loads of iteration zz: centre of plane zz + 2H, halo of plane zz + H.
+    // Every iteration issues them (the last D iterations reload the last
+    // planes), so the memory instructions in flight are the same on every
+    // path and a plane's loads are waited for alone.
     struct Pf {
         C<R> qn;
         C<R> hr[S::NSLOT];
     };
     auto fetch = [&](Pf& F, int zz) {
-        if (c.active) F.qn = *(psi + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
+        zz = zz < c.z1 ? zz : c.z1 - 1;
+        F.qn = bld<R>(pl(psi, zz + 2 * H), ocol);
+        const BRes rh = pl(psi, zz + H);
 #pragma unroll
-        for (int s = 0; s < S::NSLOT; ++s)
-            if (c.hv[s]) F.hr[s] = *(psi + Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), c.hoff[s]});
+        for (int s = 0; s < S::NSLOT; ++s) F.hr[s] = bld<R>(rh, ohal[s]);
     };
     auto step = [&](int z, Pf& F) {
         const int p = z + H;
@@ -257,7 +298,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
         for (int s = 0; s < S::NSLOT; ++s)
             if (c.hv[s]) A[c.hl[s]] = F.hr[s];
         __syncthreads();
-        if (z + D < c.z1) fetch(F, z + D);
+        fetch(F, z + D);
         if (c.active) {
             // first stage at plane p, whole block
             FusedSten<R, ORD, S::LWA, TERMS> sa;
@@ -266,11 +307,13 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qp[d];
             sa.w = w;
-            const C<R> u1 = cfma<R>(hdt, rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs)), qp[H]);
+            const C<R> u1 = cfma<R>(hdt, rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs)), qp[H]);
             qv[2 * H] = u1;
             ldsB[(p - start) % (H + 1)][c.lb] = u1;
         }
-        if (z >= c.z0 && c.inner) {
+        C<R> u2 = mkc<R>(0.0, 0.0);
+        const bool second = z >= c.z0;
+        if (second && c.inner) {
             // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[(z - start) % (H + 1)];
@@ -278,16 +321,18 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
             sb.w = w;
-            const C<R> u2 = cfma<R>(hdt, rhs_eval<R>(k, sb, c.mx, c.my, z), qp[0]);
-            if (c.store) st_stream<R>(u2out + Cell{g.plane * z, c.own}, u2);
+            u2 = cfma<R>(hdt, rhs_eval<R>(k, sb, c.mx, c.my, z), qp[0]);
         }
+        // the store is issued on every path (kNoOff where there is no output)
+        bst<R>(plane_res<R>(u2out, g.plane * (second ? z : c.z0), pbytes), second ? oown : kNoOff, u2);
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) {
             qp[d] = qp[d + 1];
             qv[d] = qv[d + 1];
         }
     };
-    fused_march<D, Pf>(start, c.z1, fetch, step);
+    fused_march<D, Pf>(start, c.z1, fetch, step,
+                       [&] { bst<R>(plane_res<R>(u2out, 0, pbytes), kNoOff, mkc<R>(0.0, 0.0)); });
 }
 
 template <typename R, int ORD, int TERMS>
@@ -304,7 +349,8 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     using S = FusedShape<ORD>;
     // one plane of loads in flight: a second set would not fit beside the
     // three register queues without spilling
-    constexpr int H = S::H, D = 1, NB = H + 1, Q = 2 * H + 1;
+    constexpr int H = S::H, NB = H + 1, Q = 2 * H + 1;
+    constexpr unsigned E = sizeof(C<R>);
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];  // u2 plane p with halo
     __shared__ C<R> ldsP[2][S::NTB];           // psi0 plane p
     __shared__ C<R> ldsB[NB][S::NTB];          // u3 planes z..z+H
@@ -318,6 +364,13 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     const FusedCtx<ORD> c(g, bid);
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
+    const unsigned pbytes = (unsigned)g.plane * E;
+    const unsigned ocol = c.active ? c.col * E : kNoOff;
+    const unsigned oown = c.store ? c.own * E : kNoOff;
+    unsigned ohal[S::NSLOT];
+#pragma unroll
+    for (int s = 0; s < S::NSLOT; ++s) ohal[s] = c.hv[s] ? c.hoff[s] * E : kNoOff;
+    auto pl = [&](const C<R>* f, int zz) { return plane_res<R>(f, g.plane * bcmap1(zz, g.nz, g.bcs), pbytes); };
 
     // Register queues, slot (plane - start) % Q: u2 and psi0 for planes
     // z..z+2H, u3 for planes z-H..z+H, and (FP32) acc for the inner planes
@@ -327,31 +380,29 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
 #pragma unroll
     for (int d = 0; d <= H; ++d) qa[d] = mkc<R>(0.0, 0.0);
-    if (c.active) {
 #pragma unroll
-        for (int d = 0; d < 2 * H; ++d) {
-            const long long zb = g.plane * bcmap(start + d, g.nz, g.bcs);
-            qu[d] = *(u2 + Cell{zb, c.col});
-            qp[d] = *(psi + Cell{zb, c.col});
-        }
+    for (int d = 0; d < 2 * H; ++d) {
+        qu[d] = bld<R>(pl(u2, start + d), ocol);
+        qp[d] = bld<R>(pl(psi, start + d), ocol);
     }
     constexpr R two3 = R(2.0 / 3.0);
     struct Pf {
         C<R> qnu, qnp;
         C<R> hr[S::NSLOT];
     };
+    // loads of iteration zz (every iteration issues them; the last one
+    // reloads its own planes)
     auto fetch = [&](Pf& F, int zz) {
-        if (c.active) {
-            const long long zb = g.plane * bcmap(zz + 2 * H, g.nz, g.bcs);
-            F.qnu = *(u2 + Cell{zb, c.col});
-            F.qnp = *(psi + Cell{zb, c.col});
-        }
+        zz = zz < c.z1 ? zz : c.z1 - 1;
+        F.qnu = bld<R>(pl(u2, zz + 2 * H), ocol);
+        F.qnp = bld<R>(pl(psi, zz + 2 * H), ocol);
+        const BRes rh = pl(u2, zz + H);
 #pragma unroll
-        for (int s = 0; s < S::NSLOT; ++s)
-            if (c.hv[s]) F.hr[s] = *(u2 + Cell{g.plane * bcmap(zz + H, g.nz, g.bcs), c.hoff[s]});
+        for (int s = 0; s < S::NSLOT; ++s) F.hr[s] = bld<R>(rh, ohal[s]);
     };
     Pf F{};
     fetch(F, start);
+    bst<R>(plane_res<R>(out, 0, pbytes), kNoOff, mkc<R>(0.0, 0.0));  // as the iterations' store (see fused_march)
     auto step = [&](int z, auto phase) {
         constexpr int PH = decltype(phase)::value;
         const int p = z + H;
@@ -367,7 +418,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
         for (int s = 0; s < S::NSLOT; ++s)
             if (c.hv[s]) A[c.hl[s]] = F.hr[s];
         __syncthreads();
-        if (z + D < c.z1) fetch(F, z + D);
+        fetch(F, z + 1);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
             FusedSten<R, ORD, S::LWA, TERMS> sa;
@@ -376,7 +427,8 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
             sa.w = w;
-            const C<R> f2 = rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+            const int pm = bcmap1(p, g.nz, g.bcs);
+            const C<R> f2 = rhs_eval<R>(k, sa, c.mx, c.my, pm);
             const C<R> p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
             const C<R> u3 = cfma<R>(dt, f2, p0);
             qv[(PH + 2 * H) % Q] = u3;
@@ -389,14 +441,16 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[(PH + d) % Q];
                 sp.w = w;
-                const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+                const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, pm);
                 const C<R> base = mkc<R>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
                 const C<R> acc = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
                 if constexpr (kAccLds) ldsQ[(p - start) % NB][threadIdx.x] = acc;
                 else qa[H] = acc;
             }
         }
-        if (z >= c.z0 && c.inner) {
+        C<R> nw = mkc<R>(0.0, 0.0);
+        const bool second = z >= c.z0;
+        if (second && c.inner) {
             // fourth stage at plane z, inner tile (ldsB of plane z was written
             // H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
@@ -405,10 +459,10 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
-            const C<R> nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z),
-                                    kAccLds ? ldsQ[(z - start) % NB][threadIdx.x] : qa[0]);
-            if (c.store) st_stream<R>(out + Cell{g.plane * z, c.own}, nw);
+            nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z),
+                         kAccLds ? ldsQ[(z - start) % NB][threadIdx.x] : qa[0]);
         }
+        bst<R>(plane_res<R>(out, g.plane * (second ? z : c.z0), pbytes), second ? oown : kNoOff, nw);
 #pragma unroll
         for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
     };

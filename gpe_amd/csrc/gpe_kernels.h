@@ -83,6 +83,47 @@ template <typename T> __device__ __forceinline__ const T* operator+(const T* p, 
     return (const T*)((const char*)(p + uniform_i64(c.zb)) + c.own * (unsigned)sizeof(T));
 }
 
+// Predicated plane access without branches.  A plane of a field becomes a
+// buffer resource (scalar base = the plane's first element, range = the
+// plane's bytes); lanes pass a 32-bit byte offset, and a lane that must not
+// access memory passes kNoOff, which the hardware range check turns into a
+// load of 0 or a dropped store with no memory traffic.  With no branch around
+// a load or store every wave issues the same memory instructions in the same
+// order, so the compiler's wait counts cover exactly the loads a value needs
+// (a load under a divergent branch may be skipped by a whole wave, and the
+// compiler then waits for every memory operation in flight, prefetches and
+// store acknowledgements included).  Needs nx * ny * sizeof(value) <=
+// kNoOff, i.e. nx * ny < 2^28 for complex128 (checked at context creation).
+constexpr unsigned kNoOff = 0xFFFFFFF0u;
+using BRes = __amdgpu_buffer_rsrc_t;
+enum { BUF_CACHED = 0, BUF_NT = 2 };  // cache policy bits of the buffer instructions (2: nt)
+
+template <typename R> __device__ __forceinline__ BRes plane_res(const C<R>* f, long long zb, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)(f + uniform_i64(zb)), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename R, int AUX = BUF_CACHED> __device__ __forceinline__ C<R> bld(BRes r, unsigned off) {
+    if constexpr (sizeof(R) == 8) {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        const u4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, AUX);
+        return __builtin_bit_cast(C<R>, v);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        const u2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, AUX);
+        return __builtin_bit_cast(C<R>, v);
+    }
+}
+
+template <typename R, int AUX = BUF_NT> __device__ __forceinline__ void bst(BRes r, unsigned off, C<R> a) {
+    if constexpr (sizeof(R) == 8) {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, a), r, (int)off, 0, AUX);
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, a), r, (int)off, 0, AUX);
+    }
+}
+
 // Sources of the stencil input.  PlainSrc reads one field.  SumSrc forms
 // u = a + s d from an FP64 field a and a complex64 increment d scaled by the
 // FP64 complex s, so a stage can stencil psi0 + (increment) without the sum

@@ -11,8 +11,9 @@ RK4 with dt = 0.005 for 1000 steps.
 
 prints one JSON line per mode (one launch per member, batched persistent
 launches, hipGraph replay, cache-resident groups, and for complex128 the
-fused RK4 of all members in two persistent launches per step, eager and in a
-hipGraph) with total cell-steps/s, and the autotuner log.
+fused RK4 once per member and of all members in two persistent launches per
+step, eager and in a hipGraph) with total cell-steps/s, the batching and
+fusion gains apart, and the autotuner log.
 """
 from __future__ import annotations
 
@@ -28,7 +29,8 @@ import numpy as np
 from . import _lib, ics
 from .params import RunParams, xr_for_spacing
 
-MODES = {"per_member": 0, "batched": 1, "graph": 2, "grouped": 3, "fused": 4, "fused_graph": 5}
+MODES = {"per_member": 0, "batched": 1, "graph": 2, "grouped": 3, "fused": 4, "fused_graph": 5,
+         "per_member_fused": 6}
 SIZES = (64, 96, 128, 160, 192)
 
 
@@ -174,10 +176,50 @@ def autotune(ens: Ensemble, dt: complex, steps: int = 10, shapes=((32, 8), (64, 
     return tx, ty, tzm, log
 
 
+def time_modes(ens: Ensemble, fields, modes, steps: int, dt: complex = 0.005 + 0j, warm: int = 2):
+    """Per mode: ms per step over `steps` steps (HIP events on the ensemble's
+    stream) after `warm` untimed ones, from the same initial fields."""
+    timer = Timer()
+    cells = ens.cells
+    esz = 16 if ens.dtype == np.complex128 else 8
+    rows = []
+    for mode in modes:
+        for m, f in enumerate(fields):
+            ens.upload(m, f)
+        ens.step(mode, dt, warm)  # warm-up (and graph capture)
+        for m, f in enumerate(fields):
+            ens.upload(m, f)
+        t0 = time.perf_counter()
+        ms = timer.time(ens, lambda: ens.step(mode, dt, steps))
+        wall = time.perf_counter() - t0
+        # field accesses per cell-step: 13 for the stage kernels, 5 fused
+        acc = 5 if "fused" in mode else 13
+        rows.append({"ensemble": mode, "steps": steps, "ms_per_step": ms / steps,
+                     "gcell_steps_s": cells * steps / (ms * 1e-3) / 1e9, "wall_s": wall, "accesses": acc,
+                     "hbm_frac": cells * acc * esz * steps / (ms * 1e-3) / 8e12,
+                     "equivalent_13_access_frac": cells * 13 * esz * steps / (ms * 1e-3) / 8e12})
+    return rows
+
+
+def gains(rows):
+    """Batching gain (one persistent launch per stage or pair for all members
+    against one launch per member, same kernels) apart from the fusion gain
+    (fused against stage kernels, one launch per member each)."""
+    r = {x["ensemble"]: x["gcell_steps_s"] for x in rows}
+    out = {}
+    if "per_member" in r:
+        out.update({f"{k}_vs_per_member": r[k] / r["per_member"] for k in ("batched", "graph", "grouped") if k in r})
+    if "per_member_fused" in r:
+        out.update({f"{k}_vs_per_member_fused": r[k] / r["per_member_fused"]
+                    for k in ("fused", "fused_graph") if k in r})
+        if "per_member" in r:
+            out["fusion_per_member_fused_vs_per_member"] = r["per_member_fused"] / r["per_member"]
+    return out
+
+
 def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
     members, fields = ensemble_config(n_members, storage=storage)
     dt = 0.005 + 0j
-    timer = Timer()
     with Ensemble(members) as ens:
         cells = ens.cells
         print(json.dumps({"ensemble": "config", "members": n_members, "cells": cells,
@@ -186,30 +228,17 @@ def run_study(n_members: int, steps: int, tune: bool, storage: str = "c128"):
             tx, ty, tzm, log = autotune(ens, dt)
             print(json.dumps({"ensemble": "autotune", "tx": tx, "ty": ty,
                               "tz_by_size": {str(m.nx): t for m, t in zip(members, tzm)}, "log": log}), flush=True)
-        rows = []
         print(json.dumps({"ensemble": "groups", "count": ens.num_groups()}), flush=True)
-        modes = ["per_member", "batched", "graph", "grouped"] + (["fused", "fused_graph"] if storage == "c128" else [])
-        for mode in modes:
-            for m, f in enumerate(fields):
-                ens.upload(m, f)
-            ens.step(mode, dt, 2)  # warm-up (and graph capture)
-            for m, f in enumerate(fields):
-                ens.upload(m, f)
-            t0 = time.perf_counter()
-            ms = timer.time(ens, lambda: ens.step(mode, dt, steps))
-            wall = time.perf_counter() - t0
-            # field accesses per cell-step: 13 for the stage kernels, 5 fused
-            acc = 5 if mode.startswith("fused") else 13
-            esz = 16 if storage == "c128" else 8
-            row = {"ensemble": mode, "steps": steps, "ms_per_step": ms / steps,
-                   "gcell_steps_s": cells * steps / (ms * 1e-3) / 1e9, "wall_s": wall, "accesses": acc,
-                   "hbm_frac": cells * acc * esz * steps / (ms * 1e-3) / 8e12,
-                   "equivalent_13_access_frac": cells * 13 * esz * steps / (ms * 1e-3) / 8e12}
-            rows.append(row)
+        modes = ["per_member", "batched", "graph", "grouped"]
+        if storage == "c128":
+            modes += ["per_member_fused", "fused", "fused_graph"]
+        rows = time_modes(ens, fields, modes, steps, dt)
+        for row in rows:
             print(json.dumps(row), flush=True)
         base = rows[0]["gcell_steps_s"]
         print(json.dumps({"ensemble": "speedup_vs_per_member",
                           **{r["ensemble"]: r["gcell_steps_s"] / base for r in rows}}), flush=True)
+        print(json.dumps({"ensemble": "gains", **gains(rows)}), flush=True)
     return rows
 
 

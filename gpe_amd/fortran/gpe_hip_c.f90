@@ -8,6 +8,7 @@ module gpe_hip_c
   integer(c_int), parameter :: GPE_OK = 0
   integer(c_int), parameter :: GPE_C128 = 0, GPE_C64 = 1, GPE_MIXED = 2
   integer(c_int), parameter :: GPE_HOST_C128 = 0, GPE_HOST_C64 = 1
+  integer(c_int), parameter :: GPE_FIELD_STATE = 0, GPE_FIELD_AUX = 1
   integer(c_int), parameter :: GPE_EULER = 0, GPE_RK2 = 1, GPE_RK4 = 2, GPE_RK45 = 3
   integer(c_int), parameter :: GPE_D_X = 0, GPE_D_Y = 1, GPE_D_Z = 2, GPE_D_XX = 3, &
                                GPE_D_YY = 4, GPE_D_ZZ = 5, GPE_D_LAP = 6
@@ -60,6 +61,28 @@ module gpe_hip_c
       integer(c_int), value :: host_dtype
       integer(c_int64_t), value :: ny_alloc, y_off, z_off
     end function gpe_upload
+
+    integer(c_int) function gpe_field_upload(ctx, field, host, host_dtype, ny_alloc, y_off, z_off) &
+        bind(C, name='gpe_field_upload')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+      type(*), dimension(*), intent(in) :: host
+      integer(c_int), value :: host_dtype
+      integer(c_int64_t), value :: ny_alloc, y_off, z_off
+    end function gpe_field_upload
+
+    integer(c_int) function gpe_select_field(ctx, field) bind(C, name='gpe_select_field')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: field
+    end function gpe_select_field
+
+    integer(c_int) function gpe_set_real_time(ctx, real_time) bind(C, name='gpe_set_real_time')
+      import
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: real_time
+    end function gpe_set_real_time
 
     integer(c_int) function gpe_download(ctx, host, host_dtype) bind(C, name='gpe_download')
       import

@@ -1,7 +1,7 @@
 ! Fortran host program of the GPE time-stepping path on one MI355X.  It
 ! follows the reference driver (src/gpe.f90:33-312): run.in namelists, the
-! dt convention (76-80), the initial state read from end_state00.dat in the
-! reference's restart format (src/ic.f90:117-186), and the main loop
+! dt convention (76-80), the initial state (with restart the checkpoint
+! end_state00.dat, src/ic.f90:117-186, else ic_state.dat), and the main loop
 ! `do while (t+im_t <= end_time)` (134-278) in the reference's order:
 !   * the RUNNING sentinel (136-159);
 !   * every save_rate steps (modulo(t+im_t, |dt| save_rate) < |dt|) the
@@ -13,15 +13,17 @@
 !     vcf/vcf<p>.dat, src/io.f90:222-311);
 !   * at save_rate2 intervals the 3D state with save_3d (idl_surface,
 !     proc00/dens<p>.dat, src/io.f90:489-529);
-!   * at p_save intervals the periodic end_state.dat and save.dat
-!     (src/io.f90:840-891);
+!   * at p_save intervals the periodic checkpoint proc00/end_state.dat and
+!     save.dat (src/io.f90:840-891);
 !   * the solver step, then the SAVE sentinel (save_run, src/io.f90:1566-1595);
 !   * for eqn 4 the norm, norm.dat, the imaginary-time switch, renorm and
 !     imprint_vl (gpe.f90:241-271).
 ! condensed_particles (mass.dat; with save_spectrum spectrum/, idf/ and
 ! eta_time.dat, src/io.f90:315-404, 632-725) runs before the loop and at
-! save_rate3 intervals, as in the reference.  After the loop: end_state.dat
-! (RUNNING removed) and, with save_3d, the final dens file.  p is written as
+! save_rate3 intervals, as in the reference.  After the loop (at end_time,
+! or when RUNNING was removed): proc00/end_state.dat (RUNNING removed) and,
+! with save_3d, the final dens file; a restart reads that checkpoint back as
+! end_state00.dat (scripts/run.sh -r copies it so).  p is written as
 ! seven digits, as itos forms it.  Not built: zeros, contours, averages and
 ! filtered surfaces.
 !
@@ -41,7 +43,8 @@ program gpe_hip_main
   type(var) :: psi
   real (pr) :: norm, prev_norm, relnorm
   logical :: state_exist, running, resident
-  integer :: u, ios, nx_prev, ny_prev, nz_prev, nsteps, ps, n, m
+  integer :: u, ios, nx_prev, ny_prev, nz_prev, nsteps, ps, n, m, stop_at_p
+  character(32) :: envv
   logical :: dirs_made(4) = .false.
   integer(8) :: c0, c1, crate
   complex (pr) :: dt_prev
@@ -84,40 +87,42 @@ program gpe_hip_main
   open (newunit=u, file='RUNNING')
   close (u)
 
-  ! initial state: the reference's restart route (src/ic.f90:92-101, 117-186)
-  if (.not. restart) call emergency_stop('ERROR: this host reads its initial &
-    &state from end_state00.dat; set restart=.true.')
-  inquire (file='end_state00.dat', exist=state_exist)
-  if (.not. state_exist) call emergency_stop('ERROR: restart=.true. &
-    &but end_state00.dat does not exist.')
-  open (newunit=u, file='end_state00.dat', access='stream', status='old', action='read')
-  read (u) nx_prev
-  read (u) ny_prev
-  read (u) nz_prev
-  read (u) p
-  read (u) t, im_t
-  read (u) dt_prev
-  if (nx_prev /= nx .or. ny_prev /= ny .or. nz_prev /= nz) &
-    call emergency_stop('ERROR: end_state00.dat grid does not match grid_params')
-  read (u) psi%new
-  close (u)
-  if (real_time) then
-    if (real(dt_prev, pr) == 0.0_pr) then
-      dt_prev = cmplx(-aimag(dt_prev), real(dt_prev, pr), pr)
-      im_t = 0.0_pr
+  ! initial state (ics, src/ic.f90:84-113).  restart: the checkpoint
+  ! end_state00.dat as state_restart reads it (src/ic.f90:117-186), times
+  ! proc00/end_state_filtered.dat with saved_restart (143-159).  Otherwise
+  ! the state in ic_state.dat (same layout; its counters are not used): the
+  ! reference composes its initial state from the compile-time ic.in, which
+  ! this host receives as a file written by the IC generators.
+  if (restart) then
+    inquire (file='end_state00.dat', exist=state_exist)
+    if (.not. state_exist) call emergency_stop('ERROR: restart=.true. &
+      &but end_state00.dat does not exist.')
+    print *, 'Getting restart conditions'
+    call read_state('end_state00.dat', .true.)
+    if (saved_restart) call apply_filtered('proc00/end_state_filtered.dat')
+    if (real_time) then
+      if (real(dt_prev, pr) == 0.0_pr) then
+        dt_prev = cmplx(-aimag(dt_prev), real(dt_prev, pr), pr)
+        im_t = 0.0_pr
+      end if
+    else
+      if (aimag(dt_prev) == 0.0_pr) then
+        dt_prev = cmplx(aimag(dt_prev), -real(dt_prev, pr), pr)
+        t = 0.0_pr
+      end if
     end if
+    select case (scheme)
+      case ('rk45')
+        dt = dt_prev
+      case default
+        if (dt_prev /= dt) p = int(p * dt_prev / dt)
+    end select
   else
-    if (aimag(dt_prev) == 0.0_pr) then
-      dt_prev = cmplx(aimag(dt_prev), -real(dt_prev, pr), pr)
-      t = 0.0_pr
-    end if
+    inquire (file='ic_state.dat', exist=state_exist)
+    if (.not. state_exist) call emergency_stop('ERROR: restart=.false. &
+      &but the initial state ic_state.dat does not exist.')
+    call read_state('ic_state.dat', .false.)
   end if
-  select case (scheme)
-    case ('rk45')
-      dt = dt_prev
-    case default
-      if (dt_prev /= dt) p = int(p * dt_prev / dt)
-  end select
 
   relnorm = 0.0_pr
   psi%old = 0.0_pr
@@ -137,14 +142,30 @@ program gpe_hip_main
   end if
   norm = prev_norm
   call condensed_particles(t)
+  ! src/gpe.f90:116-124
+  if (.not. restart) then
+    inquire (file='end_state00.dat', exist=state_exist)
+    if (state_exist) call emergency_stop('ERROR: restart=.false. but end_state00.dat exists.')
+  end if
 
   ps = int(t/p_save)
   n = int(t/save_rate2)
   m = int(t/save_rate3)
 
+  ! GPE_REMOVE_RUNNING_AT_P=k: the host removes RUNNING itself at the top of
+  ! iteration p = k, as an operator would, so that the stop and checkpoint
+  ! path runs at a chosen step (tests)
+  stop_at_p = -1
+  call get_environment_variable('GPE_REMOVE_RUNNING_AT_P', envv, status=ios)
+  if (ios == 0) read (envv, *, iostat=ios) stop_at_p
+
   nsteps = 0
   call system_clock(c0, crate)
   do while (t+im_t <= end_time)
+    if (p == stop_at_p) then
+      open (newunit=u, file='RUNNING')
+      close (u, status='delete')
+    end if
     inquire (file='RUNNING', exist=running)
     if (.not. running) then
       print *, 'Stop requested.'
@@ -229,6 +250,54 @@ program gpe_hip_main
   call close_ctx()
 
 contains
+
+  ! A state file in the end_state layout (int32 nx, ny, nz, p; t, im_t;
+  ! complex dt; psi, x fastest: src/io.f90:852-862).  counters: take p, t,
+  ! im_t and dt_prev from it (state_restart, src/ic.f90:130-141).
+  subroutine read_state(fname, counters)
+    character(*), intent(in) :: fname
+    logical, intent(in) :: counters
+    integer :: p_file
+    real (pr) :: t_file, im_t_file
+    open (newunit=u, file=fname, access='stream', status='old', action='read')
+    read (u) nx_prev
+    read (u) ny_prev
+    read (u) nz_prev
+    read (u) p_file
+    read (u) t_file, im_t_file
+    read (u) dt_prev
+    if (nx_prev /= nx .or. ny_prev /= ny .or. nz_prev /= nz) &
+      call emergency_stop('ERROR: '//fname//' grid does not match grid_params')
+    read (u) psi%new
+    close (u)
+    if (counters) then
+      p = p_file
+      t = t_file
+      im_t = im_t_file
+    end if
+  end subroutine read_state
+
+  ! saved_restart: psi times the filtered state of the previous run
+  ! (src/ic.f90:143-159; the file holds 4 integers and 2 reals before it)
+  subroutine apply_filtered(fname)
+    character(*), intent(in) :: fname
+    integer :: undef_int, i
+    real (pr) :: undef_real
+    complex (pr), allocatable :: filt(:,:,:)
+    inquire (file=fname, exist=state_exist)
+    if (.not. state_exist) call emergency_stop('ERROR: saved_restart=.true. but '//fname//' does not exist.')
+    allocate(filt(0:nx1,js:je,ks:ke))
+    open (newunit=u, file=fname, access='stream', status='old', action='read')
+    do i = 1, 4
+      read (u) undef_int
+    end do
+    read (u) undef_real
+    read (u) undef_real
+    read (u) filt
+    close (u)
+    psi%new = psi%new*filt
+    deallocate(filt)
+  end subroutine apply_filtered
 
   ! p as seven digits (itos, src/io.f90:29-47)
   character(7) function itos(k)
@@ -396,7 +465,8 @@ contains
   subroutine end_state(flag)
     integer, intent(in) :: flag
     call on_host()
-    open (newunit=u, file='end_state.dat', access='stream', status='replace')
+    call make_dir(3, 'proc00')
+    open (newunit=u, file='proc00/end_state.dat', access='stream', status='replace')
     write (u) nx
     write (u) ny
     write (u) nz

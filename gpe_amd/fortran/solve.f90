@@ -35,6 +35,12 @@ contains
     end select
   end function scheme_code
 
+  ! real_time changes when imaginary-time relaxation converges
+  ! (src/gpe.f90:251-271); rkqs reads it on a retry (src/solve.f90:195-201)
+  subroutine pass_real_time()
+    call check(gpe_set_real_time(the_ctx(), merge(1_c_int, 0_c_int, real_time)), 'solver')
+  end subroutine pass_real_time
+
   subroutine io_from_state(io)
     type(gpe_step_info), intent(out) :: io
     io%t = t
@@ -72,6 +78,7 @@ contains
     integer(c_int64_t) :: nya, yo, zo
     code = scheme_code()
     call io_from_state(io)
+    call pass_real_time()
     call halo_args(nya, yo, zo)
     call check(gpe_solver(the_ctx(), code, var_in, nya, yo, zo, var_out, host_kind(), io), 'solver')
     call state_from_io(io)
@@ -83,6 +90,7 @@ contains
     integer(c_int) :: code
     code = scheme_code()
     call io_from_state(io)
+    call pass_real_time()
     call check(gpe_step(the_ctx(), code, int(nsteps, c_int), io), 'solver')
     call state_from_io(io)
   end subroutine solver_resident

@@ -33,11 +33,20 @@ contains
     call deriv_lap(in_var, laplacian)
   end function laplacian
 
+  ! The host-array routines work on the context's auxiliary field: the
+  ! array handed in is uploaded there and selected (put_interior, put_halo)
+  ! until use_state, so a stale or auxiliary array (the reference's
+  ! save_energy(t, psi%old)) never replaces the device-resident state.
   subroutine put_interior(in_var)
     complex (pr), dimension(0:nx1,js:je,ks:ke), intent(in) :: in_var
-    call check(gpe_upload(the_ctx(), in_var, host_kind(), int(je-js+1, c_int64_t), 0_c_int64_t, 0_c_int64_t), &
-               'upload')
+    call check(gpe_field_upload(the_ctx(), GPE_FIELD_AUX, in_var, host_kind(), int(je-js+1, c_int64_t), &
+               0_c_int64_t, 0_c_int64_t), 'upload')
+    call check(gpe_select_field(the_ctx(), GPE_FIELD_AUX), 'select')
   end subroutine put_interior
+
+  subroutine use_state()
+    call check(gpe_select_field(the_ctx(), GPE_FIELD_STATE), 'select')
+  end subroutine use_state
 
   subroutine get_norm(in_var, norm)
     complex (pr), dimension(0:nx1,js:je,ks:ke), intent(in) :: in_var
@@ -45,6 +54,7 @@ contains
     real(c_double) :: v
     call put_interior(in_var)
     call check(gpe_norm(the_ctx(), v), 'get_norm')
+    call use_state()
     norm = v
   end subroutine get_norm
 
@@ -52,10 +62,9 @@ contains
     complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
     real (pr), intent(inout) :: E
     real(c_double) :: v
-    integer(c_int64_t) :: nya, yo, zo
-    call halo_args(nya, yo, zo)
-    call check(gpe_upload(the_ctx(), in_var, host_kind(), nya, yo, zo), 'upload')
+    call put_halo(in_var)
     call check(gpe_energy(the_ctx(), v), 'energy')
+    call use_state()
     E = v
   end subroutine energy
 
@@ -63,7 +72,8 @@ contains
     complex (pr), dimension(0:nx1,js-2:je+2,ks-2:ke+2), intent(in) :: in_var
     integer(c_int64_t) :: nya, yo, zo
     call halo_args(nya, yo, zo)
-    call check(gpe_upload(the_ctx(), in_var, host_kind(), nya, yo, zo), 'upload')
+    call check(gpe_field_upload(the_ctx(), GPE_FIELD_AUX, in_var, host_kind(), nya, yo, zo), 'upload')
+    call check(gpe_select_field(the_ctx(), GPE_FIELD_AUX), 'select')
   end subroutine put_halo
 
   subroutine momentum(in_var, mom)
@@ -72,6 +82,7 @@ contains
     real(c_double) :: m(3)
     call put_halo(in_var)
     call check(gpe_momentum(the_ctx(), m), 'momentum')
+    call use_state()
     mom = m
   end subroutine momentum
 
@@ -81,6 +92,7 @@ contains
     real(c_double) :: fd(0:nx1)
     call put_halo(in_var)
     call check(gpe_vcf(the_ctx(), fd), 'get_vcf')
+    call use_state()
     f = fd
   end subroutine get_vcf
 
@@ -93,6 +105,7 @@ contains
     real(c_double) :: mx(3), mn(3), sd(3), pdf(nb, 3)
     call put_halo(in_var)
     call check(gpe_velocity_pdf(the_ctx(), int(nb, c_int), cnt, mn, sd, mx, pdf), 'velocity_pdf')
+    call use_state()
     pdf_vx = pdf(:, 1)
     pdf_vy = pdf(:, 2)
     pdf_vz = pdf(:, 3)
@@ -107,6 +120,7 @@ contains
     real(c_double) :: v
     call put_interior(in_var)
     call check(gpe_mass(the_ctx(), v), 'mass')
+    call use_state()
     M = v
   end subroutine mass
 
@@ -116,6 +130,7 @@ contains
     call put_interior(var)
     call check(gpe_renormalise(the_ctx(), real(norm, c_double)), 'renormalise')
     call check(gpe_download(the_ctx(), var, host_kind()), 'download')
+    call use_state()
   end subroutine renormalise
 
   ! norm and renormalisation of the device-resident state (no host copy)
@@ -138,6 +153,7 @@ contains
     real(c_double) :: v
     call put_halo(in_var)
     call check(gpe_linelength(the_ctx(), v), 'linelength')
+    call use_state()
     linelength = v
   end function linelength
 
@@ -148,6 +164,7 @@ contains
     call put_interior(in_var)
     call state_imprint_vortex_line()
     call check(gpe_download(the_ctx(), imprint_vortex_line, host_kind()), 'download')
+    call use_state()
   end function imprint_vortex_line
 
   subroutine state_imprint_vortex_line()

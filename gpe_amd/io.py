@@ -2,9 +2,11 @@
 
 * end_state restart file (src/io.f90:840-891, read back by state_restart
   src/ic.f90:117-186): stream-unformatted int32 nx, ny, nz, p; float64 t,
-  im_t; complex128 dt; complex128 psi (x fastest).  Used to hand initial
-  states to the Fortran host (and to the reference build) and to read their
-  final states back.
+  im_t; complex128 dt; complex128 psi (x fastest).  The Fortran host writes
+  its checkpoints to proc00/end_state.dat as the reference does (io.f90:852)
+  and restarts from end_state00.dat (the name run.sh -r gives the copy); a
+  fresh run (restart = .false.) reads its initial state from ic_state.dat in
+  the same layout.
 * dens<p>.dat 3D saves of idl_surface (src/io.f90:489-529): float64 t+im_t;
   int32 nx, ny, nz, nyprocs, nzprocs, js, je, ks, ke; complex128 psi of the
   rank's block; float64 x, y, z.

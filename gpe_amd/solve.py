@@ -19,6 +19,7 @@ def solver(var_in: np.ndarray, var_out: np.ndarray) -> None:
     if P.run.scheme not in _lib.SCHEMES:
         raise _lib.GpeError("ERROR: Unrecognised time stepping scheme.")
     info = StepInfo(P.dt, P.t, P.im_t)
+    ctx.set_real_time(P.real_time)  # rkqs reads it on a retry (src/solve.f90:195-201)
     ctx.upload(np.ascontiguousarray(var_in))
     ctx.step(P.run.scheme, 1, info)
     var_out[...] = ctx.download(var_out.dtype)

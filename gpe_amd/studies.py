@@ -141,6 +141,72 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
     return rows
 
 
+def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 0.8, max_runs: int = 5):
+    """Tolerance of a method tuned so that its error at T lands at or just
+    under `target`: a secant search in log(eps) on log(error), starting from
+    error ~ eps^slope.  run(eps) -> (stats, error).  Returns (eps, stats,
+    error, tries) of the largest tolerance tried whose error is within the
+    target (tries: every (eps, error, wall_s) evaluated)."""
+    import math
+
+    tries = []
+    e_prev = None
+    eps = eps0
+    best = None
+    for k in range(max_runs):
+        st, err = run(eps)
+        tries.append((eps, err, st.wall_s))
+        if err <= target and (best is None or eps > best[0]):
+            best = (eps, st, err)
+        if best is not None and target / 1.3 <= err <= target:
+            break
+        le, lr = math.log(eps), math.log(err)
+        if e_prev is not None and abs(le - e_prev[0]) > 1e-9:
+            s = (lr - e_prev[1]) / (le - e_prev[0])
+            if best is None and s < 0.15 and le < e_prev[0]:
+                break  # the error no longer follows the tolerance: a rounding floor above the target
+            slope = min(max(s, 0.3), 2.0)
+        e_prev = (le, lr)
+        # aim a little under the target so the landing run stays within it
+        eps = math.exp(le + (math.log(0.85 * target) - lr) / slope)
+    if best is None:
+        return None, None, None, tries
+    return best[0], best[1], best[2], tries
+
+
+def time_to_t_tuned(p, psi, T: float, dt0: float, methods, target: float = 1e-6, oracle_eps: float = 1e-10,
+                    baseline: str = "plain-c128"):
+    """Wall time to T at a matched error: the FP64 reference at oracle_eps,
+    then each method (name, variant spec, controller factory or None) with
+    its tolerance tuned to land at `target` relative L2 against it.  The
+    speed-up is against `baseline` (the fastest plain complex128 RKF45: the
+    reference's controller with the fused attempt)."""
+    ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
+    out = {"oracle": dict(method=ref.method, eps=oracle_eps, wall_s=ref.wall_s, steps=ref.n_accept,
+                          rejects=ref.n_reject, rhs=ref.n_rhs)}
+    for name, spec, make_ctrl in methods:
+        storage, fused = variant(spec)
+
+        def run(eps):
+            ctrl = make_ctrl(eps) if make_ctrl else None
+            st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy=ctrl is not None, fused=fused)
+            return st, rel_l2(fin.astype(np.complex128), ref_final)
+
+        eps, st, err, tries = tune_to_error(run, target)
+        tr = [dict(eps=e, rel_l2=r, wall_s=w) for e, r, w in tries]
+        if st is None:  # the error target is out of the method's reach (complex64 rounding)
+            e, r, w = min(tries, key=lambda x: x[1])
+            out[name] = dict(storage=spec, eps=e, wall_s=w, rel_l2_vs_oracle=r, within_target=False, tuning=tr)
+        else:
+            out[name] = dict(storage=spec, eps=eps, wall_s=st.wall_s, steps=st.n_accept, rejects=st.n_reject,
+                             rhs=st.n_rhs, rel_l2_vs_oracle=err, within_target=True, tuning=tr)
+    base = out.get(baseline)
+    for name, r in out.items():
+        if name != "oracle" and base:
+            r["speedup"] = base["wall_s"] / r["wall_s"] if r["within_target"] else None
+    return out
+
+
 def attempts(n: int, count: int, storages, copy_mib: int = 0):
     """`count` RKF45 steps (rkqs: attempts until one is accepted) of the
     random-phase workload at tolerance 1e-6 for each storage, with the

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define GPE_ABI_VERSION 1
+#define GPE_ABI_VERSION 2
 
 /* error codes */
 #define GPE_OK 0
@@ -131,8 +131,39 @@ int gpe_set_fused(gpe_ctx* ctx, int on);
  * halo array var(0:nx1, js-2:je+2, ks-2:ke+2) with one rank.  Replaces the
  * main-loop copy psi%old = psi%new and ghost fill (src/gpe.f90:165-174). */
 int gpe_upload(gpe_ctx* ctx, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off, int64_t z_off);
-/* Copy the device state into an interior host array (nx*ny*nz values). */
+/* Copy the selected field (the state unless gpe_select_field chose the
+ * auxiliary field) into an interior host array (nx*ny*nz values). */
 int gpe_download(gpe_ctx* ctx, void* host, int host_dtype);
+
+/* Fields of a context: the state, which every stepper advances, and an
+ * auxiliary field that receives host arrays handed to the reference's
+ * diagnostics (get_norm(in_var, norm), energy(in_var, E), mass, momentum,
+ * get_vcf, linelength, renormalise(var, norm), imprint_vortex_line(in_var):
+ * src/variables.f90:537-611, 615-694, 467-507, 698-709, 879-1162), so that
+ * passing a stale or auxiliary array (the reference's save_energy(t,
+ * psi%old)) never replaces the device-resident state. */
+#define GPE_FIELD_STATE 0
+#define GPE_FIELD_AUX 1
+/* Upload into `field` (layout as gpe_upload); GPE_FIELD_STATE is gpe_upload. */
+int gpe_field_upload(gpe_ctx* ctx, int field, const void* host, int host_dtype, int64_t ny_alloc, int64_t y_off,
+                     int64_t z_off);
+/* The field that gpe_norm, gpe_mass, gpe_energy, gpe_hamiltonian,
+ * gpe_momentum, gpe_velocity_pdf, gpe_vcf, gpe_linelength,
+ * gpe_condensed_particles, gpe_peek, gpe_renormalise, gpe_imprint_line and
+ * gpe_download act on until the next call (GPE_FIELD_STATE at creation).
+ * gpe_step, gpe_solver, gpe_rk45_*, gpe_upload and gpe_deriv/gpe_rhs with
+ * in == NULL always use the state.  GPE_FIELD_AUX needs a prior
+ * gpe_field_upload of it (GPE_ESTATE otherwise). */
+int gpe_select_field(gpe_ctx* ctx, int field);
+
+/* The reference's module variable real_time (src/parameters.f90:26), which
+ * the driver sets to .true. when imaginary-time relaxation converges
+ * (src/gpe.f90:251-271): rkqs picks the direction of a retried step from it
+ * (dt = (0, -|dt_temp|) in imaginary time, (|dt_temp|, 0) in real time,
+ * src/solve.f90:195-201).  gpe_ctx_create takes it from gpe_params; a host
+ * whose real_time changes during the run passes the current value before
+ * each gpe_step or gpe_solver call. */
+int gpe_set_real_time(gpe_ctx* ctx, int real_time);
 
 /* Advance the device state.  scheme GPE_EULER/RK2/RK4: nsteps fixed steps
  * of dt = (io->dt_re, io->dt_im) (src/solve.f90:57-154).  GPE_RK45: nsteps
@@ -271,6 +302,8 @@ int gpe_bench_stream5(void* c, void* d, const void* a, const void* b, const void
 #define GPE_ENS_FUSED 4       /* fused RK4 (two persistent launches per step) for all members;
                                  complex128 members with the plain right-hand side */
 #define GPE_ENS_FUSED_GRAPH 5 /* GPE_ENS_FUSED captured in a hipGraph and replayed */
+#define GPE_ENS_PER_MEMBER_FUSED 6 /* the single-context fused RK4 (two launches) once per member:
+                                      the baseline of the fused batched modes */
 typedef struct gpe_ens gpe_ens;
 int gpe_ens_create(const gpe_params* members, int n, gpe_ens** out);
 void gpe_ens_destroy(gpe_ens* ens);

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert {n for n, _, _ in _lib.PROTOTYPES} == set(names)
-    assert lib.gpe_abi_version() == 1
+    assert lib.gpe_abi_version() == _lib.GPE_ABI_VERSION == 2
 
 
 def test_kernel_kind_names_match_header_count():

@@ -52,7 +52,7 @@ def test_create_rejects_mixed_members():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
-@pytest.mark.parametrize("mode", ["per_member", "batched", "graph", "grouped", "fused", "fused_graph"])
+@pytest.mark.parametrize("mode", ["per_member", "batched", "graph", "grouped", "fused", "fused_graph", "per_member_fused"])
 def test_modes_match_single_context(order, bcs, mode):
     ms = members(order, bcs)
     fs = fields(ms)

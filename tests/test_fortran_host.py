@@ -1,7 +1,7 @@
 """The Fortran host (gpe_amd/fortran): the reference-compatible modules
 solve/derivs/variables over the ISO_C_BINDING layer, and a driver with the
 reference's main loop.  GPU tests run the driver on the inputs of the
-reference-build fixtures and compare its end_state.dat with the reference's
+reference-build fixtures and compare its checkpoint proc00/end_state.dat with the reference's
 final state; the CPU test checks the error convention (emergency_stop writes
 ERROR and removes RUNNING) when no device is available."""
 import json
@@ -65,7 +65,7 @@ def test_host_matches_reference_build(tmp_path, name, mode):
     r = run_host(tmp_path, p, psi, nsteps, mode)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    out, pp, t, im_t, dt = read_end_state(str(tmp_path / "end_state.dat"))
+    out, pp, t, im_t, dt = read_end_state(str(tmp_path / "proc00" / "end_state.dat"))
     assert pp == nsteps
     err = float(np.linalg.norm(out - z["out"]) / np.linalg.norm(z["out"]))
     print(f"Fortran host ({mode}) vs reference build, {name}: rel L2 = {err:.3e}")
@@ -201,7 +201,7 @@ def test_host_save_schedule_and_files(tmp_path, mode):
     and psi_time.dat every save_rate steps (modulo(t, |dt| save_rate) < |dt|
     on the accumulated t), condensed_particles (mass.dat, spectra) before the
     loop and with the velocity PDF / VCF at save_rate3 intervals,
-    dens files at save_rate2 intervals and at the end, end_state.dat and
+    dens files at save_rate2 intervals and at the end, proc00/end_state.dat and
     save.dat at p_save intervals and at the end.  Every value is checked
     against the same run driven step by step through the C ABI."""
     p = RunParams(nx=24, ny=20, nz=18, xr=8.0, yr=7.0, zr=6.5, order=4, bcs=1, eqn_to_solve=1, scheme="rk4", tau=0.01)
@@ -268,7 +268,7 @@ def test_host_save_schedule_and_files(tmp_path, mode):
         assert np.array_equal(idf[:, 0], np.arange(1, p.nx // 2 + 1))
         assert np.allclose(idf[:, 1], c["F"], rtol=1e-8, atol=1e-300)
     # dens files: save_rate2 schedule plus the final state
-    dens = sorted(os.listdir(tmp_path / "proc00"))
+    dens = sorted(f for f in os.listdir(tmp_path / "proc00") if f.startswith("dens"))
     want = sorted({f"dens{pp:07d}.dat" for pp, _, _ in dens_ps} | {f"dens{nsteps:07d}.dat"})
     assert dens == want
     for pp, t, a in dens_ps:
@@ -282,7 +282,7 @@ def test_host_save_schedule_and_files(tmp_path, mode):
     # save_rate2 ones (the first at p = 0 again) and the final one
     assert [int(v) for v in open(tmp_path / "p_saved.dat").read().split()] == \
         [0] + [pp for pp, _, _ in dens_ps] + [nsteps]
-    out, pp, t, _, _ = read_end_state(str(tmp_path / "end_state.dat"))
+    out, pp, t, _, _ = read_end_state(str(tmp_path / "proc00" / "end_state.dat"))
     assert pp == nsteps and abs(t - t_end) <= 1e-15 and np.abs(out - final).max() <= 1e-13
     assert "Periodically saved state" in (tmp_path / "save.dat").read_text()
     assert not (tmp_path / "RUNNING").exists()
@@ -301,7 +301,7 @@ def test_host_save_sentinel(tmp_path):
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     assert not (tmp_path / "SAVE").exists()
-    assert os.listdir(tmp_path / "proc00") == ["dens0000000.dat"]
+    assert sorted(os.listdir(tmp_path / "proc00")) == ["dens0000000.dat", "end_state.dat"]
     with Context(p) as ctx:
         ctx.upload(psi)
         ctx.step("rk4", 1, StepInfo(p.dt0()))
@@ -346,7 +346,7 @@ def test_host_imaginary_time_with_imprint(tmp_path, mode):
             ctx.imprint_line(vl)
             prev = nrm
         final = ctx.download()
-    out, pp, t, im_t, _ = read_end_state(str(tmp_path / "end_state.dat"))
+    out, pp, t, im_t, _ = read_end_state(str(tmp_path / "proc00" / "end_state.dat"))
     assert pp == nsteps and abs(im_t - nsteps * p.tau) <= 1e-15
     err = np.abs(out - final).max() / np.abs(final).max()
     print(f"imprint_vl ({mode}): max rel difference {err:.2e}")
