@@ -481,7 +481,7 @@ template <typename R, int ORD> static int step_rk4_fused(gpe_ctx* c, double dre,
     const StW<R> w = make_w<R>(c->p);
     const C<R> dt = cdt<R>(dre, dim, 1.0);
     Geom g = make_geom<R>(c);
-    fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
+    rk4_fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
     const C<R>* psi = (const C<R>*)c->psi;
     C<R>* u2 = (C<R>*)c->uA;
     C<R>* out = (C<R>*)c->pnew;

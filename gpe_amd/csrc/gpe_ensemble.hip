@@ -445,7 +445,7 @@ template <int ORD> int step_member_fused(gpe_ens* e, int m, double dre, double d
     g.nz = p.nz;
     g.bcs = p.bcs;
     g.plane = (long long)p.nx * p.ny;
-    fused_tiles<ORD>(p.nx, p.ny, p.nz, e->ncu, g);
+    rk4_fused_tiles<ORD>(p.nx, p.ny, p.nz, e->ncu, g);
     const StW<double> w = make_w<double>(p);
     const RhsC<double> k = make_rhs_p<double>(p, nullptr);
     const double2 dt = mkc<double>(dre, dim);

@@ -51,21 +51,34 @@ namespace gpe {
 
 constexpr int kFusedTYB = 15;
 
-template <int ORD> struct FusedShape {
+// Block shape: an inner tile OX_ wide, blocks of (OX_ + 2H) x TYB_ threads
+// in NT-thread workgroups (1024: one block per CU; 512: two).
+template <int ORD, int OX_ = 64, int TYB_ = kFusedTYB> struct FusedShape {
     static constexpr int H = ORD / 2;
-    static constexpr int TXB = 64 + 2 * H, TYB = kFusedTYB, NTB = TXB * TYB;
-    static constexpr int OX = 64, OY = TYB - 2 * H;       // output tile
+    static constexpr int TXB = OX_ + 2 * H, TYB = TYB_, NTB = TXB * TYB;
+    static constexpr int NT = NTB <= 512 ? 512 : 1024;
+    static constexpr int OX = OX_, OY = TYB - 2 * H;       // output tile
     static constexpr int LWA = TXB + 2 * H, LHA = TYB + 2 * H;  // LDS plane with halo
     static constexpr int NHALO = 2 * H * TXB + 2 * H * TYB;
-    static constexpr int NSLOT = (NHALO + 1023) / 1024;
+    static constexpr int NSLOT = (NHALO + NT - 1) / NT;
+    static_assert(NTB <= NT, "block shape exceeds the workgroup");
 };
+
+// The shape of the fused RK4 pair: GPE_FUSED_NARROW selects 32-wide inner
+// tiles in 504-thread blocks (two blocks per CU).
+#ifndef GPE_FUSED_NARROW
+#define GPE_FUSED_NARROW 0
+#endif
+constexpr int kRk4OX = GPE_FUSED_NARROW ? 32 : 64;
+constexpr int kRk4TYB = GPE_FUSED_NARROW ? 14 : kFusedTYB;
+template <int ORD> constexpr int kRk4NT = FusedShape<ORD, kRk4OX, kRk4TYB>::NT;
 
 // Per-thread geometry of a fused block.  The thread-to-column map is a
 // per-kernel choice (measured): inner-first saves the rim lanes' share of
 // the second stage, but it moves register allocation, and some kernels then
 // spill or run slower.
-template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
-    using S = FusedShape<ORD>;
+template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> struct FusedCtx {
+    using S = FusedShape<ORD, OX_, TYB_>;
     int z0, z1;
     bool active, inner, store;
     int lx, ly, gx, gy;
@@ -87,7 +100,7 @@ template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
         // Measured at 512^3 against whole rows / 4-wide / 1-wide groups: the
         // Cash-Karp attempt 7.8 against 8.5 / 8.4 / 8.8 ms, RK4 2.64 against
         // 2.70 / 2.69 / 2.71 ms.
-        constexpr int kGroupW = 2;
+        constexpr int kGroupW = 128 / S::OX;
         const int gw = g.ntx % kGroupW == 0 ? kGroupW : g.ntx;
         const int grp = txy / (gw * g.nty), rem = txy % (gw * g.nty);
         const int ox0 = (grp * gw + rem % gw) * S::OX, oy0 = (rem / gw) * S::OY;
@@ -135,7 +148,7 @@ template <int ORD, bool INNER_FIRST = true> struct FusedCtx {
         // halo of the block plane: 2H rows above/below, then 2H columns left/right
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s) {
-            const int e = tid + s * 1024;
+            const int e = tid + s * S::NT;
             hv[s] = e < S::NHALO;
             int hx = ox0 - S::H, hy = oy0 - S::H, lxx = S::H, lyy = S::H;
             if (e < 2 * S::H * S::TXB) {
@@ -247,16 +260,16 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
 template <typename R, int ORD, int LW, int TERMS>
 using FusedSten = std::conditional_t<TERMS == 0, Sten<R, ORD, LW, false>, StenV<R, ORD, LW>>;
 
-template <typename R, int ORD, int TERMS>
+template <typename R, int ORD, int TERMS, int OX_ = 64, int TYB_ = kFusedTYB>
 __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
                                              const C<R>* __restrict__ psi, C<R>* __restrict__ u2out, C<R> hdt,
                                              int bid) {
-    using S = FusedShape<ORD>;
+    using S = FusedShape<ORD, OX_, TYB_>;
     constexpr int H = S::H, D = GPE_FUSED_DEPTH;
     constexpr unsigned E = sizeof(C<R>);
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];
     __shared__ C<R> ldsB[H + 1][S::NTB];
-    const FusedCtx<ORD> c(g, bid);
+    const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
     if (c.z0 >= c.z1) return;  // uniform per block
     const int start = c.z0 - 2 * H;
     // byte offsets within a plane; kNoOff where the lane has no access
@@ -336,17 +349,18 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
 }
 
 template <typename R, int ORD, int TERMS>
-__global__ __launch_bounds__(1024) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ psi,
-                                                  C<R>* __restrict__ u2out, C<R> hdt) {
-    rk4_f12_body<R, ORD, TERMS>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+__global__ __launch_bounds__(kRk4NT<ORD>) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k,
+                                                                             const C<R>* __restrict__ psi,
+                                                                             C<R>* __restrict__ u2out, C<R> hdt) {
+    rk4_f12_body<R, ORD, TERMS, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
-template <typename R, int ORD, int TERMS>
+template <typename R, int ORD, int TERMS, int OX_ = 64, int TYB_ = kFusedTYB>
 __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
                                              const C<R>* __restrict__ u2, const C<R>* __restrict__ psi,
                                              C<R>* __restrict__ out, C<R> dt, C<R> dt3, C<R> dt6, int bid) {
-    using S = FusedShape<ORD>;
+    using S = FusedShape<ORD, OX_, TYB_>;
     // one plane of loads in flight: a second set would not fit beside the
     // three register queues without spilling
     constexpr int H = S::H, NB = H + 1, Q = 2 * H + 1;
@@ -361,7 +375,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     // registers.
     constexpr bool kAccLds = sizeof(R) == 8;
     __shared__ C<R> ldsQ[kAccLds ? NB : 1][kAccLds ? S::OX * S::OY : 1];
-    const FusedCtx<ORD> c(g, bid);
+    const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
     if (c.z0 >= c.z1) return;
     const int start = c.z0 - 2 * H;
     const unsigned pbytes = (unsigned)g.plane * E;
@@ -470,10 +484,11 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 }
 
 template <typename R, int ORD, int TERMS>
-__global__ __launch_bounds__(1024) void k_rk4_f34(Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ u2,
-                                                  const C<R>* __restrict__ psi, C<R>* __restrict__ out,
-                                                  C<R> dt, C<R> dt3, C<R> dt6) {
-    rk4_f34_body<R, ORD, TERMS>(g, w, k, u2, psi, out, dt, dt3, dt6, xcd_remap(blockIdx.x, g.nblocks));
+__global__ __launch_bounds__(kRk4NT<ORD>) void k_rk4_f34(
+    Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ u2, const C<R>* __restrict__ psi, C<R>* __restrict__ out,
+    C<R> dt, C<R> dt3, C<R> dt6) {
+    rk4_f34_body<R, ORD, TERMS, kRk4OX, kRk4TYB>(g, w, k, u2, psi, out, dt, dt3, dt6,
+                                                 xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Fused Cash-Karp attempt (rkck, src/solve.f90:248-378, with the error
@@ -772,7 +787,8 @@ __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __r
 }
 
 // Host side (gpe_k_fused.hip).
-template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);
+template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);      // Cash-Karp pairs, batched
+template <int ORD> int rk4_fused_tiles(int nx, int ny, int nz, int ncu, Geom& g);  // k_rk4_f12 / k_rk4_f34
 template <typename R, int ORD>
 hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
                             C<R> dt, int stage_pair, hipStream_t s);  // g.general: the trap term

@@ -7,8 +7,8 @@ namespace gpe {
 // Output tiles of 64 x (15 - 2H) cells; the z-range is split into C chunks
 // so that the block count fills whole waves of one block per CU, each chunk
 // paying 2H warm-up planes.  Returns the block count.
-template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
-    using S = FusedShape<ORD>;
+template <int ORD, int OX, int TYB> static int fused_tiles_shape(int nx, int ny, int nz, int ncu, Geom& g) {
+    using S = FusedShape<ORD, OX, TYB>;
     g.ntx = (nx + S::OX - 1) / S::OX;
     g.nty = (ny + S::OY - 1) / S::OY;
     const long long tiles = (long long)g.ntx * g.nty;
@@ -30,23 +30,34 @@ template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
     return g.nblocks;
 }
 
+template <int ORD> int fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
+    return fused_tiles_shape<ORD, 64, kFusedTYB>(nx, ny, nz, ncu, g);
+}
+
+// the RK4 pair's shape; a 512-thread block runs two to a CU
+template <int ORD> int rk4_fused_tiles(int nx, int ny, int nz, int ncu, Geom& g) {
+    constexpr int per_cu = FusedShape<ORD, kRk4OX, kRk4TYB>::NT == 512 ? 2 : 1;
+    return fused_tiles_shape<ORD, kRk4OX, kRk4TYB>(nx, ny, nz, ncu * per_cu, g);
+}
+
 template <typename R, int ORD>
 hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
                             C<R> dt, int stage_pair, hipStream_t s) {
     if (g.nblocks <= 0) return hipSuccess;
+    constexpr int NT = FusedShape<ORD, kRk4OX, kRk4TYB>::NT;
     if (stage_pair == 12) {
         const C<R> hdt = mkc<R>(R(0.5) * dt.x, R(0.5) * dt.y);
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f12<R, ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, RHS_V>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
         else
-            hipLaunchKernelGGL((k_rk4_f12<R, ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, 0>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
     } else {
         const C<R> dt3 = mkc<R>(dt.x / R(3), dt.y / R(3)), dt6 = mkc<R>(dt.x / R(6), dt.y / R(6));
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f34<R, ORD, RHS_V>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k,
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, RHS_V>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k,
                                (const C<R>*)u2, psi, out, dt, dt3, dt6);
         else
-            hipLaunchKernelGGL((k_rk4_f34<R, ORD, 0>), dim3(g.nblocks), dim3(1024), 0, s, g, w, k, (const C<R>*)u2,
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, 0>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, (const C<R>*)u2,
                                psi, out, dt, dt3, dt6);
     }
     return hipGetLastError();
@@ -68,6 +79,8 @@ hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total,
 }
 
 template int fused_tiles<2>(int, int, int, int, Geom&);
+template int rk4_fused_tiles<2>(int, int, int, int, Geom&);
+template int rk4_fused_tiles<4>(int, int, int, int, Geom&);
 template int fused_tiles<4>(int, int, int, int, Geom&);
 #define GPE_FUSED_INST(R, ORD)                                                                                     \
     template hipError_t launch_rk4_fused<R, ORD>(const Geom&, const StW<R>&, const RhsC<R>&, const C<R>*, C<R>*, C<R>*, \

@@ -5,6 +5,9 @@
              2000 steps): drift of the norm and of the conserved energy
              (gpe_hamiltonian) for each, and the c64 field's distance from the
              FP64 (c128) run.
+  time_to_t_tuned  RKF45 to a fixed final time with each method's
+             tolerance tuned so its error lands at 1e-6 (the matched-error
+             comparison), against plain complex128 with the fused attempt.
   time_to_t  RKF45 to a fixed final time on the random-phase workload:
              an FP64 reference at a tight tolerance, then plain c128 (six
              stage kernels per attempt, and fused: three kernels), c64 and
@@ -23,6 +26,7 @@
 Each prints one JSON object per line.  Usage:
     python -m gpe_amd.studies drift [--n 256] [--steps 2000]
     python -m gpe_amd.studies time_to_t [--n 512] [--T 20]
+    python -m gpe_amd.studies time_to_t_tuned [--n 512] [--T 20]
     python -m gpe_amd.studies attempts [--n 512] [--attempts 6] [--storages c128/stages,c128,mixed]
     python -m gpe_amd.studies diag [--n 512] [--reps 5]
 """
@@ -182,6 +186,8 @@ def time_to_t_tuned(p, psi, T: float, dt0: float, methods, target: float = 1e-6,
     speed-up is against `baseline` (the fastest plain complex128 RKF45: the
     reference's controller with the fused attempt)."""
     ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
+    print(f"[time_to_t] reference eps {oracle_eps:g}: {ref.n_accept} steps, {ref.wall_s:.2f} s", file=sys.stderr,
+          flush=True)
     out = {"oracle": dict(method=ref.method, eps=oracle_eps, wall_s=ref.wall_s, steps=ref.n_accept,
                           rejects=ref.n_reject, rhs=ref.n_rhs)}
     for name, spec, make_ctrl in methods:
@@ -190,7 +196,10 @@ def time_to_t_tuned(p, psi, T: float, dt0: float, methods, target: float = 1e-6,
         def run(eps):
             ctrl = make_ctrl(eps) if make_ctrl else None
             st, fin = run_rkf45(p, psi, T, dt0, eps, storage, ctrl, energy=ctrl is not None, fused=fused)
-            return st, rel_l2(fin.astype(np.complex128), ref_final)
+            err = rel_l2(fin.astype(np.complex128), ref_final)
+            print(f"[time_to_t] {name} eps {eps:.3g}: {st.n_accept} steps, {st.n_reject} rejects, "
+                  f"{st.wall_s:.2f} s, rel L2 {err:.3g}", file=sys.stderr, flush=True)
+            return st, err
 
         eps, st, err, tries = tune_to_error(run, target)
         tr = [dict(eps=e, rel_l2=r, wall_s=w) for e, r, w in tries]
@@ -270,7 +279,7 @@ def diag(n: int, reps: int):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("study", choices=["drift", "time_to_t", "attempts", "diag"])
+    ap.add_argument("study", choices=["drift", "time_to_t", "time_to_t_tuned", "attempts", "diag"])
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--steps", type=int, default=2000)
@@ -291,6 +300,18 @@ def main(argv=None):
         drift(a.n or 256, a.steps, a.every)
     elif a.study == "diag":
         diag(a.n or 512, a.reps)
+    elif a.study == "time_to_t_tuned":
+        from .learned import DEFAULT_MODEL, LearnedController, load_model
+
+        n = a.n or 512
+        p, psi = ics.config_turb(n=n, storage="c128", seed=0, scheme="rk45", tau=a.dt0)
+        model = load_model(a.model or DEFAULT_MODEL)
+        methods = [("plain-c128", "c128", None), ("plain-c128/stages", "c128/stages", None),
+                   ("plain-mixed", "mixed", None),
+                   ("learned-mixed", "mixed", lambda eps: LearnedController(model, p.replace(eps=eps))),
+                   ("plain-c64", "c64", None)]
+        out = time_to_t_tuned(p, psi, a.T, a.dt0, methods, oracle_eps=a.oracle_eps)
+        emit(dict(study="time_to_t_tuned", n=n, T=a.T, error_target=1e-6, baseline="plain-c128", runs=out))
     elif a.study == "attempts":
         attempts(a.n or 512, a.attempts, tuple((a.storages or "c128/stages,c128,mixed/stages,mixed,c64").split(",")),
                  a.copy_probe_mib)
