@@ -63,6 +63,7 @@ PROTOTYPES = [
     ("gpe_field_upload", _I, [_VP, _I, _VP, _I, _I64, _I64, _I64]),
     ("gpe_select_field", _I, [_VP, _I]),
     ("gpe_set_real_time", _I, [_VP, _I]),
+    ("gpe_energy_noalloc", _I, [ct.POINTER(gpe_params), _VP, _VP, _VP]),
     ("gpe_get_tiles", _I, [_VP, ct.POINTER(_I), ct.POINTER(_I), ct.POINTER(_I)]),
     ("gpe_upload", _I, [_VP, _VP, _I, _I64, _I64, _I64]),
     ("gpe_download", _I, [_VP, _VP, _I]),

@@ -8,6 +8,7 @@
 #include <string>
 #include <utility>
 #include <type_traits>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -862,7 +863,99 @@ static void prof_clear(gpe_ctx* c) {
     c->recs.clear();
 }
 
+// ---------------------------------------------------------------------------
+// gpe_energy without allocation.  The stencil pass of the energy writes one
+// FP64 partial per tile block (the tile-block count grows with the grid);
+// here those partials live in `scratch`, a static device array of
+// kNoAllocPartials doubles (512 KiB, held by each device the library runs
+// on until the process ends).  Host code never uses the array's name as an
+// address: the first call on a device asks hipGetSymbolAddress for that
+// device's copy and keeps the pointer per device.  Every block overwrites
+// its own partial, so nothing needs clearing; calls that overlap on one
+// device share the array and race.
+constexpr int kNoAllocPartials = 65536;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+static double* g_scratch_dev[kNoAllocDevices];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, GPE_EHIP, std::string("gpe_energy_noalloc: hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kNoAllocDevices) return fail(nullptr, GPE_EINVAL, "gpe_energy_noalloc: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(nullptr, GPE_EHIP, std::string("gpe_energy_noalloc: hipGetSymbolAddress(scratch): ") +
+                                               hipGetErrorString(e));
+        g_scratch_dev[dev] = (double*)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+// out = out * a * b * c, in that order (gpe_energy's E * dx * dy * dz)
+__global__ void k_scale_result(double* out, double a, double b, double c) {
+    if (threadIdx.x == 0) out[0] = out[0] * a * b * c;
+}
+
+// The default tiling of the context (gpe_energy) for p and its block count.
+template <typename R> static Geom noalloc_geom(const gpe_params& p, Tiles& t) {
+    Geom g{};
+    g.nx = p.nx;
+    g.ny = p.ny;
+    g.nz = p.nz;
+    g.bcs = p.bcs;
+    g.plane = (long long)p.nx * p.ny;
+    g.general = make_rhs_p<R>(p, nullptr).flags != 0;
+    t = default_tiles(p);
+    tile_geom(t, g);
+    return g;
+}
+
+template <typename R, int ORD>
+static int energy_noalloc(const gpe_params& p, const void* psi, double* scr, double* out, hipStream_t s) {
+    Tiles t;
+    const Geom g = noalloc_geom<R>(p, t);
+    const int nb = g.nblocks;
+    OpEnergy<R> o;
+    hipError_t e = launch_stencil<R, ORD, OpEnergy<R>>(t, g, make_w<R>(p), (const C<R>*)psi, o, scr, s);
+    if (e == hipSuccess) e = launch_reduce_final(RED_SUM, scr, nb, 1, out, s);
+    if (e == hipSuccess) {
+        double d[3];
+        spacing(p, d);
+        hipLaunchKernelGGL(k_scale_result, dim3(1), dim3(64), 0, s, out, d[0], d[1], d[2]);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return fail(nullptr, GPE_EHIP, std::string("gpe_energy_noalloc: ") + hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" {
+
+int gpe_energy_noalloc(const gpe_params* p, const void* psi, double* energy, void* stream) {
+    std::string why;
+    if (validate(p, why)) return fail(nullptr, GPE_EINVAL, "gpe_energy_noalloc: " + why);
+    if (!psi || !energy) return fail(nullptr, GPE_EINVAL, "gpe_energy_noalloc: psi and energy must be device pointers");
+    Tiles t;
+    const int nb = p->storage != GPE_C64 ? noalloc_geom<double>(*p, t).nblocks : noalloc_geom<float>(*p, t).nblocks;
+    if (nb > kNoAllocPartials)
+        return fail(nullptr, GPE_EINVAL, "gpe_energy_noalloc: the grid needs " + std::to_string(nb) +
+                                             " block partials; the static scratch holds " +
+                                             std::to_string(kNoAllocPartials));
+    double* scr = nullptr;
+    int rc = noalloc_scratch(&scr);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    if (p->storage != GPE_C64)
+        return p->order == 2 ? energy_noalloc<double, 2>(*p, psi, scr, energy, s)
+                             : energy_noalloc<double, 4>(*p, psi, scr, energy, s);
+    return p->order == 2 ? energy_noalloc<float, 2>(*p, psi, scr, energy, s)
+                         : energy_noalloc<float, 4>(*p, psi, scr, energy, s);
+}
 
 int gpe_abi_version(void) { return GPE_ABI_VERSION; }
 

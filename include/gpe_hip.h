@@ -195,6 +195,19 @@ int gpe_deriv(gpe_ctx* ctx, int op, const void* in, int64_t ny_alloc, int64_t y_
 int gpe_rhs(gpe_ctx* ctx, const void* in, int64_t ny_alloc, int64_t y_off, int64_t z_off, void* out,
             int host_dtype);
 
+/* gpe_energy without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  psi is a device field of the grid and storage that
+ * *p describes (nx*ny*nz values, x fastest); the energy of gpe_energy is
+ * written to the device double *energy, all on `stream` (a hipStream_t, or
+ * NULL), with no allocation and no synchronisation.  The kernels' block
+ * partials live in a static device array of the library (65536 doubles,
+ * 512 KiB, held by each device until the process ends); a grid whose tile
+ * blocks outnumber it is refused (GPE_EINVAL).  Calls that overlap on one
+ * device share that array and race.  Same kernels, tiling and summation
+ * order as gpe_energy: the same result bit for bit. */
+int gpe_energy_noalloc(const gpe_params* p, const void* psi, double* energy, void* stream);
+
 /* Reductions of the device state, accumulated in FP64 in a fixed order:
  * get_norm with the end-corrected weights (src/variables.f90:537-559,
  * 713-823), energy (563-587), mass (591-611). */
