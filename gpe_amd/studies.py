@@ -171,8 +171,10 @@ def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 
                 break  # the error no longer follows the tolerance: a rounding floor above the target
             slope = min(max(s, 0.3), 2.0)
         e_prev = (le, lr)
-        # aim a little under the target so the landing run stays within it
-        eps = math.exp(le + (math.log(0.85 * target) - lr) / slope)
+        # aim a little under the target so the landing run stays within it,
+        # moving eps by at most a factor 10 per run
+        step = (math.log(0.85 * target) - lr) / slope
+        eps = math.exp(le + min(max(step, -math.log(10.0)), math.log(10.0)))
     if best is None:
         return None, None, None, tries
     return best[0], best[1], best[2], tries
