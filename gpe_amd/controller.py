@@ -47,6 +47,9 @@ class StepState:
     """History the controllers may use (most recent last)."""
     errmax: list = field(default_factory=list)
     dt: list = field(default_factory=list)
+    # the first attempt of each accepted step (dt tried, its errmax)
+    dt_first: list = field(default_factory=list)
+    errmax_first: list = field(default_factory=list)
     psi2max: float = 0.0
     energy_drift: float = 0.0
 
@@ -65,12 +68,13 @@ class RunStats:
 
 
 def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy: bool = False,
-              max_steps: int = 10 ** 7) -> RunStats:
+              max_steps: int = 10 ** 7, energy_every: int = 32) -> RunStats:
     """Advance the device state of `ctx` from t = 0 to T with Cash-Karp
     attempts and `controller`.  With clamp the last step lands on T exactly;
     without it the loop is the reference's `do while (t <= T)` (overshoot).
-    energy = True evaluates the energy after every accepted step (a feature
-    of the learned controller).  Wall time covers attempts, controller and
+    energy = True evaluates the energy every `energy_every` accepted steps
+    (the drift input of the learned controller; it changes slowly, and each
+    evaluation is a stencil pass over the field).  Wall time covers attempts, controller and
     syncs, not IC or I/O."""
     st = RunStats(method=getattr(controller, "name", "controller"))
     hist = StepState()
@@ -105,8 +109,10 @@ def integrate(ctx, T: float, dt0: float, controller, clamp: bool = True, energy:
         st.n_reject += rej
         hist.errmax.append(errmax)
         hist.dt.append(dt.real)
+        hist.dt_first.append(first[0])
+        hist.errmax_first.append(first[1])
         hist.psi2max = a2
-        if energy:
+        if energy and st.n_accept % energy_every == 0:
             E = ctx.energy()
             hist.energy_drift = (E - E0) / abs(E0) if E0 else 0.0
         st.trace.append((t, dt.real, errmax, rej, a2, hist.energy_drift, first[0], first[1]))

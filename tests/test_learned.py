@@ -1,7 +1,7 @@
 """Learned step-size proposer: the FP32 MLP kernels against the FP64 numpy
 oracle (forward, analytic gradient, finite differences), deterministic GPU
-training, and the proposer's gain over the plain controller on an RKF45 run
-(acceptance still by the Cash-Karp estimate)."""
+training, and the proposer against the plain controller on held-out RKF45
+runs (acceptance still by the Cash-Karp estimate, the plain rule as judge)."""
 import math
 
 import numpy as np
@@ -90,32 +90,49 @@ def test_training_is_deterministic_and_fits():
 
 
 @pytest.mark.gpu
-def test_learned_proposer_gain_over_plain():
+def test_learned_proposer_judged_by_plain_rule_loses_at_most_two_percent():
     """Traces from three small random-phase runs, two rounds of GPU training
-    (plain/exploring traces, then on-policy traces), then the same workload
-    with a new seed.  Prints the learned proposer's RHS evaluations and error
-    at T against the plain controller's.  The error ratio of this workload
-    fluctuates ~25% from step to step at fixed dt, which leaves a step-size
-    proposer only a few percent to gain; the test bounds the outcome (RHS
-    count within 5% of plain's, error within 3x) rather than asserting a
-    gain."""
+    (plain/exploring traces, then on-policy traces); then, on workloads the
+    network did not see (the random-phase IC with a new seed, and the
+    two-ring IC, whose error constant trends smoothly), the learned
+    controller against the plain one at the same tolerance, both against an
+    FP64 run at 1e-10.
+
+    What the network gains is bounded by what it can anticipate: on the
+    held-out turbulence run it removes only a few percent of the plain
+    rule's squared step-size error (the error constant errmax/dt^5 moves
+    like a random walk, whose best prediction is the last value, which is
+    the plain rule), so the RHS count and the error stay within noise of the
+    plain controller's.  Out of distribution the network's proposals are
+    poor, and the plain rule, judging every step, takes over.  Asserted:
+    RHS within 2% of plain's and error within 1.1x of plain's on both, and
+    the measured predictability gain below 0.25 on the turbulence run."""
     from gpe_amd import Context, ics
-    from gpe_amd.controller import PlainController, integrate
+    from gpe_amd.controller import PlainController, integrate, trace_array
     from gpe_amd.studies import rel_l2
 
     runs = [(64, 4, 0, 1e-6, 10.0, 0.35), (64, 4, 1, 1e-6, 10.0, 0.5), (64, 4, 2, 3e-6, 10.0, 0.35)]
     model = learned.train_pipeline(runs, rounds=2, steps=2000, log=print)
-    p, psi = ics.config_turb(n=64, seed=9, scheme="rk45", tau=0.01)
-    res = {}
-    for name, ctrl, eps in [("oracle", PlainController(), 1e-10), ("plain", PlainController(), 1e-6),
-                            ("learned", learned.LearnedController(model, p), 1e-6)]:
-        with Context(p.replace(eps=eps)) as ctx:
-            ctx.upload(psi)
-            st = integrate(ctx, 10.0, 0.01, ctrl, clamp=True, energy=(name == "learned"))
-            res[name] = (st, ctx.download())
-    err = {k: rel_l2(res[k][1], res["oracle"][1]) for k in ("plain", "learned")}
-    print({k: (res[k][0].n_accept, res[k][0].n_reject, res[k][0].n_rhs) for k in res}, err)
-    ratio = res["plain"][0].n_rhs / res["learned"][0].n_rhs
-    print(f"learned proposer: RHS ratio plain/learned = {ratio:.3f}, error ratio = {err['learned'] / err['plain']:.2f}")
-    assert ratio >= 0.95
-    assert err["learned"] <= 3 * err["plain"]
+    cases = {"turbulence seed 9": (*ics.config_turb(n=64, seed=9, scheme="rk45", tau=0.01), 10.0),
+             "two rings": (*ics.config_two_rings(n=64, dx=0.5), 10.0)}
+    gains = {}
+    for name, (p, psi, T) in cases.items():
+        res = {}
+        for who, eps in [("oracle", 1e-10), ("plain", 1e-6), ("learned", 1e-6)]:
+            ctrl = learned.LearnedController(model, p) if who == "learned" else PlainController()
+            with Context(p.replace(eps=eps)) as ctx:
+                ctx.upload(psi)
+                st = integrate(ctx, T, 0.01, ctrl, clamp=True, energy=(who == "learned"))
+                res[who] = (st, ctx.download(), ctrl)
+        err = {k: rel_l2(res[k][1], res["oracle"][1]) for k in ("plain", "learned")}
+        pred = learned.predictability([(p, trace_array(res["plain"][0]))], model)
+        gains[name] = pred["gain"]
+        rhs_ratio = res["learned"][0].n_rhs / res["plain"][0].n_rhs
+        print(f"{name}: plain {res['plain'][0].n_accept} steps/{res['plain'][0].n_reject} rejects, learned "
+              f"{res['learned'][0].n_accept}/{res['learned'][0].n_reject} ({res['learned'][2].n_plain_steps} "
+              f"proposed by the plain rule); RHS ratio learned/plain {rhs_ratio:.3f}, error ratio "
+              f"{err['learned'] / err['plain']:.3f}; network removes {pred['gain']:.3f} of the plain rule's "
+              f"squared step error")
+        assert rhs_ratio <= 1.02
+        assert err["learned"] <= 1.1 * err["plain"]
+    assert gains["turbulence seed 9"] < 0.25
