@@ -99,8 +99,10 @@ __device__ __forceinline__ int bcmap(int i, int n, int bcs) {
 // range every plane, in scalar registers (n >= 6 and H <= 2 keep them in
 // that band).
 __device__ __forceinline__ int bcmap1(int i, int n, int bcs) {
-    if (bcs == 1) return i < 0 ? i + n : (i >= n ? i - n : i);
-    return i < 0 ? -i : (i >= n ? 2 * (n - 1) - i : i);
+    // both rules formed and selected (no branch on bcs)
+    const int lo = bcs == 1 ? i + n : -i;
+    const int hi = bcs == 1 ? i - n : 2 * (n - 1) - i;
+    return i < 0 ? lo : (i >= n ? hi : i);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each

@@ -64,14 +64,20 @@ template <int ORD, int OX_ = 64, int TYB_ = kFusedTYB> struct FusedShape {
     static_assert(NTB <= NT, "block shape exceeds the workgroup");
 };
 
-// The shape of the fused RK4 pair: GPE_FUSED_NARROW selects 32-wide inner
-// tiles in 504-thread blocks (two blocks per CU).
-#ifndef GPE_FUSED_NARROW
-#define GPE_FUSED_NARROW 0
+// The shape of the fused RK4 pair: inner tiles GPE_RK4_OX wide in blocks of
+// GPE_RK4_TYB rows (a block of at most 512 threads runs two to a CU).
+#ifndef GPE_RK4_OX
+#define GPE_RK4_OX 64
 #endif
-constexpr int kRk4OX = GPE_FUSED_NARROW ? 32 : 64;
-constexpr int kRk4TYB = GPE_FUSED_NARROW ? 14 : kFusedTYB;
+#ifndef GPE_RK4_TYB
+#define GPE_RK4_TYB kFusedTYB
+#endif
+constexpr int kRk4OX = GPE_RK4_OX;
+constexpr int kRk4TYB = GPE_RK4_TYB;
 template <int ORD> constexpr int kRk4NT = FusedShape<ORD, kRk4OX, kRk4TYB>::NT;
+// waves per SIMD the RK4 pair asks the register allocator for: the FP32
+// stages 1+2 fit 64 VGPRs, so two 1024-thread blocks share a CU
+template <typename R, int ORD, int PAIR> constexpr int kRk4Waves = sizeof(R) == 4 && PAIR == 12 ? 8 : 1;
 
 // Per-thread geometry of a fused block.  The thread-to-column map is a
 // per-kernel choice (measured): inner-first saves the rim lanes' share of
@@ -279,10 +285,14 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
     unsigned ohal[S::NSLOT];
 #pragma unroll
     for (int s = 0; s < S::NSLOT; ++s) ohal[s] = c.hv[s] ? c.hoff[s] * E : kNoOff;
-    auto pl = [&](const C<R>* f, int zz) { return plane_res<R>(f, g.plane * bcmap1(zz, g.nz, g.bcs), pbytes); };
+    auto pl = [&](const C<R>* f, int zz) { return plane_res_m<R>(f, (unsigned)bcmap1(zz, g.nz, g.bcs), pbytes); };
 
     C<R> qp[2 * H + 1];  // psi0, planes z..z+2H
     C<R> qv[2 * H + 1];  // u1, planes z-H..z+H
+    // LDS buffer and ring slots of the iteration: (z - start) & 1,
+    // (z - start) % (H + 1) and (p - start) % (H + 1), advanced by one each
+    // iteration instead of divided out
+    int ra = 0, rz = 0, rp = H;
 #pragma unroll
     for (int d = 0; d <= 2 * H; ++d) qv[d] = mkc<R>(0.0, 0.0);
 #pragma unroll
@@ -305,7 +315,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
     auto step = [&](int z, Pf& F) {
         const int p = z + H;
         qp[2 * H] = F.qn;
-        C<R>* A = ldsA[(z - start) & 1];
+        C<R>* A = ldsA[ra];
         if (c.active) A[c.la] = qp[H];
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s)
@@ -322,14 +332,14 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
             sa.w = w;
             const C<R> u1 = cfma<R>(hdt, rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs)), qp[H]);
             qv[2 * H] = u1;
-            ldsB[(p - start) % (H + 1)][c.lb] = u1;
+            ldsB[rp][c.lb] = u1;
         }
         C<R> u2 = mkc<R>(0.0, 0.0);
         const bool second = z >= c.z0;
         if (second && c.inner) {
             // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
-            sb.L = ldsB[(z - start) % (H + 1)];
+            sb.L = ldsB[rz];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
@@ -337,19 +347,22 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
             u2 = cfma<R>(hdt, rhs_eval<R>(k, sb, c.mx, c.my, z), qp[0]);
         }
         // the store is issued on every path (kNoOff where there is no output)
-        bst<R>(plane_res<R>(u2out, g.plane * (second ? z : c.z0), pbytes), second ? oown : kNoOff, u2);
+        bst<R>(plane_res_m<R>(u2out, (unsigned)(second ? z : c.z0), pbytes), second ? oown : kNoOff, u2);
 #pragma unroll
         for (int d = 0; d < 2 * H; ++d) {
             qp[d] = qp[d + 1];
             qv[d] = qv[d + 1];
         }
+        ra ^= 1;
+        rz = rz == H ? 0 : rz + 1;
+        rp = rp == H ? 0 : rp + 1;
     };
     fused_march<D, Pf>(start, c.z1, fetch, step,
-                       [&] { bst<R>(plane_res<R>(u2out, 0, pbytes), kNoOff, mkc<R>(0.0, 0.0)); });
+                       [&] { bst<R>(plane_res_m<R>(u2out, 0u, pbytes), kNoOff, mkc<R>(0.0, 0.0)); });
 }
 
 template <typename R, int ORD, int TERMS>
-__global__ __launch_bounds__(kRk4NT<ORD>) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k,
+__global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k,
                                                                              const C<R>* __restrict__ psi,
                                                                              C<R>* __restrict__ u2out, C<R> hdt) {
     rk4_f12_body<R, ORD, TERMS, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
@@ -384,12 +397,13 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     unsigned ohal[S::NSLOT];
 #pragma unroll
     for (int s = 0; s < S::NSLOT; ++s) ohal[s] = c.hv[s] ? c.hoff[s] * E : kNoOff;
-    auto pl = [&](const C<R>* f, int zz) { return plane_res<R>(f, g.plane * bcmap1(zz, g.nz, g.bcs), pbytes); };
+    auto pl = [&](const C<R>* f, int zz) { return plane_res_m<R>(f, (unsigned)bcmap1(zz, g.nz, g.bcs), pbytes); };
 
     // Register queues, slot (plane - start) % Q: u2 and psi0 for planes
     // z..z+2H, u3 for planes z-H..z+H, and (FP32) acc for the inner planes
     // z..z+H.
     C<R> qu[Q], qp[Q], qv[Q], qa[H + 1];
+    int ra = 0, rz = 0, rp = H;  // (z - start) & 1, (z - start) % NB, (p - start) % NB
 #pragma unroll
     for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
 #pragma unroll
@@ -416,14 +430,14 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     };
     Pf F{};
     fetch(F, start);
-    bst<R>(plane_res<R>(out, 0, pbytes), kNoOff, mkc<R>(0.0, 0.0));  // as the iterations' store (see fused_march)
+    bst<R>(plane_res_m<R>(out, 0u, pbytes), kNoOff, mkc<R>(0.0, 0.0));  // as the iterations' store (see fused_march)
     auto step = [&](int z, auto phase) {
         constexpr int PH = decltype(phase)::value;
         const int p = z + H;
         qu[(PH + 2 * H) % Q] = F.qnu;
         qp[(PH + 2 * H) % Q] = F.qnp;
-        C<R>* A = ldsA[(z - start) & 1];
-        C<R>* P = ldsP[(z - start) & 1];
+        C<R>* A = ldsA[ra];
+        C<R>* P = ldsP[ra];
         if (c.active) {
             A[c.la] = qu[(PH + H) % Q];
             P[c.lb] = qp[(PH + H) % Q];
@@ -446,7 +460,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
             const C<R> p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
             const C<R> u3 = cfma<R>(dt, f2, p0);
             qv[(PH + 2 * H) % Q] = u3;
-            ldsB[(p - start) % NB][c.lb] = u3;
+            ldsB[rp][c.lb] = u3;
             if (c.inner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
                 FusedSten<R, ORD, S::TXB, TERMS> sp;
@@ -458,7 +472,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
                 const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, pm);
                 const C<R> base = mkc<R>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
                 const C<R> acc = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
-                if constexpr (kAccLds) ldsQ[(p - start) % NB][threadIdx.x] = acc;
+                if constexpr (kAccLds) ldsQ[rp][threadIdx.x] = acc;
                 else qa[H] = acc;
             }
         }
@@ -468,17 +482,20 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
             // fourth stage at plane z, inner tile (ldsB of plane z was written
             // H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
-            sb.L = ldsB[(z - start) % NB];
+            sb.L = ldsB[rz];
             sb.lc = c.lb;
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
             nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z),
-                         kAccLds ? ldsQ[(z - start) % NB][threadIdx.x] : qa[0]);
+                         kAccLds ? ldsQ[rz][threadIdx.x] : qa[0]);
         }
-        bst<R>(plane_res<R>(out, g.plane * (second ? z : c.z0), pbytes), second ? oown : kNoOff, nw);
+        bst<R>(plane_res_m<R>(out, (unsigned)(second ? z : c.z0), pbytes), second ? oown : kNoOff, nw);
 #pragma unroll
         for (int d = 0; d < H; ++d) qa[d] = qa[d + 1];
+        ra ^= 1;
+        rz = rz == NB - 1 ? 0 : rz + 1;
+        rp = rp == NB - 1 ? 0 : rp + 1;
     };
     march_rotating<Q>(start, c.z1, step);
 }

@@ -102,6 +102,14 @@ template <typename R> __device__ __forceinline__ BRes plane_res(const C<R>* f, l
     return __builtin_amdgcn_make_buffer_rsrc((void*)(f + uniform_i64(zb)), (short)0, (int)bytes, 0x00020000);
 }
 
+// Plane m of a field whose planes are pbytes long (pbytes < 2^32): the byte
+// offset is a 32 x 32-bit product, two scalar instructions where a signed
+// 64-bit index product takes seven.
+template <typename R> __device__ __forceinline__ BRes plane_res_m(const C<R>* f, unsigned m, unsigned pbytes) {
+    const char* b = reinterpret_cast<const char*>(f) + (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(m) * pbytes;
+    return __builtin_amdgcn_make_buffer_rsrc((void*)b, (short)0, (int)pbytes, 0x00020000);
+}
+
 template <typename R, int AUX = BUF_CACHED> __device__ __forceinline__ C<R> bld(BRes r, unsigned off) {
     if constexpr (sizeof(R) == 8) {
         typedef unsigned u4 __attribute__((ext_vector_type(4)));

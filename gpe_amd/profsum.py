@@ -9,6 +9,15 @@ counters in KiB).  Both counter runs include one plain copy kernel of known
 size (bench.py --copy-probe-mib); each counter is scaled by
 (bytes the copy moved) / (what the counter read for it), which is how the
 per-dispatch HBM bytes ("traffic") of the stencil kernels are obtained.
+
+    ... --sq SQ_A_DIR SQ_B_DIR
+
+adds the SQ counter table of the same kernels from two --pmc runs (the
+counter sets SQ_PASS_A and SQ_PASS_B), with rocprofv3's derived formulas:
+VALUBusy = SQ_ACTIVE_INST_VALU / CUs / GRBM_GUI_ACTIVE, LdsUtil =
+SQ_LDS_IDX_ACTIVE / CUs / GRBM_GUI_ACTIVE, SALUBusy = SQ_INST_CYCLES_SALU /
+CUs / GRBM_GUI_ACTIVE, with GRBM_GUI_ACTIVE per XCD (rocprofv3 reports it
+summed over the XCDs).
 """
 from __future__ import annotations
 
@@ -96,6 +105,64 @@ def kernel_label(name: str) -> str:
     return m.group(1) if m else name[:60]
 
 
+SQ_PASS_A = ("SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_LDS_IDX_ACTIVE "
+             "SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS SQ_WAIT_ANY GRBM_GUI_ACTIVE")
+SQ_PASS_B = ("SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_INST_CYCLES_SALU SQ_LDS_DATA_FIFO_FULL "
+             "SQ_LDS_ADDR_CONFLICT SQ_WAVES SQ_INSTS_VMEM_RD GRBM_GUI_ACTIVE")
+
+
+def sq_counters(d):
+    """{kernel name: {counter: mean per dispatch}} and (XCDs, CUs) of one --pmc run."""
+    per = {}
+    for r in csv.DictReader(open(_one(os.path.join(d, "**", "*counter_collection.csv")))):
+        k = (r["Kernel_Name"], r["Dispatch_Id"])
+        per.setdefault(k, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    out = {}
+    for (name, _), cs in per.items():
+        acc = out.setdefault(name, {"_n": 0})
+        acc["_n"] += 1
+        for c, v in cs.items():
+            acc[c] = acc.get(c, 0.0) + v
+    for acc in out.values():
+        n = acc.pop("_n")
+        for c in acc:
+            acc[c] /= n
+    agent = next(r for r in csv.DictReader(open(_one(os.path.join(d, "**", "*agent_info.csv"))))
+                 if r["Agent_Type"] == "GPU")
+    return out, (int(agent["Num_Xcc"]), int(agent["Cu_Count"]))
+
+
+def sq_lines(a_dir, b_dir, run_cmd):
+    A, (nxcc, ncu) = sq_counters(a_dir)
+    B, _ = sq_counters(b_dir)
+    lines = ["", "## SQ counters", "",
+             "    rocprofv3 --pmc " + SQ_PASS_A + " -d <SQ_A_DIR> -o c --output-format csv -- \\",
+             f"        {run_cmd}",
+             "    rocprofv3 --pmc " + SQ_PASS_B + " -d <SQ_B_DIR> -o c --output-format csv -- \\",
+             f"        {run_cmd}", "",
+             f"{nxcc} XCDs, {ncu} CUs.  Busy figures are fractions of the kernel's GPU-active cycles; the wait "
+             "figures are fractions of the waves' own cycles; instructions are per wave.", "",
+             "| kernel | VALUBusy | LdsUtil | LDS bank-conflict stall | SALUBusy | waves waiting (any) | "
+             "waves waiting for LDS issue | VALU / LDS / SALU / VMEM-read instructions per wave |",
+             "|---|---|---|---|---|---|---|---|"]
+    for name, a in A.items():
+        if not any(f in name for f in ("k_stencil", "k_rk4_f", "k_ck_pair", "k_ckmx_pair")) or name not in B:
+            continue
+        label = kernel_label(name)
+        if len(label.split()) < 2:
+            continue
+        b = B[name]
+        ga, gb = a["GRBM_GUI_ACTIVE"] / nxcc, b["GRBM_GUI_ACTIVE"] / nxcc
+        w = b["SQ_WAVES"]
+        lines.append(
+            f"| {label} | {100 * a['SQ_ACTIVE_INST_VALU'] / ncu / ga:.1f}% | {100 * a['SQ_LDS_IDX_ACTIVE'] / ncu / ga:.1f}% | "
+            f"{100 * a['SQ_LDS_BANK_CONFLICT'] / ncu / ga:.1f}% | {100 * b['SQ_INST_CYCLES_SALU'] / ncu / gb:.1f}% | "
+            f"{100 * a['SQ_WAIT_ANY'] / a['SQ_WAVE_CYCLES']:.1f}% | {100 * a['SQ_WAIT_INST_LDS'] / a['SQ_WAVE_CYCLES']:.1f}% | "
+            f"{b['SQ_INSTS_VALU'] / w:.0f} / {b['SQ_INSTS_LDS'] / w:.0f} / {b['SQ_INSTS_SALU'] / w:.0f} / "
+            f"{b['SQ_INSTS_VMEM_RD'] / w:.0f} |")
+    return lines
+
+
 def counters(path):
     per = {}
     for r in csv.DictReader(open(path)):
@@ -113,6 +180,8 @@ def main(argv=None):
     ap.add_argument("--out", default="profiles")
     ap.add_argument("--workload", choices=["rk4", "rkck"], default="rk4",
                     help="rk4: bench.py runs; rkck: gpe_amd.studies attempts runs")
+    ap.add_argument("--sq", nargs=2, metavar=("SQ_A_DIR", "SQ_B_DIR"), default=None,
+                    help="the two SQ counter runs (SQ_PASS_A, SQ_PASS_B) of the same command")
     a = ap.parse_args(argv)
 
     stats = list(csv.DictReader(open(_one(os.path.join(a.kt, "**", "*kernel_stats.csv")))))
@@ -173,6 +242,8 @@ def main(argv=None):
         lines.append(f"| {label} | {r['Calls']} | {avg_ms:.4f} | {alg / 1e9:.3f} | "
                      f"{alg / (avg_ms * 1e-3) / 1e9:.0f} | {100 * alg / (avg_ms * 1e-3) / 8e12:.1f} | "
                      f"{(rd or 0) / 1e9:.3f} | {(wr or 0) / 1e9:.3f} | {(t / alg) if (t and alg) else float('nan'):.3f} |")
+    if a.sq:
+        lines += sq_lines(a.sq[0], a.sq[1], run_pmc.replace(f" --copy-probe-mib {a.copy_mib}", ""))
     out = os.path.join(a.out, f"{a.workload}_stencil_{a.n}.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(open(out).read())
