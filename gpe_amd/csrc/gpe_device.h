@@ -123,6 +123,13 @@ struct Geom {
     int general;      // 1: the right-hand side has optional terms (RhsC::flags != 0)
 };
 
+// Offset in cells of plane m (0 <= m < nz): a 32 x 32-bit product, since a
+// plane holds fewer than 2^28 cells (a signed 64-bit product takes seven
+// scalar instructions).
+__device__ __forceinline__ long long plane_off(const Geom& g, int m) {
+    return (long long)((unsigned long long)(unsigned)m * (unsigned)g.plane);
+}
+
 // Stencil weights.  2nd order: d1 = (f+1 - f-1) w1, d2 = (f+1 - 2f + f-1) w2
 // with w1 = 1/(2d), w2 = 1/d^2.  4th order: d1 = (8(f+1 - f-1) - (f+2 - f-2)) w1,
 // d2 = (16(f+1 + f-1) - (f+2 + f-2) - 30 f) w2 with w1 = 1/(12d), w2 = 1/(12d^2)

@@ -584,6 +584,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
         C<R> qu[Q], qv[Q], qs[NB];
         double2 qe[NB];
         double qd[NB];
+        int ra = 0, rz = 0, rp = H;  // (z - start) & 1, (z - start) % NB, (p - start) % NB
 #pragma unroll
         for (int d = 0; d < Q; ++d) qv[d] = mkc<R>(0.0, 0.0);
 #pragma unroll
@@ -594,7 +595,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
         }
         if (c.active) {
 #pragma unroll
-            for (int d = 0; d < 2 * H; ++d) qu[d] = *(a.u + Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
+            for (int d = 0; d < 2 * H; ++d) qu[d] = *(a.u + Cell{plane_off(g, bcmap1(start + d, g.nz, g.bcs)), c.col});
         }
         struct Pw {
             C<R> p0, f0, k2, k3, k4;
@@ -619,9 +620,9 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
             Pw pw;
         };
         auto fetch = [&](Pf& F, int zz) {
-            const long long pb = g.plane * bcmap(zz + H, g.nz, g.bcs);
+            const long long pb = plane_off(g, bcmap1(zz + H, g.nz, g.bcs));
             if (c.active) {
-                F.qn = *(a.u + Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
+                F.qn = *(a.u + Cell{plane_off(g, bcmap1(zz + 2 * H, g.nz, g.bcs)), c.col});
                 if constexpr (PAIR == 2 && !kLate) load_pw(F.pw, pb);
             }
 #pragma unroll
@@ -639,9 +640,9 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
             if constexpr (P3 && !kLate) {
                 // pair 3 loads its pointwise inputs at the top of the plane's
                 // iteration: a second set in flight would not fit
-                if (c.active) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+                if (c.active) load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
             }
-            C<R>* A = ldsA[(z - start) & 1];
+            C<R>* A = ldsA[ra];
             if (c.active) A[c.la] = qu[(PH + H) % Q];
 #pragma unroll
             for (int s = 0; s < S::NSLOT; ++s)
@@ -656,30 +657,30 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
                 sa.w = w;
-                const C<R> fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+                const C<R> fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
                 C<R> ub;
                 if constexpr (PAIR == 1) {
                     const C<R> p0 = qu[(PH + H) % Q];
                     const C<R> k1 = cmul<R>(a.dt, fa);
-                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.f0out + Cell{g.plane * p, c.own}, fa);
+                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.f0out + Cell{plane_off(g, p), c.own}, fa);
                     ub = caxpy<R>(t.b21, k1, p0);
                     qs[H] = caxpy<R>(t.b31, k1, p0);
                 } else if constexpr (PAIR == 2) {
-                    if constexpr (kLate) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+                    if constexpr (kLate) load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
                     const C<R> k3 = cmul<R>(a.dt, fa);
                     const C<R> k1 = cmul<R>(a.dt, pw.f0);
-                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.ka + Cell{g.plane * p, c.own}, k3);
+                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.ka + Cell{plane_off(g, p), c.own}, k3);
                     ub = caxpy<R>(t.b43, k3, caxpy<R>(t.b42, pw.k2, caxpy<R>(t.b41, k1, pw.p0)));
                     qs[H] = caxpy<R>(t.b53, k3, caxpy<R>(t.b52, pw.k2, caxpy<R>(t.b51, k1, pw.p0)));
                 } else {
-                    if constexpr (kLate) load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+                    if constexpr (kLate) load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
                     const C<R> k5 = cmul<R>(a.dt, fa);
                     const C<R> k1 = cmul<R>(a.dt, pw.f0);
                     ub = caxpy<R>(t.b65, k5,
                                   caxpy<R>(t.b64, pw.k4,
                                            caxpy<R>(t.b63, pw.k3, caxpy<R>(t.b62, pw.k2, caxpy<R>(t.b61, k1, pw.p0)))));
                     if (c.inner) {
-                        const int slot = (p - start) % NB, li = kErrLds ? (int)threadIdx.x : c.lb;
+                        const int slot = rp, li = kErrLds ? (int)threadIdx.x : c.lb;
                         ldsK[slot][li] = caxpy<R>(t.c4, pw.k4, caxpy<R>(t.c3, pw.k3, caxpy<R>(t.c1, k1, pw.p0)));
                         const double2 se = mkc<double>(t.dc1 * (double)k1.x + t.dc3 * (double)pw.k3.x +
                                                            t.dc4 * (double)pw.k4.x + t.dc5 * (double)k5.x,
@@ -694,19 +695,19 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                     }
                 }
                 qv[(PH + 2 * H) % Q] = ub;
-                ldsB[(p - start) % NB][c.lb] = ub;
+                ldsB[rp][c.lb] = ub;
             }
             if (z >= c.z0 && c.inner) {
                 // second stage at plane z, inner tile (ldsB of plane z was
                 // written H iterations ago)
                 FusedSten<R, ORD, S::TXB, TERMS> sb;
-                sb.L = ldsB[(z - start) % NB];
+                sb.L = ldsB[rz];
                 sb.lc = c.lb;
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
                 sb.w = w;
                 const C<R> kb = cmul<R>(a.dt, rhs_eval<R>(k, sb, c.mx, c.my, z));
-                const Cell oz{g.plane * z, c.own};
+                const Cell oz{plane_off(g, z), c.own};
                 if constexpr (PAIR == 1) {
                     if (c.store) {
                         st_stream<R>(a.kb + oz, kb);
@@ -719,7 +720,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                     }
                 } else {
                     if (c.store) {
-                        const int slot = (z - start) % NB, li = kErrLds ? (int)threadIdx.x : c.lb;
+                        const int slot = rz, li = kErrLds ? (int)threadIdx.x : c.lb;
                         const C<R> nw = caxpy<R>(t.c6, kb, ldsK[slot][li]);
                         const double2 se = kErrLds ? ldsE[slot][li] : qe[0];
                         const double er = se.x + t.dc6 * (double)kb.x;
@@ -739,6 +740,9 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                 qe[d] = qe[d + 1];
                 qd[d] = qd[d + 1];
             }
+            ra ^= 1;
+            rz = rz == NB - 1 ? 0 : rz + 1;
+            rp = rp == NB - 1 ? 0 : rp + 1;
         };
         march_rotating<Q>(start, c.z1, step);
     }

@@ -76,6 +76,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         // qd (pair 3) |psi_scal|^2.
         CD qu[Q], qv[Q], qs[NB];
         double qd[NB];
+        int ra = 0, rz = 0, rp = H;  // (z - start) & 1, (z - start) % NB, (p - start) % NB
 #pragma unroll
         for (int d = 0; d < Q; ++d) qv[d] = mkc<double>(0.0, 0.0);
 #pragma unroll
@@ -85,7 +86,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         }
         if (c.active) {
 #pragma unroll
-            for (int d = 0; d < 2 * H; ++d) qu[d] = src(Cell{g.plane * bcmap(start + d, g.nz, g.bcs), c.col});
+            for (int d = 0; d < 2 * H; ++d) qu[d] = src(Cell{plane_off(g, bcmap1(start + d, g.nz, g.bcs)), c.col});
         }
         struct Pw {
             CD p0;
@@ -111,9 +112,9 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
             Pw pw;
         };
         auto fetch = [&](Pf& F, int zz) {
-            const long long pb = g.plane * bcmap(zz + H, g.nz, g.bcs);
+            const long long pb = plane_off(g, bcmap1(zz + H, g.nz, g.bcs));
             if (c.active) {
-                F.qn = src(Cell{g.plane * bcmap(zz + 2 * H, g.nz, g.bcs), c.col});
+                F.qn = src(Cell{plane_off(g, bcmap1(zz + 2 * H, g.nz, g.bcs)), c.col});
                 if constexpr (PAIR == 2) load_pw(F.pw, pb);
             }
 #pragma unroll
@@ -127,7 +128,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
             const int p = z + H;
             qu[(PH + 2 * H) % Q] = F.qn;
             Pw pw = F.pw;
-            CD* A = ldsA[(z - start) & 1];
+            CD* A = ldsA[ra];
             if (c.active) A[c.la] = qu[(PH + H) % Q];
 #pragma unroll
             for (int s = 0; s < S::NSLOT; ++s)
@@ -142,9 +143,9 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
                 sa.w = w;
-                const CD fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap(p, g.nz, g.bcs));
+                const CD fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
                 const bool own_p = c.store && p >= c.z0 && p < c.z1;
-                const Cell op{g.plane * p, c.own};
+                const Cell op{plane_off(g, p), c.own};
                 CD ub;
                 if constexpr (PAIR == 1) {
                     // OpMxCk1, then the source of OpMxCk2 and its D3 partial
@@ -166,7 +167,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     // OpMxCkMid<5>, then the source of OpMxCkLast and its partials;
                     // the pointwise inputs are loaded after the stencil (a set
                     // live across it spills)
-                    load_pw(pw, g.plane * bcmap(p, g.nz, g.bcs));
+                    load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
                     const CD k5 = cmul<double>(a.dt, fa);
                     const CD k1 = cmul<double>(a.dt, tod(pw.f0));
                     const CD k3v = tod(pw.k3), k4v = tod(pw.k4), k5v = tod(tof(k5));
@@ -177,9 +178,9 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     ub = sum1.comb(pw.p0, d6);
                     if (c.inner) {
                         const CD p0 = csub<double>(ub, tod(d6));  // as OpMxCkLast: u6 - D6
-                        ldsS[(p - start) % NB][threadIdx.x] =
+                        ldsS[rp][threadIdx.x] =
                             caxpy<double>(t.c4, k4v, caxpy<double>(t.c3, k3v, caxpy<double>(t.c1, k1, p0)));
-                        ldsE[(p - start) % NB][threadIdx.x] =
+                        ldsE[rp][threadIdx.x] =
                             mkc<double>(t.dc1 * k1.x + t.dc3 * k3v.x + t.dc4 * k4v.x + t.dc5 * k5v.x,
                                         t.dc1 * k1.y + t.dc3 * k3v.y + t.dc4 * k4v.y + t.dc5 * k5v.y);
                         const CD f0v = tod(pw.f0);
@@ -190,19 +191,19 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     }
                 }
                 qv[(PH + 2 * H) % Q] = ub;
-                ldsB[(p - start) % NB][c.lb] = ub;
+                ldsB[rp][c.lb] = ub;
             }
             if (z >= c.z0 && c.inner) {
                 // second stage at plane z, inner tile (ldsB of plane z was
                 // written H iterations ago)
                 FusedSten<R, ORD, S::TXB, TERMS> sb;
-                sb.L = ldsB[(z - start) % NB];
+                sb.L = ldsB[rz];
                 sb.lc = c.lb;
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
                 sb.w = w;
                 const CD kb = cmul<double>(a.dt, rhs_eval<R>(k, sb, c.mx, c.my, z));
-                const Cell oz{g.plane * z, c.own};
+                const Cell oz{plane_off(g, z), c.own};
                 if constexpr (PAIR <= 2) {
                     if (c.store) {
                         const double bn = PAIR == 1 ? t.b32 : t.b54;
@@ -211,9 +212,9 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     }
                 } else {
                     if (c.store) {
-                        const CD nw = caxpy<double>(t.c6, kb, ldsS[(z - start) % NB][threadIdx.x]);
+                        const CD nw = caxpy<double>(t.c6, kb, ldsS[rz][threadIdx.x]);
                         st_stream<double>(a.out + oz, nw);
-                        const CD se = ldsE[(z - start) % NB][threadIdx.x];
+                        const CD se = ldsE[rz][threadIdx.x];
                         const double er = se.x + t.dc6 * kb.x;
                         const double ei = se.y + t.dc6 * kb.y;
                         const double e = sqrt((er * er + ei * ei) / qd[0]);
@@ -228,6 +229,9 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                 qs[d] = qs[d + 1];
                 qd[d] = qd[d + 1];
             }
+            ra ^= 1;
+            rz = rz == NB - 1 ? 0 : rz + 1;
+            rp = rp == NB - 1 ? 0 : rp + 1;
         };
         march_rotating<Q>(start, c.z1, step);
     }
