@@ -195,8 +195,9 @@ def main(argv=None):
     os.makedirs(a.out, exist_ok=True)
     if a.workload == "rk4":
         title = f"# RK4 step kernels, {a.n}^3 random-phase workload (bench.py)"
-        run_kt = "python3 bench.py --steps 20 --warmup 5 --cpu-steps 0 --ref-steps 0 --rk45-steps 0"
-        run_pmc = f"python3 bench.py --steps 2 --warmup 1 --cpu-steps 0 --ref-steps 0 --rk45-steps 0 --copy-probe-mib {a.copy_mib}"
+        legs = "--cpu-steps 0 --ref-steps 0 --rk45-steps 0 --ttt-T 0 --ens-steps 0"
+        run_kt = f"python3 bench.py --steps 20 --warmup 5 {legs}"
+        run_pmc = f"python3 bench.py --steps 2 --warmup 1 {legs} --copy-probe-mib {a.copy_mib}"
     else:
         title = f"# RKF45 (Cash-Karp) stage and fused kernels, {a.n}^3 random-phase workload, tolerance 1e-6"
         run_kt = f"python3 -m gpe_amd.studies attempts --n {a.n} --attempts 6 --storages c128/stages,c128,mixed/stages,mixed,c64/stages,c64"
