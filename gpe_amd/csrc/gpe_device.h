@@ -231,6 +231,74 @@ template <typename R, int ORD, int LW> struct StenV : Sten<R, ORD, LW, true> {
     static constexpr int kTerms = RHS_V;
 };
 
+// Folded form of a stage update u = q + h f for a right-hand side whose only
+// optional term is the trap V (the fused kernels): f = i pim w with
+// w = a lap + (b + c|psi|^2 + d V) psi.  pim (+-1/2, +-1) and a (1, -1/2) are
+// powers of two, so u = q + G w with G = i pim h, and a folded into the
+// Laplacian weights, gives the same w and the same product terms as the
+// unfolded form; only the roundings of the complex product differ.  With a
+// real h (real time) G is imaginary and the product takes two FMAs instead of
+// a complex FMA's four and the rotation's two multiplies; with an imaginary h
+// (imaginary time) G is real.
+enum { GM_GEN = 0, GM_IM = 1, GM_RE = 2 };
+
+// which of the three forms G = i pim h takes (h = 0 counts as real)
+__host__ __device__ inline int g_mode(double hre, double him) { return him == 0.0 ? GM_IM : (hre == 0.0 ? GM_RE : GM_GEN); }
+
+// A wave-uniform value moved to scalar registers (FP64 and FP32 arithmetic
+// runs on the vector ALU, so a product of two kernel arguments would
+// otherwise occupy vector registers for the whole kernel).
+__device__ __forceinline__ float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uni(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename R> __device__ __forceinline__ C<R> g_of(R pim, C<R> h) {
+    return mkc<R>(uni(-pim * h.y), uni(pim * h.x));
+}
+
+// q + G w in the form GM
+template <typename R, int GM> __device__ __forceinline__ C<R> gaxpy(C<R> G, C<R> w, C<R> q) {
+    if constexpr (GM == GM_IM) {
+        return mkc<R>(q.x - G.y * w.y, q.y + G.y * w.x);
+    } else if constexpr (GM == GM_RE) {
+        return mkc<R>(q.x + G.x * w.x, q.y + G.x * w.y);
+    } else {
+        return cfma<R>(G, w, q);
+    }
+}
+
+// The Laplacian weights times a (exact: a is a power of two).
+template <typename R> __device__ __forceinline__ StW<R> scale_w(StW<R> w, R a) {
+    w.w2x = uni(w.w2x * a);
+    w.w2y = uni(w.w2y * a);
+    w.w2z = uni(w.w2z * a);
+    w.w2c = uni(w.w2c * a);
+    return w;
+}
+
+// w = a lap + (b + c|psi|^2 [+ d V]) psi with a already in the stencil's
+// weights (scale_w); S is Sten<.., false> or StenV.
+template <typename R, class S>
+__device__ __forceinline__ C<R> rhs_w(const RhsC<R>& k, const S& st, int x, int y, int z) {
+    static_assert((S::kTerms & ~RHS_V) == 0, "rhs_w: the trap V is the only optional term");
+    const C<R> p = st.c();
+    const C<R> L = st.lap();
+    R s = k.b + k.c * (p.x * p.x + p.y * p.y);
+    if constexpr ((S::kTerms & RHS_V) != 0) {
+        if (k.flags & RHS_V) {
+            const R xc = k.xl + R(x) * k.dx, yc = k.yl + R(y) * k.dy, zc = k.zl + R(z) * k.dz;
+            s += k.d * (k.vx * (xc * xc) + k.vy * (yc * yc) + k.vz * (zc * zc));
+        }
+    }
+    return mkc<R>(L.x + s * p.x, L.y + s * p.y);
+}
+
 template <typename R, class S>
 __device__ __forceinline__ C<R> rhs_eval(const RhsC<R>& k, const S& st, int x, int y, int z) {
     const C<R> p = st.c();

@@ -266,8 +266,9 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
 template <typename R, int ORD, int LW, int TERMS>
 using FusedSten = std::conditional_t<TERMS == 0, Sten<R, ORD, LW, false>, StenV<R, ORD, LW>>;
 
-template <typename R, int ORD, int TERMS, int OX_ = 64, int TYB_ = kFusedTYB>
-__device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
+// GM: the form of G = i pim h of the stage updates (gaxpy).
+template <typename R, int ORD, int TERMS, int GM, int OX_ = 64, int TYB_ = kFusedTYB>
+__device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, const RhsC<R>& k,
                                              const C<R>* __restrict__ psi, C<R>* __restrict__ u2out, C<R> hdt,
                                              int bid) {
     using S = FusedShape<ORD, OX_, TYB_>;
@@ -277,6 +278,8 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
     __shared__ C<R> ldsB[H + 1][S::NTB];
     const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
     if (c.z0 >= c.z1) return;  // uniform per block
+    const StW<R> w = scale_w<R>(w0, k.a);
+    const C<R> G = g_of<R>(k.pim, hdt);  // u = q + G w (dt/2 f)
     const int start = c.z0 - 2 * H;
     // byte offsets within a plane; kNoOff where the lane has no access
     const unsigned pbytes = (unsigned)g.plane * E;
@@ -330,7 +333,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qp[d];
             sa.w = w;
-            const C<R> u1 = cfma<R>(hdt, rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs)), qp[H]);
+            const C<R> u1 = gaxpy<R, GM>(G, rhs_w<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs)), qp[H]);
             qv[2 * H] = u1;
             ldsB[rp][c.lb] = u1;
         }
@@ -344,7 +347,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
             sb.w = w;
-            u2 = cfma<R>(hdt, rhs_eval<R>(k, sb, c.mx, c.my, z), qp[0]);
+            u2 = gaxpy<R, GM>(G, rhs_w<R>(k, sb, c.mx, c.my, z), qp[0]);
         }
         // the store is issued on every path (kNoOff where there is no output)
         bst<R>(plane_res_m<R>(u2out, (unsigned)(second ? z : c.z0), pbytes), second ? oown : kNoOff, u2);
@@ -361,16 +364,16 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w, con
                        [&] { bst<R>(plane_res_m<R>(u2out, 0u, pbytes), kNoOff, mkc<R>(0.0, 0.0)); });
 }
 
-template <typename R, int ORD, int TERMS>
+template <typename R, int ORD, int TERMS, int GM>
 __global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k,
                                                                              const C<R>* __restrict__ psi,
                                                                              C<R>* __restrict__ u2out, C<R> hdt) {
-    rk4_f12_body<R, ORD, TERMS, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+    rk4_f12_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
-template <typename R, int ORD, int TERMS, int OX_ = 64, int TYB_ = kFusedTYB>
-__device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, const RhsC<R>& k,
+template <typename R, int ORD, int TERMS, int GM, int OX_ = 64, int TYB_ = kFusedTYB>
+__device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, const RhsC<R>& k,
                                              const C<R>* __restrict__ u2, const C<R>* __restrict__ psi,
                                              C<R>* __restrict__ out, C<R> dt, C<R> dt3, C<R> dt6, int bid) {
     using S = FusedShape<ORD, OX_, TYB_>;
@@ -390,6 +393,9 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     __shared__ C<R> ldsQ[kAccLds ? NB : 1][kAccLds ? S::OX * S::OY : 1];
     const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
     if (c.z0 >= c.z1) return;
+    const StW<R> w = scale_w<R>(w0, k.a);
+    // u = q + G w for the steps dt, dt/3 and dt/6
+    const C<R> G1 = g_of<R>(k.pim, dt), G3 = g_of<R>(k.pim, dt3), G6 = g_of<R>(k.pim, dt6);
     const int start = c.z0 - 2 * H;
     const unsigned pbytes = (unsigned)g.plane * E;
     const unsigned ocol = c.active ? c.col * E : kNoOff;
@@ -456,9 +462,9 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
             for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
             sa.w = w;
             const int pm = bcmap1(p, g.nz, g.bcs);
-            const C<R> f2 = rhs_eval<R>(k, sa, c.mx, c.my, pm);
+            const C<R> w2 = rhs_w<R>(k, sa, c.mx, c.my, pm);
             const C<R> p0 = qp[(PH + H) % Q], uu = qu[(PH + H) % Q];
-            const C<R> u3 = cfma<R>(dt, f2, p0);
+            const C<R> u3 = gaxpy<R, GM>(G1, w2, p0);
             qv[(PH + 2 * H) % Q] = u3;
             ldsB[rp][c.lb] = u3;
             if (c.inner) {
@@ -469,9 +475,9 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sp.q[d] = qp[(PH + d) % Q];
                 sp.w = w;
-                const C<R> f1 = rhs_eval<R>(k, sp, c.mx, c.my, pm);
+                const C<R> w1 = rhs_w<R>(k, sp, c.mx, c.my, pm);
                 const C<R> base = mkc<R>(p0.x + two3 * (uu.x - p0.x), p0.y + two3 * (uu.y - p0.y));
-                const C<R> acc = cfma<R>(dt3, f2, cfma<R>(dt6, f1, base));
+                const C<R> acc = gaxpy<R, GM>(G3, w2, gaxpy<R, GM>(G6, w1, base));
                 if constexpr (kAccLds) ldsQ[rp][threadIdx.x] = acc;
                 else qa[H] = acc;
             }
@@ -487,8 +493,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
             sb.w = w;
-            nw = cfma<R>(dt6, rhs_eval<R>(k, sb, c.mx, c.my, z),
-                         kAccLds ? ldsQ[rz][threadIdx.x] : qa[0]);
+            nw = gaxpy<R, GM>(G6, rhs_w<R>(k, sb, c.mx, c.my, z), kAccLds ? ldsQ[rz][threadIdx.x] : qa[0]);
         }
         bst<R>(plane_res_m<R>(out, (unsigned)(second ? z : c.z0), pbytes), second ? oown : kNoOff, nw);
 #pragma unroll
@@ -500,12 +505,12 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w, con
     march_rotating<Q>(start, c.z1, step);
 }
 
-template <typename R, int ORD, int TERMS>
+template <typename R, int ORD, int TERMS, int GM>
 __global__ __launch_bounds__(kRk4NT<ORD>) void k_rk4_f34(
     Geom g, StW<R> w, RhsC<R> k, const C<R>* __restrict__ u2, const C<R>* __restrict__ psi, C<R>* __restrict__ out,
     C<R> dt, C<R> dt3, C<R> dt6) {
-    rk4_f34_body<R, ORD, TERMS, kRk4OX, kRk4TYB>(g, w, k, u2, psi, out, dt, dt3, dt6,
-                                                 xcd_remap(blockIdx.x, g.nblocks));
+    rk4_f34_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB>(g, w, k, u2, psi, out, dt, dt3, dt6,
+                                                     xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Fused Cash-Karp attempt (rkck, src/solve.f90:248-378, with the error
@@ -786,22 +791,22 @@ __device__ __forceinline__ int fused_member_of(const FusedMember* __restrict__ m
     return lo;
 }
 
-template <int ORD>
+template <int ORD, int GM>
 __global__ __launch_bounds__(1024) void k_rk4_f12_batched(const FusedMember* __restrict__ mem, int nmem, int total,
                                                           double2 hdt, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f12_body<double, ORD, 0>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
+        rk4_f12_body<double, ORD, 0, GM>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
         __syncthreads();  // the next item reuses the LDS planes
     }
 }
 
-template <int ORD>
+template <int ORD, int GM>
 __global__ __launch_bounds__(1024) void k_rk4_f34_batched(const FusedMember* __restrict__ mem, int nmem, int total,
                                                           double2 dt, double2 dt3, double2 dt6, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f34_body<double, ORD, 0>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
+        rk4_f34_body<double, ORD, 0, GM>(M.g, M.w, M.k, M.u2, flip ? M.alt : M.psi, flip ? M.psi : M.alt, dt, dt3, dt6,
                           item - M.begin);
         __syncthreads();
     }

@@ -40,40 +40,61 @@ template <int ORD> int rk4_fused_tiles(int nx, int ny, int nz, int ncu, Geom& g)
     return fused_tiles_shape<ORD, kRk4OX, kRk4TYB>(nx, ny, nz, ncu * per_cu, g);
 }
 
-template <typename R, int ORD>
-hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
-                            C<R> dt, int stage_pair, hipStream_t s) {
-    if (g.nblocks <= 0) return hipSuccess;
+// The kernels of one G form (gaxpy): stage pair 12 or 34, with or without the trap term.
+template <typename R, int ORD, int GM>
+static void launch_rk4_fused_gm(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
+                                C<R> dt, int stage_pair, hipStream_t s) {
     constexpr int NT = FusedShape<ORD, kRk4OX, kRk4TYB>::NT;
     if (stage_pair == 12) {
         const C<R> hdt = mkc<R>(R(0.5) * dt.x, R(0.5) * dt.y);
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f12<R, ORD, RHS_V>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, RHS_V, GM>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
         else
-            hipLaunchKernelGGL((k_rk4_f12<R, ORD, 0>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
+            hipLaunchKernelGGL((k_rk4_f12<R, ORD, 0, GM>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, psi, u2, hdt);
     } else {
         const C<R> dt3 = mkc<R>(dt.x / R(3), dt.y / R(3)), dt6 = mkc<R>(dt.x / R(6), dt.y / R(6));
         if (g.general)
-            hipLaunchKernelGGL((k_rk4_f34<R, ORD, RHS_V>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k,
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, RHS_V, GM>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k,
                                (const C<R>*)u2, psi, out, dt, dt3, dt6);
         else
-            hipLaunchKernelGGL((k_rk4_f34<R, ORD, 0>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, (const C<R>*)u2,
+            hipLaunchKernelGGL((k_rk4_f34<R, ORD, 0, GM>), dim3(g.nblocks), dim3(NT), 0, s, g, w, k, (const C<R>*)u2,
                                psi, out, dt, dt3, dt6);
     }
+}
+
+template <typename R, int ORD>
+hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* u2, C<R>* out,
+                            C<R> dt, int stage_pair, hipStream_t s) {
+    if (g.nblocks <= 0) return hipSuccess;
+    switch (g_mode((double)dt.x, (double)dt.y)) {
+        case GM_IM: launch_rk4_fused_gm<R, ORD, GM_IM>(g, w, k, psi, u2, out, dt, stage_pair, s); break;
+        case GM_RE: launch_rk4_fused_gm<R, ORD, GM_RE>(g, w, k, psi, u2, out, dt, stage_pair, s); break;
+        default: launch_rk4_fused_gm<R, ORD, GM_GEN>(g, w, k, psi, u2, out, dt, stage_pair, s); break;
+    }
     return hipGetLastError();
+}
+
+template <int ORD, int GM>
+static void launch_rk4_fused_batched_gm(const FusedMember* mem, int nmem, int total, int grid, double2 dt,
+                                        int stage_pair, int flip, hipStream_t s) {
+    if (stage_pair == 12) {
+        const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
+        hipLaunchKernelGGL((k_rk4_f12_batched<ORD, GM>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, hdt, flip);
+    } else {
+        const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
+        hipLaunchKernelGGL((k_rk4_f34_batched<ORD, GM>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, dt, dt3,
+                           dt6, flip);
+    }
 }
 
 template <int ORD>
 hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
                                     int flip, hipStream_t s) {
     if (total <= 0) return hipSuccess;
-    if (stage_pair == 12) {
-        const double2 hdt = mkc<double>(0.5 * dt.x, 0.5 * dt.y);
-        hipLaunchKernelGGL((k_rk4_f12_batched<ORD>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, hdt, flip);
-    } else {
-        const double2 dt3 = mkc<double>(dt.x / 3.0, dt.y / 3.0), dt6 = mkc<double>(dt.x / 6.0, dt.y / 6.0);
-        hipLaunchKernelGGL((k_rk4_f34_batched<ORD>), dim3(grid), dim3(1024), 0, s, mem, nmem, total, dt, dt3, dt6,
-                           flip);
+    switch (g_mode(dt.x, dt.y)) {
+        case GM_IM: launch_rk4_fused_batched_gm<ORD, GM_IM>(mem, nmem, total, grid, dt, stage_pair, flip, s); break;
+        case GM_RE: launch_rk4_fused_batched_gm<ORD, GM_RE>(mem, nmem, total, grid, dt, stage_pair, flip, s); break;
+        default: launch_rk4_fused_batched_gm<ORD, GM_GEN>(mem, nmem, total, grid, dt, stage_pair, flip, s); break;
     }
     return hipGetLastError();
 }
