@@ -273,6 +273,32 @@ template <typename R, int GM> __device__ __forceinline__ C<R> gaxpy(C<R> G, C<R>
     }
 }
 
+// G w in the form GM (a right-hand side's increment dt f = G w).
+template <typename R, int GM> __device__ __forceinline__ C<R> gmul(C<R> G, C<R> w) {
+    if constexpr (GM == GM_IM) {
+        return mkc<R>(-G.y * w.y, G.y * w.x);
+    } else if constexpr (GM == GM_RE) {
+        return mkc<R>(G.x * w.x, G.x * w.y);
+    } else {
+        return cmul<R>(G, w);
+    }
+}
+
+// h f for a step h of the form GM (GM_IM: h real, GM_RE: h imaginary): the
+// complex product without its zero terms, so the same rounding.
+template <typename R, int GM> __device__ __forceinline__ C<R> hmul(C<R> h, C<R> f) {
+    if constexpr (GM == GM_IM) {
+        return mkc<R>(h.x * f.x, h.x * f.y);
+    } else if constexpr (GM == GM_RE) {
+        return mkc<R>(-h.y * f.y, h.y * f.x);
+    } else {
+        return cmul<R>(h, f);
+    }
+}
+
+// f = i pim w
+template <typename R> __device__ __forceinline__ C<R> rot_pim(R pim, C<R> w) { return mkc<R>(-pim * w.y, pim * w.x); }
+
 // The Laplacian weights times a (exact: a is a power of two).
 template <typename R> __device__ __forceinline__ StW<R> scale_w(StW<R> w, R a) {
     w.w2x = uni(w.w2x * a);

@@ -83,10 +83,22 @@ template <typename R, int ORD, int PAIR> constexpr int kRk4Waves = sizeof(R) == 
 // per-kernel choice (measured): inner-first saves the rim lanes' share of
 // the second stage, but it moves register allocation, and some kernels then
 // spill or run slower.
-template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> struct FusedCtx {
+//
+//
+// REP (a per-kernel choice, measured): the workgroup's threads beyond the
+// block's NTB columns (NT - NTB of them) each repeat a rim column of the last
+// row, so every lane is active and the per-plane branches on `active` fold
+// away (a repeated column computes and writes the same values to the same
+// LDS cells, and a rim column stores nothing); and with the inner-first map
+// and an inner tile of whole wavefronts, `winner` is the inner flag as a
+// wave-uniform (scalar) value, so the second-stage branch is a scalar
+// branch.  The larger basic blocks move register allocation: some kernels
+// spill with it.  Without REP, winner == inner.
+template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB, bool REP = false> struct FusedCtx {
     using S = FusedShape<ORD, OX_, TYB_>;
+    static constexpr bool kWaveInner = REP && INNER_FIRST && (S::OX * S::OY) % 64 == 0;
     int z0, z1;
-    bool active, inner, store;
+    bool active, inner, winner, store;
     int lx, ly, gx, gy;
     int mx, my;  // the column's in-domain (mapped) indices
     unsigned col, own;
@@ -96,7 +108,8 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> 
     bool hv[S::NSLOT];
 
     __device__ __forceinline__ FusedCtx(const Geom& g, int bid) {
-        const int tid = threadIdx.x;
+        // with REP, threads beyond the block repeat rim columns of the last row
+        const int tid = !REP || (int)threadIdx.x < S::NTB ? (int)threadIdx.x : (int)threadIdx.x - (S::NT - S::NTB);
         const int ntxy = g.ntx * g.nty;
         const int txy = bid % ntxy, zc = bid / ntxy;
         // Tiles run in column groups 2 tiles wide (all rows of a group, then
@@ -117,8 +130,9 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> 
         // runs wave-uniform; the others own the H-wide rim: the H rows above
         // and below, then the H columns left and right of each inner row.
         // Otherwise thread t owns column (t % TXB, t / TXB) of the block.
-        active = tid < S::NTB;
         constexpr int NIN = S::OX * S::OY;
+        static_assert(S::NT - S::NTB <= S::TXB && S::NT - S::NTB <= S::NTB - NIN, "repeated columns must be rim columns");
+        active = REP || tid < S::NTB;
         if (!INNER_FIRST) {
             lx = tid % S::TXB;
             ly = active ? tid / S::TXB : 0;
@@ -144,6 +158,8 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> 
         gx = ox0 - S::H + lx;
         gy = oy0 - S::H + ly;
         inner = INNER_FIRST ? tid < NIN : active && lx >= S::H && lx < S::TXB - S::H && ly >= S::H && ly < S::TYB - S::H;
+        if constexpr (kWaveInner) winner = __builtin_amdgcn_readfirstlane((int)inner) != 0;
+        else winner = inner;
         store = inner && gx < g.nx && gy < g.ny;
         mx = bcmap(gx, g.nx, g.bcs);
         my = bcmap(gy, g.ny, g.bcs);
@@ -154,7 +170,7 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB> 
         // halo of the block plane: 2H rows above/below, then 2H columns left/right
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s) {
-            const int e = tid + s * S::NT;
+            const int e = (int)threadIdx.x + s * S::NT;  // every workgroup thread takes halo cells
             hv[s] = e < S::NHALO;
             int hx = ox0 - S::H, hy = oy0 - S::H, lxx = S::H, lyy = S::H;
             if (e < 2 * S::H * S::TXB) {
@@ -276,7 +292,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, co
     constexpr unsigned E = sizeof(C<R>);
     __shared__ C<R> ldsA[2][S::LHA * S::LWA];
     __shared__ C<R> ldsB[H + 1][S::NTB];
-    const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
+    const FusedCtx<ORD, true, OX_, TYB_, sizeof(R) == 4> c(g, bid);  // REP measured: FP32 faster, FP64 slower
     if (c.z0 >= c.z1) return;  // uniform per block
     const StW<R> w = scale_w<R>(w0, k.a);
     const C<R> G = g_of<R>(k.pim, hdt);  // u = q + G w (dt/2 f)
@@ -339,7 +355,7 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, co
         }
         C<R> u2 = mkc<R>(0.0, 0.0);
         const bool second = z >= c.z0;
-        if (second && c.inner) {
+        if (second && c.winner) {
             // second stage at plane z, inner tile (ldsB of plane z was written H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
             sb.L = ldsB[rz];
@@ -391,7 +407,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
     // registers.
     constexpr bool kAccLds = sizeof(R) == 8;
     __shared__ C<R> ldsQ[kAccLds ? NB : 1][kAccLds ? S::OX * S::OY : 1];
-    const FusedCtx<ORD, true, OX_, TYB_> c(g, bid);
+    const FusedCtx<ORD, true, OX_, TYB_, true> c(g, bid);
     if (c.z0 >= c.z1) return;
     const StW<R> w = scale_w<R>(w0, k.a);
     // u = q + G w for the steps dt, dt/3 and dt/6
@@ -467,7 +483,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
             const C<R> u3 = gaxpy<R, GM>(G1, w2, p0);
             qv[(PH + 2 * H) % Q] = u3;
             ldsB[rp][c.lb] = u3;
-            if (c.inner) {
+            if (c.winner) {
                 // acc at plane p from k1 = f(psi0) and the recovered dt k2 = 2 (u2 - psi0)
                 FusedSten<R, ORD, S::TXB, TERMS> sp;
                 sp.L = P;
@@ -484,7 +500,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
         }
         C<R> nw = mkc<R>(0.0, 0.0);
         const bool second = z >= c.z0;
-        if (second && c.inner) {
+        if (second && c.winner) {
             // fourth stage at plane z, inner tile (ldsB of plane z was written
             // H iterations ago)
             FusedSten<R, ORD, S::TXB, TERMS> sb;
@@ -551,8 +567,9 @@ template <typename R> struct CkPairArgs {
     double dt0_re, dt0_im;  // the step's dt on entry to rkqs (psi_scal)
 };
 
-template <typename R, int ORD, int TERMS, int PAIR>
-__device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, const RhsC<R>& k, const CkPairArgs<R>& a,
+// GM: the form of G = i pim dt (gmul, hmul).
+template <typename R, int ORD, int TERMS, int PAIR, int GM>
+__device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w0, const RhsC<R>& k, const CkPairArgs<R>& a,
                                              double* __restrict__ partial, int bid, int pidx) {
     using S = FusedShape<ORD>;
     constexpr int H = S::H, NB = H + 1, Q = 2 * H + 1;
@@ -579,6 +596,8 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
     __shared__ double2 ldsE[P3 && kErrLds ? NB : 1][P3 && kErrLds ? NIN : 1];
     __shared__ double red_sh[16];
     const FusedCtx<ORD, !(P3 && sizeof(R) == 4)> c(g, bid);  // the map measured faster per kernel
+    const StW<R> w = scale_w<R>(w0, k.a);
+    const C<R> G = g_of<R>(k.pim, a.dt);  // k = dt f = G w
     double red0 = 0.0, red1 = 0.0;
     if (c.z0 < c.z1) {  // uniform per block
         const int start = c.z0 - 2 * H;
@@ -662,29 +681,30 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
                 sa.w = w;
-                const C<R> fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
+                const C<R> wa = rhs_w<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
                 C<R> ub;
                 if constexpr (PAIR == 1) {
                     const C<R> p0 = qu[(PH + H) % Q];
-                    const C<R> k1 = cmul<R>(a.dt, fa);
-                    if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.f0out + Cell{plane_off(g, p), c.own}, fa);
+                    const C<R> k1 = gmul<R, GM>(G, wa);
+                    if (c.store && p >= c.z0 && p < c.z1)
+                        st_stream<R>(a.f0out + Cell{plane_off(g, p), c.own}, rot_pim<R>(k.pim, wa));
                     ub = caxpy<R>(t.b21, k1, p0);
                     qs[H] = caxpy<R>(t.b31, k1, p0);
                 } else if constexpr (PAIR == 2) {
                     if constexpr (kLate) load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
-                    const C<R> k3 = cmul<R>(a.dt, fa);
-                    const C<R> k1 = cmul<R>(a.dt, pw.f0);
+                    const C<R> k3 = gmul<R, GM>(G, wa);
+                    const C<R> k1 = hmul<R, GM>(a.dt, pw.f0);
                     if (c.store && p >= c.z0 && p < c.z1) st_stream<R>(a.ka + Cell{plane_off(g, p), c.own}, k3);
                     ub = caxpy<R>(t.b43, k3, caxpy<R>(t.b42, pw.k2, caxpy<R>(t.b41, k1, pw.p0)));
                     qs[H] = caxpy<R>(t.b53, k3, caxpy<R>(t.b52, pw.k2, caxpy<R>(t.b51, k1, pw.p0)));
                 } else {
                     if constexpr (kLate) load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
-                    const C<R> k5 = cmul<R>(a.dt, fa);
-                    const C<R> k1 = cmul<R>(a.dt, pw.f0);
+                    const C<R> k5 = gmul<R, GM>(G, wa);
+                    const C<R> k1 = hmul<R, GM>(a.dt, pw.f0);
                     ub = caxpy<R>(t.b65, k5,
                                   caxpy<R>(t.b64, pw.k4,
                                            caxpy<R>(t.b63, pw.k3, caxpy<R>(t.b62, pw.k2, caxpy<R>(t.b61, k1, pw.p0)))));
-                    if (c.inner) {
+                    if (c.winner) {
                         const int slot = rp, li = kErrLds ? (int)threadIdx.x : c.lb;
                         ldsK[slot][li] = caxpy<R>(t.c4, pw.k4, caxpy<R>(t.c3, pw.k3, caxpy<R>(t.c1, k1, pw.p0)));
                         const double2 se = mkc<double>(t.dc1 * (double)k1.x + t.dc3 * (double)pw.k3.x +
@@ -702,7 +722,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
                 qv[(PH + 2 * H) % Q] = ub;
                 ldsB[rp][c.lb] = ub;
             }
-            if (z >= c.z0 && c.inner) {
+            if (z >= c.z0 && c.winner) {
                 // second stage at plane z, inner tile (ldsB of plane z was
                 // written H iterations ago)
                 FusedSten<R, ORD, S::TXB, TERMS> sb;
@@ -711,7 +731,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
                 sb.w = w;
-                const C<R> kb = cmul<R>(a.dt, rhs_eval<R>(k, sb, c.mx, c.my, z));
+                const C<R> kb = gmul<R, GM>(G, rhs_w<R>(k, sb, c.mx, c.my, z));
                 const Cell oz{plane_off(g, z), c.own};
                 if constexpr (PAIR == 1) {
                     if (c.store) {
@@ -761,10 +781,10 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w, con
     }
 }
 
-template <typename R, int ORD, int TERMS, int PAIR>
+template <typename R, int ORD, int TERMS, int PAIR, int GM>
 __global__ __launch_bounds__(1024) void k_ck_pair(Geom g, StW<R> w, RhsC<R> k, CkPairArgs<R> a,
                                                   double* __restrict__ partial) {
-    ck_pair_body<R, ORD, TERMS, PAIR>(g, w, k, a, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
+    ck_pair_body<R, ORD, TERMS, PAIR, GM>(g, w, k, a, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
 }
 
 // Batched form for an ensemble of grids: persistent blocks walk a work list

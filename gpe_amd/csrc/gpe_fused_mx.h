@@ -43,8 +43,9 @@ struct CkMxArgs {
     double dt0_re, dt0_im;
 };
 
-template <int ORD, int TERMS, int PAIR>
-__device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>& w, const RhsC<double>& k,
+// GM: the form of G = i pim dt (gmul, hmul).
+template <int ORD, int TERMS, int PAIR, int GM>
+__device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>& w0, const RhsC<double>& k,
                                                const CkMxArgs& a, double* __restrict__ partial, int bid, int pidx) {
     using S = FusedShape<ORD>;
     using R = double;
@@ -60,6 +61,8 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     __shared__ CD ldsE[P3 ? NB : 1][P3 ? NIN : 1];
     __shared__ double red_sh[16];
     const FusedCtx<ORD, PAIR != 1> c(g, bid);  // the map measured faster per pair
+    const StW<double> w = scale_w<double>(w0, k.a);
+    const CD G = g_of<double>(k.pim, a.dt);  // k = dt f = G w
     const SumSrc sum1{a.p0, a.d, mkc<double>(1.0, 0.0)};
     const SumSrc sum21{a.p0, a.f0, a.s21};
     const CkTab<double>& t = a.t;
@@ -143,22 +146,22 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sa.q[d] = qu[(PH + d) % Q];
                 sa.w = w;
-                const CD fa = rhs_eval<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
+                const CD wa = rhs_w<R>(k, sa, c.mx, c.my, bcmap1(p, g.nz, g.bcs));
                 const bool own_p = c.store && p >= c.z0 && p < c.z1;
                 const Cell op{plane_off(g, p), c.own};
                 CD ub;
                 if constexpr (PAIR == 1) {
                     // OpMxCk1, then the source of OpMxCk2 and its D3 partial
-                    const CF f0 = tof(fa);
+                    const CF f0 = tof(rot_pim<double>(k.pim, wa));
                     if (own_p) st_stream<float>(a.f0out + op, f0);
                     ub = sum21.comb(qu[(PH + H) % Q], f0);
-                    qs[H] = cscale<double>(t.b31, cmul<double>(a.dt, tod(f0)));
+                    qs[H] = cscale<double>(t.b31, hmul<double, GM>(a.dt, tod(f0)));
                 } else if constexpr (PAIR == 2) {
                     // OpMxCkMid<3>, then the source of OpMxCkMid<4> and its D5 partial
-                    const CD k3 = cmul<double>(a.dt, fa);
+                    const CD k3 = gmul<double, GM>(G, wa);
                     const CF k3f = tof(k3);
                     if (own_p) st_stream<float>(a.ka + op, k3f);
-                    const CD k1 = cmul<double>(a.dt, tod(pw.f0));
+                    const CD k1 = hmul<double, GM>(a.dt, tod(pw.f0));
                     const CD k2v = tod(pw.k2);
                     const CF d4 = tof(caxpy<double>(t.b43, k3, caxpy<double>(t.b42, k2v, cscale<double>(t.b41, k1))));
                     ub = sum1.comb(pw.p0, d4);
@@ -168,15 +171,15 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     // the pointwise inputs are loaded after the stencil (a set
                     // live across it spills)
                     load_pw(pw, plane_off(g, bcmap1(p, g.nz, g.bcs)));
-                    const CD k5 = cmul<double>(a.dt, fa);
-                    const CD k1 = cmul<double>(a.dt, tod(pw.f0));
+                    const CD k5 = gmul<double, GM>(G, wa);
+                    const CD k1 = hmul<double, GM>(a.dt, tod(pw.f0));
                     const CD k3v = tod(pw.k3), k4v = tod(pw.k4), k5v = tod(tof(k5));
                     const CF d6 = tof(caxpy<double>(
                         t.b65, k5,
                         caxpy<double>(t.b64, k4v,
                                       caxpy<double>(t.b63, k3v, caxpy<double>(t.b62, tod(pw.k2), cscale<double>(t.b61, k1))))));
                     ub = sum1.comb(pw.p0, d6);
-                    if (c.inner) {
+                    if (c.winner) {
                         const CD p0 = csub<double>(ub, tod(d6));  // as OpMxCkLast: u6 - D6
                         ldsS[rp][threadIdx.x] =
                             caxpy<double>(t.c4, k4v, caxpy<double>(t.c3, k3v, caxpy<double>(t.c1, k1, p0)));
@@ -193,7 +196,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                 qv[(PH + 2 * H) % Q] = ub;
                 ldsB[rp][c.lb] = ub;
             }
-            if (z >= c.z0 && c.inner) {
+            if (z >= c.z0 && c.winner) {
                 // second stage at plane z, inner tile (ldsB of plane z was
                 // written H iterations ago)
                 FusedSten<R, ORD, S::TXB, TERMS> sb;
@@ -202,7 +205,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
 #pragma unroll
                 for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[(PH + d) % Q];
                 sb.w = w;
-                const CD kb = cmul<double>(a.dt, rhs_eval<R>(k, sb, c.mx, c.my, z));
+                const CD kb = gmul<double, GM>(G, rhs_w<R>(k, sb, c.mx, c.my, z));
                 const Cell oz{plane_off(g, z), c.own};
                 if constexpr (PAIR <= 2) {
                     if (c.store) {
@@ -245,10 +248,10 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     }
 }
 
-template <int ORD, int TERMS, int PAIR>
+template <int ORD, int TERMS, int PAIR, int GM>
 __global__ __launch_bounds__(1024) void k_ckmx_pair(Geom g, StW<double> w, RhsC<double> k, CkMxArgs a,
                                                     double* __restrict__ partial) {
-    ckmx_pair_body<ORD, TERMS, PAIR>(g, w, k, a, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
+    ckmx_pair_body<ORD, TERMS, PAIR, GM>(g, w, k, a, partial, xcd_remap(blockIdx.x, g.nblocks), blockIdx.x);
 }
 
 template <int ORD>
