@@ -388,6 +388,17 @@ __global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_rk4_f1
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
+// Planes of loads in flight (1 or 2; FP64 keeps one: a second set spills)
+// and repeated rim columns (FusedCtx; with the trap term FP64 spills with
+// them) of k_rk4_f34.
+#ifndef GPE_F34_DEPTH
+#define GPE_F34_DEPTH 1
+#endif
+#ifndef GPE_F34_REP
+#define GPE_F34_REP 1
+#endif
+template <typename R> constexpr int kF34Depth = sizeof(R) == 4 ? GPE_F34_DEPTH : 1;
+template <typename R, int TERMS> constexpr bool kF34Rep = GPE_F34_REP != 0 && (sizeof(R) == 4 || TERMS == 0);
 template <typename R, int ORD, int TERMS, int GM, int OX_ = 64, int TYB_ = kFusedTYB>
 __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, const RhsC<R>& k,
                                              const C<R>* __restrict__ u2, const C<R>* __restrict__ psi,
@@ -407,7 +418,8 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
     // registers.
     constexpr bool kAccLds = sizeof(R) == 8;
     __shared__ C<R> ldsQ[kAccLds ? NB : 1][kAccLds ? S::OX * S::OY : 1];
-    const FusedCtx<ORD, true, OX_, TYB_, true> c(g, bid);
+    const FusedCtx<ORD, true, OX_, TYB_, kF34Rep<R, TERMS>> c(g, bid);
+    constexpr int DEPTH = kF34Depth<R>;
     if (c.z0 >= c.z1) return;
     const StW<R> w = scale_w<R>(w0, k.a);
     // u = q + G w for the steps dt, dt/3 and dt/6
@@ -450,11 +462,18 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
 #pragma unroll
         for (int s = 0; s < S::NSLOT; ++s) F.hr[s] = bld<R>(rh, ohal[s]);
     };
-    Pf F{};
-    fetch(F, start);
+    // DEPTH sets of loads in flight: iteration z consumes set z % depth
+    // and refills it for iteration z + depth
+    Pf FA{}, FB{};
+    fetch(FA, start);
     bst<R>(plane_res_m<R>(out, 0u, pbytes), kNoOff, mkc<R>(0.0, 0.0));  // as the iterations' store (see fused_march)
+    if constexpr (DEPTH == 2) {
+        fetch(FB, start + 1);
+        bst<R>(plane_res_m<R>(out, 0u, pbytes), kNoOff, mkc<R>(0.0, 0.0));
+    }
     auto step = [&](int z, auto phase) {
         constexpr int PH = decltype(phase)::value;
+        Pf& F = (DEPTH == 2 && PH % 2 == 1) ? FB : FA;
         const int p = z + H;
         qu[(PH + 2 * H) % Q] = F.qnu;
         qp[(PH + 2 * H) % Q] = F.qnp;
@@ -468,7 +487,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
         for (int s = 0; s < S::NSLOT; ++s)
             if (c.hv[s]) A[c.hl[s]] = F.hr[s];
         __syncthreads();
-        fetch(F, z + 1);
+        fetch(F, z + DEPTH);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
             FusedSten<R, ORD, S::LWA, TERMS> sa;
@@ -518,7 +537,7 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
         rz = rz == NB - 1 ? 0 : rz + 1;
         rp = rp == NB - 1 ? 0 : rp + 1;
     };
-    march_rotating<Q>(start, c.z1, step);
+    march_rotating<(DEPTH == 2 && Q % 2 == 1) ? 2 * Q : Q>(start, c.z1, step);
 }
 
 template <typename R, int ORD, int TERMS, int GM>
