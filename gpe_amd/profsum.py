@@ -71,6 +71,10 @@ def _one(pattern):
     return files[0]
 
 
+def _gm(g) -> str:
+    return "" if g in (None, "1") else (" imaginary-dt" if g == "2" else " complex-dt")
+
+
 def kernel_label(name: str) -> str:
     m = re.search(r"k_stencil<(\w+), (\d), (\d+), (\d+), gpe::(\w+)", name)
     ms = re.search(r"k_stencil_sum<(\d), (\d+), (\d+), gpe::(\w+)", name)
@@ -84,23 +88,25 @@ def kernel_label(name: str) -> str:
         if kind.startswith("mx_"):
             prec = "mixed"
         return f"{kind} {prec} order{order} tile{tx}x{ty}"
-    mf = re.search(r"k_rk4_f(12|34)<(double|float), (\d), (\d+)>", name)
+    # the last template argument of the fused kernels is the form of the step
+    # constant G = i pim dt: 1 real dt (real time), 2 imaginary dt, 0 complex
+    mf = re.search(r"k_rk4_f(12|34)<(double|float), (\d), (\d+)(?:, (\d))?>", name)
     if mf:
         prec = "c128" if mf.group(2) == "double" else "c64"
         h = int(mf.group(3)) // 2
         trap = " with-trap" if mf.group(4) != "0" else ""
-        return f"rk4_fused{mf.group(1)} {prec} order{mf.group(3)} tile{64 + 2 * h}x15{trap}"
-    mc = re.search(r"k_ck_pair<(double|float), (\d), (\d+), (\d)>", name)
+        return f"rk4_fused{mf.group(1)} {prec} order{mf.group(3)} tile{64 + 2 * h}x15{trap}{_gm(mf.group(5))}"
+    mc = re.search(r"k_ck_pair<(double|float), (\d), (\d+), (\d)(?:, (\d))?>", name)
     if mc:
         prec = "c128" if mc.group(1) == "double" else "c64"
         h = int(mc.group(2)) // 2
         trap = " with-trap" if mc.group(3) != "0" else ""
-        return f"{CK_PAIR[mc.group(4)]} {prec} order{mc.group(2)} tile{64 + 2 * h}x15{trap}"
-    mm = re.search(r"k_ckmx_pair<(\d), (\d+), (\d)>", name)
+        return f"{CK_PAIR[mc.group(4)]} {prec} order{mc.group(2)} tile{64 + 2 * h}x15{trap}{_gm(mc.group(5))}"
+    mm = re.search(r"k_ckmx_pair<(\d), (\d+), (\d)(?:, (\d))?>", name)
     if mm:
         h = int(mm.group(1)) // 2
         trap = " with-trap" if mm.group(2) != "0" else ""
-        return f"mx_{CK_PAIR[mm.group(3)]} mixed order{mm.group(1)} tile{64 + 2 * h}x15{trap}"
+        return f"mx_{CK_PAIR[mm.group(3)]} mixed order{mm.group(1)} tile{64 + 2 * h}x15{trap}{_gm(mm.group(4))}"
     m = re.search(r"gpe::(\w+)", name)
     return m.group(1) if m else name[:60]
 
