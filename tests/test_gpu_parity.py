@@ -332,6 +332,58 @@ def test_fused_rkck_attempt_matches_oracle_and_stage_kernels(orc, shape, order, 
         assert abs(ef[i] - es[i]) <= 1e-13 * es[i]
 
 
+G_FORMS = [(0.004 + 0j, "real-dt"), (-0.004j, "imaginary-dt"), (0.004 - 0.003j, "complex-dt")]
+
+
+@pytest.mark.parametrize("eqn", [1, 4])
+@pytest.mark.parametrize("dt", [g for g, _ in G_FORMS], ids=[n for _, n in G_FORMS])
+def test_fused_step_constant_forms(orc, dt, eqn):
+    """The fused kernels fold i*pim*dt into one constant G whose form selects
+    the arithmetic of every stage update: imaginary for a real dt, real for an
+    imaginary dt, general for a complex dt (the C ABI takes any complex dt).
+    Each form, with and without the trap term: complex128 RK4 steps and a
+    Cash-Karp attempt against the FP64 oracle (1e-12) and the stage kernels
+    (1e-13); complex64 and mixed storage against their own stage kernels at
+    the accuracy of their FP32 roundings (1e-6 for complex64 fields; 1e-9
+    for mixed storage, whose FP32 increments may round either way when the
+    FP64 products they are rounded from differ in the last bit)."""
+    kw = dict(g=1.0, mu=3.0, omx=0.2, omy=0.25, omz=0.3) if eqn == 4 else {}
+    p = RunParams(nx=30, ny=26, nz=22, xr=0.35 * 30, yr=0.35 * 26, zr=0.35 * 22, order=4, bcs=2 if eqn == 4 else 1,
+                  eqn_to_solve=eqn, **kw)
+    psi = smooth_field(p, seed=21)
+    exp, _ = orc.rk4_n(p, psi, dt, 4)
+    new_o, _ = orc.rkck(p, psi, 5 * dt)
+    for storage in ("c128", "c64", "mixed"):
+        q = p.replace(storage=storage)
+        res = {}
+        for fused in (True, False):
+            with Context(q) as ctx:
+                ctx.set_fused(fused)
+                if storage != "mixed":
+                    ctx.upload(psi)
+                    ctx.step("rk4", 4, StepInfo(dt))
+                    rk4 = ctx.download()
+                    prof = ctx.kernel_profile(lambda: ctx.step("rk4", 1, StepInfo(dt)))
+                    assert ("rk4_fused34" in prof) == fused
+                else:
+                    rk4 = None
+                ctx.upload(psi)
+                em, a2 = ctx.rk45_attempt(5 * dt, 5 * dt)
+                ctx.rk45_accept()
+                res[fused] = (rk4, em, ctx.download())
+        f, st = res[True], res[False]
+        tol = {"c128": 1e-13, "c64": 1e-6, "mixed": 1e-9}[storage]
+        print(f"{storage} dt={dt}: RK4 fused/stage "
+              f"{rel(f[0], st[0]) if f[0] is not None else 0:.2e}, attempt {rel(f[2], st[2]):.2e}")
+        if f[0] is not None:
+            assert rel(f[0], st[0]) < tol
+        assert rel(f[2], st[2]) < tol
+        if storage == "c128":
+            assert rel(f[0], exp) < 1e-12
+            assert rel(f[2], new_o) < 1e-12
+            assert abs(f[1] - st[1]) <= 1e-9 * st[1] + 1e-9
+
+
 def test_fused_rkck_rkqs_run_and_complex64():
     """rkqs with the fused attempt reproduces the stage kernels' step
     sequence (accepts, rejects, dt) over 12 steps from a too-large dt; with
