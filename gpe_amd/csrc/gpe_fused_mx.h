@@ -15,9 +15,10 @@
 //   pair 3: k5 = dt f(u5), D6 = b61 k1 + .. + b65 k5, u6 = psi0 + D6;
 //           k6 = dt f(u6), psi_new, err and the block maxima as OpMxCkLast
 //                                   reads psi0, D5 (stencil), F0, K2, K3, K4 (56 B)   writes psi_new (16 B)
-// 176 B per cell per attempt (the mixed stage kernels: 328 B).  Pairs 2 and 3
-// also read psi0 of the own column at the first stage's plane, two planes
-// after the stencil source read it (mostly from cache).
+// 176 B per cell per attempt (the mixed stage kernels: 328 B).  Pair 2
+// keeps psi0 of the own column from its stencil source's load in an LDS ring
+// until the first stage needs it, H planes later; pair 3 (its LDS is full)
+// reads it again at the first stage's plane, mostly from cache.
 #pragma once
 
 #include "gpe_fused.h"
@@ -59,6 +60,14 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     constexpr int NIN = S::OX * S::OY;
     __shared__ CD ldsS[P3 ? NB : 1][P3 ? NIN : 1];
     __shared__ CD ldsE[P3 ? NB : 1][P3 ? NIN : 1];
+    // pair 2: psi0 of the own column, planes p..p+2H-1, kept from the stencil
+    // source's load (psi0 + D3) until the first stage needs it at plane p;
+    // read again from memory two planes later it mostly missed L2 (1.36x
+    // the algorithmic bytes).  H+2 slots, so a plane's slot is written
+    // before the barrier of the iteration after the one that read it last.
+    constexpr bool kP0Ring = PAIR == 2;
+    constexpr int NR = H + 2;
+    __shared__ CD ldsR[kP0Ring ? NR : 1][kP0Ring ? S::NTB : 1];
     __shared__ double red_sh[16];
     const FusedCtx<ORD, PAIR != 1> c(g, bid);  // the map measured faster per pair
     const StW<double> w = scale_w<double>(w0, k.a);
@@ -87,9 +96,21 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
             qs[d] = mkc<double>(0.0, 0.0);
             qd[d] = 1.0;
         }
+        // ring slot of plane m: (m - start) % NR; iteration z writes plane
+        // z + 2H and reads plane p = z + H
+        int rw = (2 * H) % NR, rr = H % NR;
         if (c.active) {
 #pragma unroll
-            for (int d = 0; d < 2 * H; ++d) qu[d] = src(Cell{plane_off(g, bcmap1(start + d, g.nz, g.bcs)), c.col});
+            for (int d = 0; d < 2 * H; ++d) {
+                const Cell ci{plane_off(g, bcmap1(start + d, g.nz, g.bcs)), c.col};
+                if constexpr (kP0Ring) {
+                    const CD p0 = *(a.p0 + ci);
+                    qu[d] = sum1.comb(p0, *(a.d + ci));
+                    ldsR[d % NR][c.lb] = p0;
+                } else {
+                    qu[d] = src(ci);
+                }
+            }
         }
         struct Pw {
             CD p0;
@@ -98,7 +119,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         auto load_pw = [&](Pw& v, long long pb) {
             const Cell ci{pb, c.col};
             if constexpr (PAIR >= 2) {
-                v.p0 = *(a.p0 + ci);
+                if constexpr (!kP0Ring) v.p0 = *(a.p0 + ci);
                 v.f0 = *(a.f0 + ci);
                 v.k2 = *(a.k2 + ci);
             }
@@ -110,14 +131,21 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         // loads of iteration zz: centre of plane zz + 2H, halo and (pair 2)
         // the pointwise inputs of plane zz + H
         struct Pf {
-            CD qn;
+            CD qn;  // pair 2: psi0 of the centre (the source adds D at the step)
+            CF qd;  // pair 2: D3 of the centre
             CD hr[S::NSLOT];
             Pw pw;
         };
         auto fetch = [&](Pf& F, int zz) {
             const long long pb = plane_off(g, bcmap1(zz + H, g.nz, g.bcs));
             if (c.active) {
-                F.qn = src(Cell{plane_off(g, bcmap1(zz + 2 * H, g.nz, g.bcs)), c.col});
+                const Cell cc{plane_off(g, bcmap1(zz + 2 * H, g.nz, g.bcs)), c.col};
+                if constexpr (kP0Ring) {
+                    F.qn = *(a.p0 + cc);
+                    F.qd = *(a.d + cc);
+                } else {
+                    F.qn = src(cc);
+                }
                 if constexpr (PAIR == 2) load_pw(F.pw, pb);
             }
 #pragma unroll
@@ -129,7 +157,12 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
         auto step = [&](int z, auto phase) {
             constexpr int PH = decltype(phase)::value;
             const int p = z + H;
-            qu[(PH + 2 * H) % Q] = F.qn;
+            if constexpr (kP0Ring) {
+                qu[(PH + 2 * H) % Q] = sum1.comb(F.qn, F.qd);
+                if (c.active) ldsR[rw][c.lb] = F.qn;
+            } else {
+                qu[(PH + 2 * H) % Q] = F.qn;
+            }
             Pw pw = F.pw;
             CD* A = ldsA[ra];
             if (c.active) A[c.la] = qu[(PH + H) % Q];
@@ -164,7 +197,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
                     const CD k1 = hmul<double, GM>(a.dt, tod(pw.f0));
                     const CD k2v = tod(pw.k2);
                     const CF d4 = tof(caxpy<double>(t.b43, k3, caxpy<double>(t.b42, k2v, cscale<double>(t.b41, k1))));
-                    ub = sum1.comb(pw.p0, d4);
+                    ub = sum1.comb(ldsR[rr][c.lb], d4);
                     qs[H] = caxpy<double>(t.b53, tod(k3f), caxpy<double>(t.b52, k2v, cscale<double>(t.b51, k1)));
                 } else {
                     // OpMxCkMid<5>, then the source of OpMxCkLast and its partials;
@@ -235,6 +268,10 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
             ra ^= 1;
             rz = rz == NB - 1 ? 0 : rz + 1;
             rp = rp == NB - 1 ? 0 : rp + 1;
+            if constexpr (kP0Ring) {
+                rw = rw == NR - 1 ? 0 : rw + 1;
+                rr = rr == NR - 1 ? 0 : rr + 1;
+            }
         };
         march_rotating<Q>(start, c.z1, step);
     }
