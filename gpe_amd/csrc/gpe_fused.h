@@ -86,8 +86,8 @@ template <typename R, int ORD, int PAIR> constexpr int kRk4Waves = sizeof(R) == 
 //
 //
 // REP (a per-kernel choice, measured): the workgroup's threads beyond the
-// block's NTB columns (NT - NTB of them) each repeat a rim column of the last
-// row, so every lane is active and the per-plane branches on `active` fold
+// block's NTB columns (NT - NTB of them) each repeat a rim column, so every
+// lane is active and the per-plane branches on `active` fold
 // away (a repeated column computes and writes the same values to the same
 // LDS cells, and a rim column stores nothing); and with the inner-first map
 // and an inner tile of whole wavefronts, `winner` is the inner flag as a
@@ -108,8 +108,13 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB, 
     bool hv[S::NSLOT];
 
     __device__ __forceinline__ FusedCtx(const Geom& g, int bid) {
-        // with REP, threads beyond the block repeat rim columns of the last row
-        const int tid = !REP || (int)threadIdx.x < S::NTB ? (int)threadIdx.x : (int)threadIdx.x - (S::NT - S::NTB);
+        // with REP, threads beyond the block repeat rim columns: the last
+        // rim threads of the map (inner-first: the left and right rim
+        // columns; otherwise the last row)
+        constexpr int NIN = S::OX * S::OY;
+        constexpr int NREP = INNER_FIRST ? S::NTB - NIN : S::TXB;
+        const int t0 = (int)threadIdx.x;
+        const int tid = !REP || t0 < S::NTB ? t0 : S::NTB - 1 - (t0 - S::NTB) % NREP;
         const int ntxy = g.ntx * g.nty;
         const int txy = bid % ntxy, zc = bid / ntxy;
         // Tiles run in column groups 2 tiles wide (all rows of a group, then
@@ -130,8 +135,6 @@ template <int ORD, bool INNER_FIRST = true, int OX_ = 64, int TYB_ = kFusedTYB, 
         // runs wave-uniform; the others own the H-wide rim: the H rows above
         // and below, then the H columns left and right of each inner row.
         // Otherwise thread t owns column (t % TXB, t / TXB) of the block.
-        constexpr int NIN = S::OX * S::OY;
-        static_assert(S::NT - S::NTB <= S::TXB && S::NT - S::NTB <= S::NTB - NIN, "repeated columns must be rim columns");
         active = REP || tid < S::NTB;
         if (!INNER_FIRST) {
             lx = tid % S::TXB;
