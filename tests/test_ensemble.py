@@ -112,6 +112,25 @@ def test_c64_ensemble_matches_single_context():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dt", [-0.005j, 0.004 - 0.003j], ids=["imaginary-dt", "complex-dt"])
+@pytest.mark.parametrize("mode", ["fused", "fused_graph", "per_member_fused", "batched"])
+def test_modes_match_single_context_other_dt_forms(mode, dt):
+    """The fused batched kernels are instantiated per form of the step
+    constant i*pim*dt (real, imaginary, complex dt): every form against the
+    single-context run of each member (1e-14)."""
+    ms = members(4, 2)
+    fs = fields(ms, seed=5)
+    exp = [single(m, f, 4, dt) for m, f in zip(ms, fs)]
+    with Ensemble(ms) as ens:
+        for i, f in enumerate(fs):
+            ens.upload(i, f)
+        ens.step(mode, dt, 4)
+        for i in range(len(ms)):
+            got = ens.download(i)
+            assert np.linalg.norm(got - exp[i]) <= 1e-14 * np.linalg.norm(exp[i]), (mode, dt, i)
+
+
+@pytest.mark.gpu
 def test_fused_modes_alternate_buffers_and_replay():
     """Fused modes keep each member's state in one of two buffers: odd step
     counts, graph replays with eager fused steps between and downloads in
