@@ -195,6 +195,10 @@ __device__ __forceinline__ void ll_line(const LlGeom& g, long long line, int& j,
 __device__ __forceinline__ int ll_last_lane(unsigned long long mask) { return 63 - __clzll(mask); }
 
 // Pass 1: last[line] = (1, h2) of the line's last two-hit cell, or (0, ...).
+// The chunks of 64 cells are visited from the end of the line, and the first
+// chunk with a two-hit cell ends the search: its last such lane is the
+// line's last two-hit cell.  Where vortex lines are dense that is the last
+// chunk, so most lines evaluate 64 cells instead of nx - 2.
 __global__ __launch_bounds__(kLlThreads) void k_ll_last(const double2* __restrict__ psi, LlGeom g,
                                                         double* __restrict__ last) {
     const int lane = threadIdx.x & 63;
@@ -203,17 +207,18 @@ __global__ __launch_bounds__(kLlThreads) void k_ll_last(const double2* __restric
     int j, k;
     ll_line(g, line, j, k);
     double has = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
-    for (int i0 = 1; i0 <= g.nx - 2; i0 += 64) {
+    for (int i0 = 1 + ((g.nx - 3) / 64) * 64; i0 >= 1; i0 -= 64) {
         const int i = i0 + lane;
         LlCell c{};
         if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
         const unsigned long long w = __ballot(c.nhits >= 2);
-        if (w) {
+        if (w) {  // uniform per wavefront
             const int src = ll_last_lane(w);
             lx = __shfl(c.h2[0], src, 64);
             ly = __shfl(c.h2[1], src, 64);
             lz = __shfl(c.h2[2], src, 64);
             has = 1.0;
+            break;
         }
     }
     if (lane == 0) {
