@@ -16,7 +16,7 @@ GPE_C128, GPE_C64, GPE_MIXED = 0, 1, 2
 GPE_HOST_C128, GPE_HOST_C64 = 0, 1
 GPE_FIELD_STATE, GPE_FIELD_AUX = 0, 1
 GPE_ABI_VERSION = 2
-GPE_NKINDS = 35  # kernel kinds counted by gpe_prof_read
+GPE_NKINDS = 37  # kernel kinds counted by gpe_prof_read
 SCHEMES = {"euler": 0, "rk2": 1, "rk4": 2, "rk45": 3}
 DERIV_OPS = {"x": 0, "y": 1, "z": 2, "xx": 3, "yy": 4, "zz": 5, "lap": 6}
 

@@ -101,7 +101,7 @@ static const char* const kKindNames[GPE_NKINDS] = {
     "hamiltonian", "rk4_stage3", "mx_rk4_stage1", "mx_rk4_stage2_3", "mx_rk4_stage4", "mx_rkck_stage1",
     "mx_rkck_stage2", "mx_rkck_stage3", "mx_rkck_stage4", "mx_rkck_stage5", "mx_rkck_stage6_error",
     "rk4_fused12", "rk4_fused34", "momentum", "velocity", "rkck_fused12", "rkck_fused34", "rkck_fused56_error",
-    "mx_rkck_fused12", "mx_rkck_fused34", "mx_rkck_fused56_error"};
+    "mx_rkck_fused12", "mx_rkck_fused34", "mx_rkck_fused56_error", "rk2_fused", "euler2_fused"};
 
 static int fail(gpe_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -336,6 +336,14 @@ static bool fused_rk4(const gpe_ctx* c, int scheme) {
            (f == 0 || f == RHS_V);
 }
 
+// The fused RK2 step and the fused pair of Euler steps have the same
+// conditions as fused RK4.
+static bool fused_pair(const gpe_ctx* c, int scheme) {
+    const int f = make_rhs_p<double>(c->p, nullptr).flags;
+    return c->fused && (scheme == GPE_RK2 || scheme == GPE_EULER) &&
+           (c->p.storage == GPE_C128 || c->p.storage == GPE_C64) && (f == 0 || f == RHS_V);
+}
+
 // The fused Cash-Karp attempt has the same conditions, and also applies to
 // mixed storage.
 static bool fused_ck(const gpe_ctx* c) {
@@ -350,6 +358,7 @@ static int need_rk_buffers(gpe_ctx* c, int scheme) {
         if ((rc = dalloc(c, &c->uA, fb, "uA"))) return rc;
         return dalloc(c, &c->pnew, fb, "pnew");
     }
+    if (fused_pair(c, scheme)) return dalloc(c, &c->uA, fb, "uA");
     if (c->p.storage == GPE_MIXED && (scheme == GPE_RK4 || scheme == GPE_RK45)) {
         const int nmx = scheme == GPE_RK4 ? 3 : 7;
         for (int j = 0; j < nmx; ++j)
@@ -497,6 +506,34 @@ template <typename R, int ORD> static int step_rk4_fused(gpe_ctx* c, double dre,
         return rc;
     std::swap(c->psi, c->pnew);
     return 0;
+}
+
+struct FusedTagRk2 {
+    static constexpr int kKind = 35, kAccess = 2, kNRed = 0, kMode = RED_NONE;
+};
+struct FusedTagEuler2 {
+    static constexpr int kKind = 36, kAccess = 2, kNRed = 0, kMode = RED_NONE;
+};
+
+// One midpoint RK2 step (pk = PK_RK2) or two Euler steps (PK_EULER2) as one
+// fused kernel (gpe_fused.h): psi -> uA, swapped in.
+template <typename R, int ORD> static int step_pair_fused(gpe_ctx* c, double dre, double dim, int pk) {
+    const RhsC<R> k = make_rhs<R>(c);
+    const StW<R> w = make_w<R>(c->p);
+    const C<R> dt = cdt<R>(dre, dim, 1.0);
+    Geom g = make_geom<R>(c);
+    rk4_fused_tiles<ORD>(g.nx, g.ny, g.nz, c->ncu, g);
+    auto launch = [&](Geom gg) {
+        return launch_pair_fused<R, ORD>(gg, w, k, (const C<R>*)c->psi, (C<R>*)c->uA, dt, pk, c->stream);
+    };
+    const int rc = pk == PK_RK2 ? run_launch<FusedTagRk2>(c, g, launch) : run_launch<FusedTagEuler2>(c, g, launch);
+    if (rc) return rc;
+    std::swap(c->psi, c->uA);
+    return 0;
+}
+
+template <typename R> static int dispatch_pair(gpe_ctx* c, double dre, double dim, int pk) {
+    return c->p.order == 2 ? step_pair_fused<R, 2>(c, dre, dim, pk) : step_pair_fused<R, 4>(c, dre, dim, pk);
 }
 
 struct FusedCkTag1 {
@@ -1097,6 +1134,22 @@ int gpe_step(gpe_ctx* c, int scheme, int nsteps, gpe_step_info* io) {
     }
     const int nrhs = scheme == GPE_EULER ? 1 : scheme == GPE_RK2 ? 2 : 4;
     for (int s = 0; s < nsteps; ++s) {
+        if (fused_pair(c, scheme) && (scheme == GPE_RK2 || s + 1 < nsteps)) {
+            // one fused RK2 step, or two Euler steps in one kernel
+            const int pk = scheme == GPE_RK2 ? PK_RK2 : PK_EULER2;
+            rc = c->p.storage == GPE_C64 ? dispatch_pair<float>(c, io->dt_re, io->dt_im, pk)
+                                         : dispatch_pair<double>(c, io->dt_re, io->dt_im, pk);
+            if (rc) return rc;
+            const int nst = scheme == GPE_RK2 ? 1 : 2;
+            for (int q = 0; q < nst; ++q) {
+                io->t += io->dt_re;
+                io->im_t += fabs(io->dt_im);
+                io->n_accept += 1;
+                io->n_rhs += nrhs;
+            }
+            s += nst - 1;
+            continue;
+        }
         if (fused_rk4(c, scheme))
             rc = c->p.storage == GPE_C64
                      ? (c->p.order == 2 ? step_rk4_fused<float, 2>(c, io->dt_re, io->dt_im)

@@ -285,11 +285,18 @@ __device__ __forceinline__ void march_rotating(int start, int z1, Step step) {
 template <typename R, int ORD, int LW, int TERMS>
 using FusedSten = std::conditional_t<TERMS == 0, Sten<R, ORD, LW, false>, StenV<R, ORD, LW>>;
 
+// What the stages-1+2 body computes from psi0 (h1, h2 the steps of its two
+// stages): the first two stages of RK4, u2 = psi0 + dt/2 f(psi0 + dt/2 f(psi0));
+// one midpoint RK2 step (src/solve.f90:78-105), psi0 + dt f(psi0 + dt/2 f(psi0));
+// or two Euler steps (src/solve.f90:57-74), u1 + dt f(u1) with
+// u1 = psi0 + dt f(psi0).
+enum { PK_RK4 = 0, PK_RK2 = 1, PK_EULER2 = 2 };
+
 // GM: the form of G = i pim h of the stage updates (gaxpy).
-template <typename R, int ORD, int TERMS, int GM, int OX_ = 64, int TYB_ = kFusedTYB>
+template <typename R, int ORD, int TERMS, int GM, int OX_ = 64, int TYB_ = kFusedTYB, int PK = PK_RK4>
 __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, const RhsC<R>& k,
-                                             const C<R>* __restrict__ psi, C<R>* __restrict__ u2out, C<R> hdt,
-                                             int bid) {
+                                             const C<R>* __restrict__ psi, C<R>* __restrict__ u2out, C<R> h1,
+                                             C<R> h2, int bid) {
     using S = FusedShape<ORD, OX_, TYB_>;
     constexpr int H = S::H, D = GPE_FUSED_DEPTH;
     constexpr unsigned E = sizeof(C<R>);
@@ -298,7 +305,8 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, co
     const FusedCtx<ORD, true, OX_, TYB_, sizeof(R) == 4> c(g, bid);  // REP measured: FP32 faster, FP64 slower
     if (c.z0 >= c.z1) return;  // uniform per block
     const StW<R> w = scale_w<R>(w0, k.a);
-    const C<R> G = g_of<R>(k.pim, hdt);  // u = q + G w (dt/2 f)
+    const C<R> G = g_of<R>(k.pim, h1);  // u = q + G w: the first stage
+    const C<R> G2 = PK == PK_RK4 ? G : g_of<R>(k.pim, h2);  // the second
     const int start = c.z0 - 2 * H;
     // byte offsets within a plane; kNoOff where the lane has no access
     const unsigned pbytes = (unsigned)g.plane * E;
@@ -366,7 +374,8 @@ __device__ __forceinline__ void rk4_f12_body(const Geom& g, const StW<R>& w0, co
 #pragma unroll
             for (int d = 0; d <= 2 * H; ++d) sb.q[d] = qv[d];
             sb.w = w;
-            u2 = gaxpy<R, GM>(G, rhs_w<R>(k, sb, c.mx, c.my, z), qp[0]);
+            // the second stage's base: psi0, or (two Euler steps) u1 of plane z
+            u2 = gaxpy<R, GM>(G2, rhs_w<R>(k, sb, c.mx, c.my, z), PK == PK_EULER2 ? qv[H] : qp[0]);
         }
         // the store is issued on every path (kNoOff where there is no output)
         bst<R>(plane_res_m<R>(u2out, (unsigned)(second ? z : c.z0), pbytes), second ? oown : kNoOff, u2);
@@ -387,7 +396,28 @@ template <typename R, int ORD, int TERMS, int GM>
 __global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_rk4_f12(Geom g, StW<R> w, RhsC<R> k,
                                                                              const C<R>* __restrict__ psi,
                                                                              C<R>* __restrict__ u2out, C<R> hdt) {
-    rk4_f12_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, xcd_remap(blockIdx.x, g.nblocks));
+    rk4_f12_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB>(g, w, k, psi, u2out, hdt, hdt,
+                                                     xcd_remap(blockIdx.x, g.nblocks));
+}
+
+// One midpoint RK2 step (hdt = dt/2) or two Euler steps (hdt = dt) of the
+// fused block: psi -> out, 2 field accesses per cell instead of 5 (RK2) or 4
+// (two Euler steps) in the stage kernels.
+template <typename R, int ORD, int TERMS, int GM>
+__global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_rk2_fused(Geom g, StW<R> w, RhsC<R> k,
+                                                                               const C<R>* __restrict__ psi,
+                                                                               C<R>* __restrict__ out, C<R> hdt,
+                                                                               C<R> dt) {
+    rk4_f12_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB, PK_RK2>(g, w, k, psi, out, hdt, dt,
+                                                             xcd_remap(blockIdx.x, g.nblocks));
+}
+
+template <typename R, int ORD, int TERMS, int GM>
+__global__ __launch_bounds__(kRk4NT<ORD>, (kRk4Waves<R, ORD, 12>)) void k_euler2_fused(Geom g, StW<R> w, RhsC<R> k,
+                                                                                  const C<R>* __restrict__ psi,
+                                                                                  C<R>* __restrict__ out, C<R> dt) {
+    rk4_f12_body<R, ORD, TERMS, GM, kRk4OX, kRk4TYB, PK_EULER2>(g, w, k, psi, out, dt, dt,
+                                                                xcd_remap(blockIdx.x, g.nblocks));
 }
 
 // Stages 3+4 of the fused block `bid` of a grid.
@@ -838,7 +868,7 @@ __global__ __launch_bounds__(1024) void k_rk4_f12_batched(const FusedMember* __r
                                                           double2 hdt, int flip) {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         const FusedMember& M = mem[fused_member_of(mem, nmem, item)];
-        rk4_f12_body<double, ORD, 0, GM>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, item - M.begin);
+        rk4_f12_body<double, ORD, 0, GM>(M.g, M.w, M.k, flip ? M.alt : M.psi, M.u2, hdt, hdt, item - M.begin);
         __syncthreads();  // the next item reuses the LDS planes
     }
 }
@@ -863,6 +893,11 @@ hipError_t launch_rk4_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, co
 template <int ORD>
 hipError_t launch_rk4_fused_batched(const FusedMember* mem, int nmem, int total, int grid, double2 dt, int stage_pair,
                                     int flip, hipStream_t s);
+// one RK2 step (pk = PK_RK2) or two Euler steps (PK_EULER2): psi -> out, on
+// the tiling of rk4_fused_tiles (gpe_k_fused2.hip)
+template <typename R, int ORD>
+hipError_t launch_pair_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const C<R>* psi, C<R>* out, C<R> dt,
+                             int pk, hipStream_t s);
 template <typename R, int ORD>
 hipError_t launch_ck_fused(const Geom& g, const StW<R>& w, const RhsC<R>& k, const CkPairArgs<R>& a, int pair,
                            double* partial, hipStream_t s);  // g.general: the trap term

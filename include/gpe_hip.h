@@ -118,11 +118,14 @@ int gpe_get_tiles(const gpe_ctx* ctx, int* tx, int* ty, int* tz);
  * right-hand side whose only optional term is the trap V (no Urhs,
  * dissipation or eqn-3 mask): gpe_step with GPE_RK4 runs the four stages as
  * two kernels that each evaluate two stages (5 field accesses per cell
- * instead of 13), and each Cash-Karp attempt of GPE_RK45 (and
- * gpe_rk45_attempt) runs its six stages as three such kernels (18 accesses
- * instead of 36).  on = 0 selects the stage-by-stage kernels.  No
- * counterpart in the reference: a performance switch of the same rk4 and
- * rkck (src/solve.f90:109-154, 248-378). */
+ * instead of 13); with GPE_RK2 one kernel evaluates both stages of a step
+ * (2 accesses instead of 5); with GPE_EULER one kernel evaluates two steps
+ * (2 accesses instead of 4; an odd last step runs alone); each Cash-Karp
+ * attempt of GPE_RK45 (and gpe_rk45_attempt) runs its six stages as three
+ * such kernels (18 accesses instead of 36).  on = 0 selects the
+ * stage-by-stage kernels.  No counterpart in the reference: a performance
+ * switch of the same euler, rk2, rk4 and rkck (src/solve.f90:57-154,
+ * 248-378). */
 int gpe_set_fused(gpe_ctx* ctx, int on);
 
 /* Copy a host field into the device state.  The host array has extent
@@ -291,7 +294,7 @@ int gpe_renormalise(gpe_ctx* ctx, double norm);
  * for one kernel kind, the summed duration, the launch count and the
  * algorithmic HBM bytes of one launch (fields read + written per cell times
  * the cell count).  No reference counterpart: measurement support. */
-#define GPE_NKINDS 35
+#define GPE_NKINDS 37
 int gpe_prof_enable(gpe_ctx* ctx, int on);
 int gpe_prof_read(gpe_ctx* ctx, int kind, double* total_ms, int64_t* launches, double* bytes_per_launch);
 const char* gpe_kernel_kind_name(int kind);

@@ -332,6 +332,45 @@ def test_fused_rkck_attempt_matches_oracle_and_stage_kernels(orc, shape, order, 
         assert abs(ef[i] - es[i]) <= 1e-13 * es[i]
 
 
+@pytest.mark.parametrize("scheme,nsteps", [("euler", 1), ("euler", 2), ("euler", 5), ("rk2", 3)])
+@pytest.mark.parametrize("order,bcs,eqn,real_time", [(4, 1, 1, True), (2, 2, 4, True), (4, 2, 4, False)])
+def test_fused_euler_and_rk2(orc, scheme, nsteps, order, bcs, eqn, real_time):
+    """Euler steps fused in pairs and the midpoint RK2 step fused into one
+    kernel (the stages-1+2 body of the fused RK4 pair): against the FP64
+    oracle (1e-12) and the stage kernels (1e-13), with odd Euler step counts
+    (the last step runs alone), the trap, and imaginary time; complex64
+    against its stage kernels at FP32 accuracy (1e-6)."""
+    kw = dict(g=1.0, mu=2.0, omx=0.3, omy=0.2, omz=0.25) if eqn == 4 else {}
+    p = RunParams(nx=30, ny=26, nz=22, xr=0.3 * 30, yr=0.3 * 26, zr=0.3 * 22, order=order, bcs=bcs,
+                  eqn_to_solve=eqn, real_time=real_time, tau=0.004, **kw)
+    psi = smooth_field(p, seed=17)
+    dt = p.dt0()
+    exp, t = psi, (0.0, 0.0)
+    for _ in range(nsteps):
+        exp, t = orc.step(p, scheme, exp, dt, t)
+    fused_kind = "rk2_fused" if scheme == "rk2" else "euler2_fused"
+    for storage in ("c128", "c64"):
+        q = p.replace(storage=storage)
+        outs = {}
+        for fused in (True, False):
+            with Context(q) as ctx:
+                ctx.set_fused(fused)
+                ctx.upload(psi)
+                info = StepInfo(dt)
+                ctx.step(scheme, nsteps, info)
+                outs[fused] = (ctx.download(), info.t, info.im_t, info.n_accept, info.n_rhs)
+                prof = ctx.kernel_profile(lambda: ctx.step(scheme, 2, StepInfo(dt)))
+                assert (fused_kind in prof) == fused, (fused, prof.keys())
+        f, st = outs[True], outs[False]
+        assert f[1:] == st[1:]
+        assert abs(f[1] - t[0]) < 1e-15 and abs(f[2] - t[1]) < 1e-15
+        if storage == "c128":
+            assert rel(f[0], exp) < 1e-12
+            assert rel(f[0], st[0]) < 1e-13
+        else:
+            assert rel(f[0], st[0]) < 1e-6
+
+
 G_FORMS = [(0.004 + 0j, "real-dt"), (-0.004j, "imaginary-dt"), (0.004 - 0.003j, "complex-dt")]
 
 
