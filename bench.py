@@ -13,7 +13,8 @@ Prints one JSON line: value = Gcell-steps/s, plus the roofline of the
 dominant kernel (the one with the largest share of the step's time), timed
 live with HIP events around each launch on the context's stream, the
 four-stage-kernel run (13 field accesses per cell, the bandwidth-bound
-formulation), the complex64- and mixed-storage runs, the time per RKF45
+formulation), the complex64- and mixed-storage runs, the time per Euler and
+RK2 step of the fused and stage kernels (euler_rk2), the time per RKF45
 (Cash-Karp) attempt of the stage and fused kernels in complex128 and mixed
 storage on the same grid (rkf45_attempt), the
 time to T = 2 of plain complex128, mixed-storage, learned and complex64
@@ -106,6 +107,33 @@ def rkf45_attempts(p, psi, count):
                      "hbm_frac": round(attempt_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "kernels": kernel_table(prof)}
         print(f"[bench] rkf45 {spec}: {ms:.3f} ms/attempt", file=sys.stderr, flush=True)
+    return out
+
+
+def euler_rk2_steps(p, psi, steps):
+    """Euler and midpoint RK2 (src/solve.f90:57-105) on the same 512^3
+    workload: ms per step of the fused kernels (RK2: one kernel per step;
+    Euler: one kernel per two steps) and of the stage kernels, from the
+    per-launch event timing of `steps` steps after two untimed ones."""
+    from gpe_amd import Context, StepInfo
+
+    out = {}
+    for scheme in ("euler", "rk2"):
+        out[scheme] = {}
+        for fused in (True, False):
+            with Context(p) as ctx:
+                ctx.set_fused(fused)
+                ctx.upload(psi)
+                info = StepInfo(p.dt0())
+                ctx.step(scheme, 2, info)
+                prof = ctx.kernel_profile(lambda: ctx.step(scheme, steps, info))
+            ms = sum(v["avg_ms"] * v["launches"] for v in prof.values()) / steps
+            step_bytes = sum(v["bytes"] * v["launches"] for v in prof.values()) / steps
+            out[scheme]["fused" if fused else "stages"] = {
+                "ms_per_step": round(ms, 4), "gcell_steps_s": round(p.cells / (ms * 1e-3) / 1e9, 3),
+                "hbm_frac": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "kernels": kernel_table(prof)}
+            print(f"[bench] {scheme} {'fused' if fused else 'stage kernels'}: {ms:.3f} ms/step", file=sys.stderr,
+                  flush=True)
     return out
 
 
@@ -261,6 +289,8 @@ def main(argv=None) -> int:
                          "one-rank fallback runs 40)")
     ap.add_argument("--no-c64", action="store_true", help="skip the complex64-storage run")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-storage run")
+    ap.add_argument("--euler-rk2-steps", type=int, default=10,
+                    help="timed Euler and RK2 steps, fused and stage kernels (0: skip)")
     ap.add_argument("--rk45-steps", type=int, default=3,
                     help="timed RKF45 steps per Cash-Karp variant (c128 and mixed, stage and fused; 0: skip)")
     ap.add_argument("--ttt-T", type=float, default=2.0,
@@ -349,6 +379,8 @@ def main(argv=None) -> int:
                            "roofline": roofline(profm, dominant(profm),
                                                 load_traffic(f"traffic_{dominant(profm)}_mixed_{args.n}.json")),
                            "kernels": kernel_table(profm)}
+    if args.euler_rk2_steps > 0:
+        result["euler_rk2"] = euler_rk2_steps(p, psi, args.euler_rk2_steps)
     if args.rk45_steps > 0:
         result["rkf45_attempt"] = rkf45_attempts(p, psi, args.rk45_steps)
     if args.ttt_T > 0:
