@@ -53,34 +53,65 @@ __device__ __forceinline__ double pdf_edge(int i, double vmax, int nbins) {
     return (2.0 * (double)i) * vmax / (double)nbins;
 }
 
+// x > pdf_edge(i, vmax, nbins), with the division only where x lies within
+// a few ulp of the edge: t * inv_nb is the edge to within 2 ulp, so outside
+// 4 ulp of it the comparison is decided the same way.
+__device__ __forceinline__ bool above_edge(double x, int i, double vmax, int nbins, double inv_nb) {
+    const double t = (2.0 * (double)i) * vmax;
+    const double ea = t * inv_nb;
+    const double tol = 8.9e-16 * fabs(ea);
+    if (x > ea + tol) return true;
+    if (x < ea - tol) return false;
+    return x > t / (double)nbins;
+}
+
 // Histogram of the valid values per component, counted in LDS per block and
-// added into hist[a * nbins + (i + nbins/2 - 1)].
+// added into hist[a * nbins + (i + nbins/2 - 1)].  Lanes count into ncopy
+// interleaved copies of the bins (lane % ncopy; each copy padded by one word
+// so equal bins of different copies fall in different LDS banks): the
+// values pile up in a few central bins, and atomics of one wavefront to one
+// LDS word are serialised.
+constexpr int kHistMaxCopies = 8;
+constexpr int kHistLdsBytes = 64 * 1024;
+inline int hist_copies(int nbins) {
+    const int c = kHistLdsBytes / ((3 * nbins + 1) * (int)sizeof(unsigned int));
+    return c < 1 ? 1 : (c > kHistMaxCopies ? kHistMaxCopies : c);
+}
 __global__ __launch_bounds__(kDiagThreads) void k_vel_hist(const double* __restrict__ vel, long long n, int nbins,
-                                                          const double* __restrict__ vmax3,
+                                                          int ncopy, const double* __restrict__ vmax3,
                                                           unsigned long long* __restrict__ hist) {
-    extern __shared__ unsigned int hsh[];  // 3 * nbins
-    for (int j = threadIdx.x; j < 3 * nbins; j += kDiagThreads) hsh[j] = 0u;
+    extern __shared__ unsigned int hsh[];  // ncopy * (3 * nbins + 1)
+    const int stride = 3 * nbins + 1;
+    for (int j = threadIdx.x; j < ncopy * stride; j += kDiagThreads) hsh[j] = 0u;
     __syncthreads();
+    unsigned int* mine = hsh + (threadIdx.x % ncopy) * stride;
     const int lo = -nbins / 2 + 1, hi = nbins / 2;
+    const double inv_nb = 1.0 / (double)nbins;
     for (int a = 0; a < 3; ++a) {
         const double* v = vel + a * n;
         const double vmax = vmax3[a];
+        if (!(vmax > 0.0)) continue;  // uniform
+        const double scale = (double)nbins / (2.0 * vmax);
         for (long long i = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i < n;
              i += (long long)gridDim.x * kDiagThreads) {
             const double x = v[i];
-            if (!(x < kVelValid) || !(vmax > 0.0)) continue;
-            double q = ceil(x * (double)nbins / (2.0 * vmax));
+            if (!(x < kVelValid)) continue;
+            // a first guess of the bin, then settle on the bin whose edges (as
+            // the reference forms them) hold x
+            double q = ceil(x * scale);
             q = q < lo - 1 ? lo - 1 : (q > hi + 1 ? hi + 1 : q);
             int b = (int)q;
-            // settle on the bin whose edges (as the reference forms them) hold x
-            while (b <= hi && x > pdf_edge(b, vmax, nbins)) ++b;
-            while (b >= lo && x <= pdf_edge(b - 1, vmax, nbins)) --b;
-            if (b >= lo && b <= hi) atomicAdd(&hsh[a * nbins + (b - lo)], 1u);
+            while (b <= hi && above_edge(x, b, vmax, nbins, inv_nb)) ++b;
+            while (b >= lo && !above_edge(x, b - 1, vmax, nbins, inv_nb)) --b;
+            if (b >= lo && b <= hi) atomicAdd(&mine[a * nbins + (b - lo)], 1u);
         }
     }
     __syncthreads();
-    for (int j = threadIdx.x; j < 3 * nbins; j += kDiagThreads)
-        if (hsh[j]) atomicAdd(&hist[j], (unsigned long long)hsh[j]);
+    for (int j = threadIdx.x; j < 3 * nbins; j += kDiagThreads) {
+        unsigned int c = 0u;
+        for (int r = 0; r < ncopy; ++r) c += hsh[r * stride + j];
+        if (c) atomicAdd(&hist[j], (unsigned long long)c);
+    }
 }
 
 // Velocity correlation along x: block b sums, over its lines (y, z), the
@@ -191,8 +222,9 @@ hipError_t launch_vel_stats(const double* vel, long long n, double* partial, int
 
 hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,
                            int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_vel_hist, dim3(nblocks), dim3(kDiagThreads), (size_t)3 * nbins * sizeof(unsigned int), s, vel,
-                       n, nbins, vmax3, hist);
+    const int ncopy = hist_copies(nbins);
+    hipLaunchKernelGGL(k_vel_hist, dim3(nblocks), dim3(kDiagThreads),
+                       (size_t)ncopy * (3 * nbins + 1) * sizeof(unsigned int), s, vel, n, nbins, ncopy, vmax3, hist);
     return hipGetLastError();
 }
 

@@ -40,15 +40,18 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
     double e[kSpectrumShells], cnt[kSpectrumShells];
 #pragma unroll
     for (int m = 0; m < kSpectrumShells; ++m) e[m] = cnt[m] = 0.0;
-    const long long n = (long long)nx * ny * nz;
     const double l2 = log(2.0);
-    for (long long q = blockIdx.x * (long long)kSpThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kSpThreads) {
-        const int i = (int)(q % nx);
-        const long long r = q / nx;
-        const int j = (int)(r % ny), k = (int)(r / ny);
-        const long long k2 = mode_sq(i, nx) + mode_sq(j, ny) + mode_sq(k, nz);
+    // rows (j, k) block by block, x across the threads: the row's indices
+    // are formed once per row, not divided out of a linear index per mode
+    const long long nrow = (long long)ny * nz;
+    for (long long row = blockIdx.x; row < nrow; row += gridDim.x) {
+      const int j = (int)(row % ny), k = (int)(row / ny);
+      const long long kjk = mode_sq(j, ny) + mode_sq(k, nz);
+      const C<R>* __restrict__ ar = a + row * nx;
+      for (int i = threadIdx.x; i < nx; i += kSpThreads) {
+        const long long k2 = mode_sq(i, nx) + kjk;
         if (k2 == 0) continue;
-        const C<R> v = a[q];
+        const C<R> v = ar[i];
         const double p2 = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * inv_n;
         // smallest kk with kk^2 >= k2, i.e. sqrt(k2) <= kk
         long long s = (long long)sqrt((double)k2);
@@ -63,6 +66,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
                 cnt[m] += 1.0;
             }
         }
+      }
     }
     const int nb = gridDim.x;
 #pragma unroll
@@ -98,19 +102,20 @@ __global__ __launch_bounds__(kSpThreads) void k_tot_axes(int nx, int ny, int nz,
 __global__ __launch_bounds__(kSpThreads) void k_tot_partial(int nx, int ny, int nz, const double* __restrict__ sym,
                                                             double rho0, double* __restrict__ partial) {
     __shared__ double red[kSpThreads / 64];
-    const long long n = (long long)nx * ny * nz;
     const double* sx = sym;
     const double* sy = sym + nx;
     const double* sz = sym + nx + ny;
     double acc = 0.0;
-    for (long long q = blockIdx.x * (long long)kSpThreads + threadIdx.x; q < n; q += (long long)gridDim.x * kSpThreads) {
-        const int i = (int)(q % nx);
-        const long long r = q / nx;
-        const int j = (int)(r % ny), k = (int)(r / ny);
-        const double k2 = -(1.0 / 12.0) * (sx[i] + sy[j] + sz[k]);
-        if (k2 == 0.0) continue;
-        const double k4 = k2 * k2;
-        acc += (k2 + rho0) / (k4 + 2.0 * rho0 * k2);
+    const long long nrow = (long long)ny * nz;
+    for (long long row = blockIdx.x; row < nrow; row += gridDim.x) {
+        const int j = (int)(row % ny), k = (int)(row / ny);
+        const double yj = sy[j], zk = sz[k];
+        for (int i = threadIdx.x; i < nx; i += kSpThreads) {
+            const double k2 = -(1.0 / 12.0) * (sx[i] + yj + zk);
+            if (k2 == 0.0) continue;
+            const double k4 = k2 * k2;
+            acc += (k2 + rho0) / (k4 + 2.0 * rho0 * k2);
+        }
     }
     const double s = block_reduce<kSpThreads, RED_SUM>(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
