@@ -239,11 +239,20 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
     const long long per = (nlines + kLlScan - 1) / kLlScan;
     const long long l0 = (long long)threadIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
     double has = 0.0, x = 0.0, y = 0.0, z = 0.0;
-    for (long long l = l1 - 1; l >= l0; --l)
-        if (last[4 * l] != 0.0) {
-            has = 1.0, x = last[4 * l + 1], y = last[4 * l + 2], z = last[4 * l + 3];
-            break;
-        }
+    // the chunk's last line with a two-hit cell: groups from the end, their
+    // flags loaded together
+    for (long long g1 = l1; g1 > l0 && has == 0.0; g1 -= 8) {
+        const long long g0 = g1 - 8 > l0 ? g1 - 8 : l0;
+        double f[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) f[q] = g0 + q < g1 ? last[4 * (g0 + q)] : 0.0;
+#pragma unroll
+        for (int q = 7; q >= 0; --q)
+            if (has == 0.0 && f[q] != 0.0) {
+                const long long lq = g0 + q;
+                has = 1.0, x = last[4 * lq + 1], y = last[4 * lq + 2], z = last[4 * lq + 3];
+            }
+    }
     chunk_last[threadIdx.x][0] = has;
     chunk_last[threadIdx.x][1] = x;
     chunk_last[threadIdx.x][2] = y;
@@ -258,7 +267,23 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
     }
     __syncthreads();
     x = chunk_in[threadIdx.x][0], y = chunk_in[threadIdx.x][1], z = chunk_in[threadIdx.x][2];
-    for (long long l = l0; l < l1; ++l) {
+    // groups of kLlGroup lines: their entries are loaded before the serial
+    // update, so the loads of a group are in flight together
+    constexpr int kLlGroup = 8;
+    long long l = l0;
+    for (; l + kLlGroup <= l1; l += kLlGroup) {
+        double e[kLlGroup][4];
+#pragma unroll
+        for (int q = 0; q < kLlGroup; ++q)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) e[q][c] = last[4 * (l + q) + c];
+#pragma unroll
+        for (int q = 0; q < kLlGroup; ++q) {
+            carry[3 * (l + q) + 0] = x, carry[3 * (l + q) + 1] = y, carry[3 * (l + q) + 2] = z;
+            if (e[q][0] != 0.0) x = e[q][1], y = e[q][2], z = e[q][3];
+        }
+    }
+    for (; l < l1; ++l) {
         carry[3 * l + 0] = x, carry[3 * l + 1] = y, carry[3 * l + 2] = z;
         if (last[4 * l] != 0.0) x = last[4 * l + 1], y = last[4 * l + 2], z = last[4 * l + 3];
     }
