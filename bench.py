@@ -166,7 +166,8 @@ def time_to_t_leg(p, psi, T):
     method's tolerance tuned so its relative L2 error against an FP64 run at
     tolerance 1e-10 lands at or just under 1e-6; wall time (attempts,
     controller, syncs) and the speed-up over the fastest plain complex128
-    RKF45 (the reference's controller, fused Cash-Karp attempt)."""
+    RKF45 (the reference's controller, fused Cash-Karp attempt).  Also
+    classic RK4 (fused, complex128) at a fixed step T/N tuned the same way."""
     from gpe_amd.learned import DEFAULT_MODEL, LearnedController, load_model
     from gpe_amd.studies import time_to_t_tuned
 
@@ -174,13 +175,17 @@ def time_to_t_leg(p, psi, T):
     model = load_model(DEFAULT_MODEL)
     methods = [("plain-c128", "c128", None), ("plain-mixed", "mixed", None),
                ("learned-mixed", "mixed", lambda eps: LearnedController(model, pr.replace(eps=eps))),
-               ("plain-c64", "c64", None)]
+               ("plain-c64", "c64", None), ("rk4-c128", "rk4:c128", None)]
     t0 = time.perf_counter()
     runs = time_to_t_tuned(pr, psi, T, 0.01, methods)
     out = {"workload": f"random-phase turbulence {p.nx}^3, dx 0.3, 4th-order FD, eqn 1, RKF45 to T={T:g}",
            "error_target": 1e-6, "baseline": "plain-c128", "runs": runs,
            "leg_wall_s": round(time.perf_counter() - t0, 2)}
-    print("[bench] time to T: " + ", ".join(f"{k} {v['wall_s']:.2f} s (eps {v['eps']:.2g}, err "
+
+    def knob(v):
+        return f"dt {v['dt']:.3g}" if "dt" in v else f"eps {v['eps']:.2g}"
+
+    print("[bench] time to T: " + ", ".join(f"{k} {v['wall_s']:.2f} s ({knob(v)}, err "
                                            f"{v.get('rel_l2_vs_oracle', 0):.2e})" for k, v in runs.items()),
           file=sys.stderr, flush=True)
     return out

@@ -7,7 +7,8 @@
              FP64 (c128) run.
   time_to_t_tuned  RKF45 to a fixed final time with each method's
              tolerance tuned so its error lands at 1e-6 (the matched-error
-             comparison), against plain complex128 with the fused attempt.
+             comparison), against plain complex128 with the fused attempt;
+             also classic RK4 at a fixed step tuned the same way.
   time_to_t  RKF45 to a fixed final time on the random-phase workload:
              an FP64 reference at a tight tolerance, then plain c128 (six
              stage kernels per attempt, and fused: three kernels), c64 and
@@ -41,7 +42,7 @@ import numpy as np
 
 from . import ics
 from .context import Context, StepInfo
-from .controller import PlainController, integrate
+from .controller import PlainController, RunStats, integrate
 
 
 def rel_l2(a, b) -> float:
@@ -108,6 +109,30 @@ def run_rkf45(p, psi, T, dt0, eps, storage, controller=None, energy=False, name=
     return st, final
 
 
+def run_rk4_fixed(p, psi, T, dt, storage, fused=True, name=None):
+    """Classic RK4 at a fixed step onto T: N = ceil(T/dt) steps of T/N, issued
+    in one gpe_step call (no host round trip per step).  Wall time covers the
+    steps and the final sync, not IC or I/O."""
+    import math
+
+    n = max(1, math.ceil(T / dt * (1 - 1e-12)))
+    h = T / n
+    pp = p.replace(storage=storage, scheme="rk4", tau=h)
+    with Context(pp) as ctx:
+        ctx.set_fused(fused)
+        ctx.upload(psi)
+        info = StepInfo(dt=complex(h, 0.0))
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step("rk4", n, info)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        final = ctx.download()
+    st = RunStats(method=name or f"rk4-{storage}{'' if fused else '/stages'}-dt{h:.4g}", t=info.t,
+                  n_accept=n, n_rhs=4 * n, wall_s=wall)
+    return st, final, h
+
+
 def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned_model=None,
               storages=("c128/stages", "c128", "c64", "mixed/stages", "mixed")):
     """The baseline of the speed-ups is plain c128 with the six stage kernels
@@ -145,12 +170,15 @@ def time_to_t(n: int, T: float, dt0: float, oracle_eps: float, eps_list, learned
     return rows
 
 
-def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 0.8, max_runs: int = 5):
+def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 0.8, max_runs: int = 5,
+                  slope_max: float = 2.0, max_factor: float = 10.0):
     """Tolerance of a method tuned so that its error at T lands at or just
     under `target`: a secant search in log(eps) on log(error), starting from
     error ~ eps^slope.  run(eps) -> (stats, error).  Returns (eps, stats,
     error, tries) of the largest tolerance tried whose error is within the
-    target (tries: every (eps, error, wall_s) evaluated)."""
+    target (tries: every (eps, error, wall_s) evaluated).  The knob may be
+    any quantity the error grows with (a fixed step: slope ~4 for RK4); a
+    non-finite error (an unstable step) counts as an error of 1e3."""
     import math
 
     tries = []
@@ -159,6 +187,8 @@ def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 
     best = None
     for k in range(max_runs):
         st, err = run(eps)
+        if not math.isfinite(err):
+            err = 1e3
         tries.append((eps, err, st.wall_s))
         if err <= target and (best is None or eps > best[0]):
             best = (eps, st, err)
@@ -169,12 +199,13 @@ def tune_to_error(run, target: float = 1e-6, eps0: float = 1e-6, slope: float = 
             s = (lr - e_prev[1]) / (le - e_prev[0])
             if best is None and s < 0.15 and le < e_prev[0]:
                 break  # the error no longer follows the tolerance: a rounding floor above the target
-            slope = min(max(s, 0.3), 2.0)
+            slope = min(max(s, 0.3), slope_max)
         e_prev = (le, lr)
         # aim a little under the target so the landing run stays within it,
-        # moving eps by at most a factor 10 per run
+        # moving eps by at most max_factor per run
         step = (math.log(0.85 * target) - lr) / slope
-        eps = math.exp(le + min(max(step, -math.log(10.0)), math.log(10.0)))
+        lim = math.log(max_factor)
+        eps = math.exp(le + min(max(step, -lim), lim))
     if best is None:
         return None, None, None, tries
     return best[0], best[1], best[2], tries
@@ -184,15 +215,39 @@ def time_to_t_tuned(p, psi, T: float, dt0: float, methods, target: float = 1e-6,
                     baseline: str = "plain-c128"):
     """Wall time to T at a matched error: the FP64 reference at oracle_eps,
     then each method (name, variant spec, controller factory or None) with
-    its tolerance tuned to land at `target` relative L2 against it.  The
-    speed-up is against `baseline` (the fastest plain complex128 RKF45: the
-    reference's controller with the fused attempt)."""
+    its tolerance tuned to land at `target` relative L2 against it.  A spec
+    'rk4:<variant>' is classic RK4 at a fixed step instead of RKF45; its
+    step (T/N, starting from dt0) is tuned the same way.  The speed-up is
+    against `baseline` (the fastest plain complex128 RKF45: the reference's
+    controller with the fused attempt)."""
     ref, ref_final = run_rkf45(p, psi, T, dt0, oracle_eps, "c128", name=f"oracle-c128-eps{oracle_eps:g}")
     print(f"[time_to_t] reference eps {oracle_eps:g}: {ref.n_accept} steps, {ref.wall_s:.2f} s", file=sys.stderr,
           flush=True)
     out = {"oracle": dict(method=ref.method, eps=oracle_eps, wall_s=ref.wall_s, steps=ref.n_accept,
                           rejects=ref.n_reject, rhs=ref.n_rhs)}
     for name, spec, make_ctrl in methods:
+        if spec.startswith("rk4:"):
+            storage, fused = variant(spec[4:])
+
+            def run_dt(dt):
+                st, fin, h = run_rk4_fixed(p, psi, T, dt, storage, fused=fused)
+                err = rel_l2(fin.astype(np.complex128), ref_final)
+                print(f"[time_to_t] {name} dt {h:.4g}: {st.n_accept} steps, {st.wall_s:.2f} s, rel L2 {err:.3g}",
+                      file=sys.stderr, flush=True)
+                st.dt = h
+                return st, err
+
+            # steps of T/N: error ~ dt^4; an unstable step blows up, so move by at most 2x per run
+            dt, st, err, tries = tune_to_error(run_dt, target, eps0=dt0, slope=4.0, slope_max=6.0, max_factor=2.0)
+            tr = [dict(dt=e, rel_l2=r, wall_s=w) for e, r, w in tries]
+            if st is None:
+                e, r, w = min(tries, key=lambda x: x[1])
+                out[name] = dict(storage=spec, scheme="rk4", dt=e, wall_s=w, rel_l2_vs_oracle=r, within_target=False,
+                                 tuning=tr)
+            else:
+                out[name] = dict(storage=spec, scheme="rk4", dt=st.dt, wall_s=st.wall_s, steps=st.n_accept,
+                                 rejects=0, rhs=st.n_rhs, rel_l2_vs_oracle=err, within_target=True, tuning=tr)
+            continue
         storage, fused = variant(spec)
 
         def run(eps):
@@ -295,6 +350,9 @@ def main(argv=None):
                     help="comma-separated variants: c128, c64, mixed, each optionally /stages "
                          "(default: time_to_t c128/stages,c128,c64,mixed/stages,mixed; "
                          "attempts c128/stages,c128,mixed/stages,mixed,c64)")
+    ap.add_argument("--methods", default=None,
+                    help="time_to_t_tuned: comma-separated subset of plain-c128, plain-c128/stages, plain-mixed, "
+                         "learned-mixed, plain-c64, rk4-c128 (default: all)")
     ap.add_argument("--attempts", type=int, default=6)
     ap.add_argument("--copy-probe-mib", type=int, default=0)
     a = ap.parse_args(argv)
@@ -311,7 +369,10 @@ def main(argv=None):
         methods = [("plain-c128", "c128", None), ("plain-c128/stages", "c128/stages", None),
                    ("plain-mixed", "mixed", None),
                    ("learned-mixed", "mixed", lambda eps: LearnedController(model, p.replace(eps=eps))),
-                   ("plain-c64", "c64", None)]
+                   ("plain-c64", "c64", None), ("rk4-c128", "rk4:c128", None)]
+        if a.methods:
+            keep = a.methods.split(",")
+            methods = [m for m in methods if m[0] in keep]
         out = time_to_t_tuned(p, psi, a.T, a.dt0, methods, oracle_eps=a.oracle_eps)
         emit(dict(study="time_to_t_tuned", n=n, T=a.T, error_target=1e-6, baseline="plain-c128", runs=out))
     elif a.study == "attempts":
