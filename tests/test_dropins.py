@@ -143,3 +143,45 @@ def test_set_real_time_steers_rkqs_retries(orc, storage):
         info = StepInfo(dt0)
         ctx.step("rk45", 1, info)
         assert info.n_reject >= 1 and info.dt.real == 0.0 and info.dt.imag < 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,bcs", [(2, 1), (4, 2)])
+def test_python_module_mirror_with_halo_arrays(orc, order, bcs):
+    """The Python mirror of the reference's modules, called as the reference
+    calls them: derivs.deriv_* and variables.laplacian take the halo array
+    var(0:nx1, js-2:je+2, ks-2:ke+2) (src/derivs.f90:20-21,
+    src/variables.f90:55-71); solve.solver(var_in, var_out) advances t, im_t
+    and dt in `parameters` (src/solve.f90:25-53).  Against the oracle."""
+    from gpe_amd import derivs as D
+    from gpe_amd import solve as S
+
+    p = RunParams(nx=18, ny=14, nz=12, xr=3.0, yr=2.5, zr=2.0, order=order, bcs=bcs, eqn_to_solve=1,
+                  scheme="rk4", tau=0.004)
+    psi, _ = fields(p)
+    halo = np.zeros((p.nz + 4, p.ny + 4, p.nx), dtype=np.complex128)
+    halo[2:-2, 2:-2, :] = psi
+    P.setup(p)
+    try:
+        for fn, op in [(D.deriv_x, 0), (D.deriv_y, 1), (D.deriv_z, 2), (D.deriv_xx, 3), (D.deriv_yy, 4),
+                       (D.deriv_zz, 5)]:
+            out = np.empty(p.shape, dtype=np.complex128)
+            fn(halo, out)
+            exp = orc.deriv(p, op, psi)
+            assert np.linalg.norm(out - exp) <= 1e-13 * np.linalg.norm(exp), fn.__name__
+        lap = V.laplacian(halo)
+        exp = orc.deriv(p, 6, psi)
+        assert np.linalg.norm(lap - exp) <= 1e-13 * np.linalg.norm(exp)
+        x = psi
+        for n in range(3):
+            var_out = np.empty(p.shape, dtype=np.complex128)
+            h = np.zeros_like(halo)
+            h[2:-2, 2:-2, :] = x
+            S.solver(h, var_out)
+            x, (t, im_t) = orc.step(p, "rk4", x, p.dt0(), (t, im_t) if n else (0.0, 0.0))
+            assert np.linalg.norm(var_out - x) <= 1e-13 * np.linalg.norm(x)
+            assert P.t == t and P.im_t == im_t and P.dt == p.dt0()
+            x = var_out
+    finally:
+        P.ctx.close()
+        P.ctx = None
