@@ -85,6 +85,21 @@ __device__ __forceinline__ int ll_cross(const double* w, double* xs, double* ys)
     return n;
 }
 
+// Re psi (or Im psi) of one strict sign at all eight corners of cell
+// (i, j, k): ll_cell returns at once with no hit and no ending.
+__device__ __forceinline__ bool ll_quiet(const double2* __restrict__ psi, int nx, int ny, int i, int j, int k) {
+    bool upos = true, uneg = true, vpos = true, vneg = true;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double2 z = psi[((long long)(k + (q >> 2)) * ny + (j + ((q >> 1) & 1))) * nx + (i + (q & 1))];
+        upos = upos && z.x > 0.0;
+        uneg = uneg && z.x < 0.0;
+        vpos = vpos && z.y > 0.0;
+        vneg = vneg && z.y < 0.0;
+    }
+    return upos || uneg || vpos || vneg;
+}
+
 __device__ LlCell ll_cell(const double2* __restrict__ psi, int nx, int ny, int i, int j, int k) {
 #pragma clang fp contract(off)
     LlCell r;
@@ -137,6 +152,16 @@ __device__ LlCell ll_cell(const double2* __restrict__ psi, int nx, int ny, int i
                 r.ended = 2;
                 return r;
             }
+        // ll_cross keeps at most one crossing per edge with a sign change, so
+        // a face with fewer than two such edges in u or in v has no hit:
+        // skipped before any division
+        int nu = 0, nv = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            nu += u[q] != 0.0 && u[q] * u[q + 1] < 0.0;
+            nv += v[q] != 0.0 && v[q] * v[q + 1] < 0.0;
+        }
+        if (nu < 2 || nv < 2) continue;
         double xu[2], yu[2], xv[2], yv[2];
         if (ll_cross(u, xu, yu) < 2 || ll_cross(v, xv, yv) < 2) continue;
         double den = xv[1] * (yu[0] - yu[1]) + xv[0] * (yu[1] - yu[0]) + (xu[0] - xu[1]) * (yv[0] - yv[1]);
@@ -289,42 +314,91 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
     }
 }
 
+// Segments of kLlSeg chunks: most cells are quiet (ll_quiet), but nearly
+// every chunk of 64 holds a few that are not, so a chunk evaluated lane by
+// lane runs the face rules for all of its lanes.  Per segment the wavefront
+// first ranks its non-quiet cells in cell order (ballots), evaluates them 64
+// at a time on all lanes, adds the contributions that need no neighbour
+// (ended cells, two-hit cells) and keeps in LDS what the ordered pass needs:
+// whether the cell has two hits (its h2 feeds later single-hit cells) or one
+// (its h1).  The ordered pass then resolves the single-hit cells chunk by
+// chunk as before.
+constexpr int kLlSeg = 4;
+constexpr int kLlWaves = kLlThreads / 64;
+__device__ __forceinline__ void ll_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Pass 3: partial[block] = the block's sum of cell contributions.
 __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict__ psi, LlGeom g,
                                                        const double* __restrict__ carry, double* __restrict__ partial) {
-    __shared__ double red[kLlThreads / 64];
-    const int lane = threadIdx.x & 63;
-    const long long line = (long long)blockIdx.x * (kLlThreads / 64) + (threadIdx.x >> 6);
+    __shared__ double red[kLlWaves];
+    __shared__ int sl_i[kLlWaves][kLlSeg * 64];          // non-quiet cells (i), then their flags
+    __shared__ double sl_h[kLlWaves][3][kLlSeg * 64];    // h2 of a two-hit cell, h1 of a single-hit one
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long line = (long long)blockIdx.x * kLlWaves + wv;
+    const unsigned long long below = (1ull << lane) - 1ull;
     double acc = 0.0;
     if (line < g.nlines) {  // uniform per wavefront
         int j, k;
         ll_line(g, line, j, k);
         double cx = carry[3 * line + 0], cy = carry[3 * line + 1], cz = carry[3 * line + 2];
-        for (int i0 = 1; i0 <= g.nx - 2; i0 += 64) {
-            const int i = i0 + lane;
-            LlCell c{};
-            if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
-            const unsigned long long w = __ballot(c.nhits >= 2);
-            // the last two-hit lane before this one, else the carry
-            const unsigned long long before = w & ((1ull << lane) - 1ull);
-            const int src = before ? ll_last_lane(before) : lane;
-            const double sx = __shfl(c.h2[0], src, 64), sy = __shfl(c.h2[1], src, 64), sz = __shfl(c.h2[2], src, 64);
-            if (c.ended == 1) {
-                acc += 0.5;
-            } else if (c.ended == 2) {
-                acc += 0.25;
-            } else if (c.nhits >= 2) {
-                acc += ll_pair(c.h1, c.h2);
-            } else if (c.nhits == 1) {
-                const double h2[3] = {before ? sx : cx, before ? sy : cy, before ? sz : cz};
-                acc += ll_pair(c.h1, h2);
+        for (int s0 = 1; s0 <= g.nx - 2; s0 += 64 * kLlSeg) {
+            unsigned long long m[kLlSeg];
+            int base[kLlSeg];
+            int nh = 0;
+#pragma unroll
+            for (int c = 0; c < kLlSeg; ++c) {
+                const int i = s0 + 64 * c + lane;
+                const bool busy = i <= g.nx - 2 && !ll_quiet(psi, g.nx, g.ny, i, j, k);
+                m[c] = __ballot(busy);
+                base[c] = nh;
+                if (busy) sl_i[wv][nh + __popcll(m[c] & below)] = i;
+                nh += __popcll(m[c]);
             }
-            if (w) {
-                const int s = ll_last_lane(w);
-                cx = __shfl(c.h2[0], s, 64);
-                cy = __shfl(c.h2[1], s, 64);
-                cz = __shfl(c.h2[2], s, 64);
+            ll_wave_sync();
+            for (int r0 = 0; r0 < nh; r0 += 64) {
+                const int q = r0 + lane;
+                if (q < nh) {
+                    const LlCell c = ll_cell(psi, g.nx, g.ny, sl_i[wv][q], j, k);
+                    const bool two = c.nhits >= 2, single = c.ended == 0 && c.nhits == 1;
+                    if (c.ended == 1) acc += 0.5;
+                    else if (c.ended == 2) acc += 0.25;
+                    else if (two) acc += ll_pair(c.h1, c.h2);
+                    sl_i[wv][q] = (two ? 1 : 0) | (single ? 2 : 0);
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) sl_h[wv][d][q] = two ? c.h2[d] : c.h1[d];
+                }
             }
+            ll_wave_sync();
+#pragma unroll
+            for (int c = 0; c < kLlSeg; ++c) {
+                int fl = 0;
+                double hv[3] = {0.0, 0.0, 0.0};
+                if ((m[c] >> lane) & 1ull) {
+                    const int q = base[c] + __popcll(m[c] & below);
+                    fl = sl_i[wv][q];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) hv[d] = sl_h[wv][d][q];
+                }
+                const unsigned long long w = __ballot(fl & 1);
+                // the last two-hit lane before this one, else the carry
+                const unsigned long long bw = w & below;
+                const int src = bw ? ll_last_lane(bw) : lane;
+                const double sx = __shfl(hv[0], src, 64), sy = __shfl(hv[1], src, 64), sz = __shfl(hv[2], src, 64);
+                if (fl & 2) {
+                    const double h2[3] = {bw ? sx : cx, bw ? sy : cy, bw ? sz : cz};
+                    acc += ll_pair(hv, h2);
+                }
+                if (w) {
+                    const int s = ll_last_lane(w);
+                    cx = __shfl(hv[0], s, 64);
+                    cy = __shfl(hv[1], s, 64);
+                    cz = __shfl(hv[2], s, 64);
+                }
+            }
+            ll_wave_sync();  // the next segment reuses sl_i and sl_h
         }
     }
     const double s = block_reduce<kLlThreads, RED_SUM>(acc, red);
