@@ -153,7 +153,13 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_partial(const double* __re
 // broadcast read of v(i + ii) and K + K reads of v(i + r) per step instead of
 // two LDS reads per product.  Lines are zero-padded in LDS to nx + 2K, so
 // the products past the line's end add exact zeros and need no masks.  The
-// four wavefronts' sums are added in a fixed order.
+// four wavefronts' sums are added in a fixed order.  Element e of a line
+// sits at e + e / K (one unused slot after every K): the lanes' windows,
+// K elements apart, are then K + 1 doubles apart, an odd stride that spreads
+// a read of the 64 lanes over all LDS banks (K apart, they fell on a few).
+template <int K> __device__ __forceinline__ int vcf_slot(int e) { return e + e / K; }
+template <int K> constexpr int vcf_line_len(int nx) { return (nx + 2 * K) + (nx + 2 * K) / K + 1; }
+
 template <int K>
 __global__ __launch_bounds__(kDiagThreads) void k_vcf_lags(const double* __restrict__ vx, int nx, long long nlines,
                                                           double* __restrict__ partial) {
@@ -161,7 +167,7 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_lags(const double* __restr
     constexpr int NW = kDiagThreads / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lw = nx + 2 * K;
-    double* L = lines + wv * lw;
+    double* L = lines + wv * vcf_line_len<K>(nx);
     const long long per = (nlines + gridDim.x - 1) / gridDim.x;
     const long long l0 = (long long)blockIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
     const int r0 = lane * K;
@@ -171,18 +177,19 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_lags(const double* __restr
     for (long long lg = l0; lg < l1; lg += NW) {
         const long long l = lg + wv;
         __syncthreads();
-        for (int i = lane; i < lw; i += 64) L[i] = (l < l1 && i < nx) ? vx[l * nx + i] : 0.0;
+        for (int i = lane; i < lw; i += 64) L[vcf_slot<K>(i)] = (l < l1 && i < nx) ? vx[l * nx + i] : 0.0;
         __syncthreads();
         if (l < l1 && r0 < nx) {
             double b[2 * K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) b[j] = L[r0 + j];
+            for (int j = 0; j < K; ++j) b[j] = L[vcf_slot<K>(r0) + j];
             for (int i0 = 0; i0 < nx - r0; i0 += K) {
+                const int sb = vcf_slot<K>(i0 + r0 + K), sa = vcf_slot<K>(i0);  // i0, r0: multiples of K
 #pragma unroll
-                for (int j = 0; j < K; ++j) b[K + j] = L[i0 + r0 + K + j];
+                for (int j = 0; j < K; ++j) b[K + j] = L[sb + j];
 #pragma unroll
                 for (int ii = 0; ii < K; ++ii) {
-                    const double a = L[i0 + ii];
+                    const double a = L[sa + ii];
 #pragma unroll
                     for (int q = 0; q < K; ++q) acc[q] += a * b[ii + q];
                 }
@@ -230,7 +237,7 @@ hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const doub
 
 template <int K>
 static void launch_vcf_lags(const double* vx, int nx, long long nlines, double* partial, int nblocks, hipStream_t s) {
-    const size_t lds = (size_t)(kDiagThreads / 64) * (nx + 2 * K) * sizeof(double);
+    const size_t lds = (size_t)(kDiagThreads / 64) * vcf_line_len<K>(nx) * sizeof(double);
     hipLaunchKernelGGL((k_vcf_lags<K>), dim3(nblocks), dim3(kDiagThreads), lds, s, vx, nx, nlines, partial);
 }
 
