@@ -12,9 +12,12 @@
 // passes:
 //   1. one wavefront per x-line (j, k) finds the line's last two-hit cell;
 //   2. one block scans the lines in order: the carry into each line;
-//   3. one wavefront per line evaluates its cells in chunks of 64, resolves
+//   3. one wavefront per line walks its cells in chunks of 64, resolves
 //      single-hit cells against the last two-hit lane before them in the
 //      chunk (ballot + shuffle) or the carry, and sums the contributions.
+// In pass 3 the face rules run only for the cells that are not quiet
+// (ll_quiet), pooled over the block's four lines and evaluated on all of its
+// threads.
 // The cell arithmetic keeps the reference's operation order with
 // contraction off, so every cell's contribution and hits are bit-identical
 // to the reference's; only the order of the final sum differs.
@@ -219,6 +222,9 @@ __device__ __forceinline__ void ll_line(const LlGeom& g, long long line, int& j,
 
 __device__ __forceinline__ int ll_last_lane(unsigned long long mask) { return 63 - __clzll(mask); }
 
+constexpr int kLlWaves = kLlThreads / 64;
+constexpr int kLlWaveBits = 28;  // pooled cell: i | wavefront << 28 (i < nx < 2^28)
+
 // Pass 1: last[line] = (1, h2) of the line's last two-hit cell, or (0, ...).
 // The chunks of 64 cells are visited from the end of the line, and the first
 // chunk with a two-hit cell ends the search: its last such lane is the
@@ -314,73 +320,85 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
     }
 }
 
-// Segments of kLlSeg chunks: most cells are quiet (ll_quiet), but nearly
-// every chunk of 64 holds a few that are not, so a chunk evaluated lane by
-// lane runs the face rules for all of its lanes.  Per segment the wavefront
-// first ranks its non-quiet cells in cell order (ballots), evaluates them 64
-// at a time on all lanes, adds the contributions that need no neighbour
-// (ended cells, two-hit cells) and keeps in LDS what the ordered pass needs:
-// whether the cell has two hits (its h2 feeds later single-hit cells) or one
-// (its h1).  The ordered pass then resolves the single-hit cells chunk by
-// chunk as before.
+// Segments of kLlSeg chunks: per segment each wavefront ranks the
+// non-quiet cells of its line (most cells are quiet, but nearly every chunk
+// of 64 holds a few that are not, and a chunk evaluated lane by lane runs
+// the branchy face rules for all of its lanes), the block lays the four
+// lists end to end in LDS (entry: i and the wavefront) and evaluates them on
+// all threads, adding the contributions that need no
+// neighbour (ended cells, two-hit cells) and keeping in LDS what the ordered
+// pass needs: whether a cell has two hits (its h2 feeds later single-hit
+// cells) or one (its h1).  Each wavefront then resolves its line's
+// single-hit cells chunk by chunk in cell order.
 constexpr int kLlSeg = 4;
-constexpr int kLlWaves = kLlThreads / 64;
-__device__ __forceinline__ void ll_wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Pass 3: partial[block] = the block's sum of cell contributions.
 __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict__ psi, LlGeom g,
                                                        const double* __restrict__ carry, double* __restrict__ partial) {
+    constexpr int NP = kLlWaves * kLlSeg * 64;
     __shared__ double red[kLlWaves];
-    __shared__ int sl_i[kLlWaves][kLlSeg * 64];          // non-quiet cells (i), then their flags
-    __shared__ double sl_h[kLlWaves][3][kLlSeg * 64];    // h2 of a two-hit cell, h1 of a single-hit one
+    __shared__ int pool_i[NP];          // pooled cells (i | wavefront << 28), then their flags
+    __shared__ double pool_h[3][NP];    // h2 of a two-hit cell, h1 of a single-hit one
+    __shared__ int cnt[kLlWaves];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const long long line = (long long)blockIdx.x * kLlWaves + wv;
+    const bool valid = line < g.nlines;  // uniform per wavefront
     const unsigned long long below = (1ull << lane) - 1ull;
-    double acc = 0.0;
-    if (line < g.nlines) {  // uniform per wavefront
-        int j, k;
+    int j = 1, k = 1;
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    if (valid) {
         ll_line(g, line, j, k);
-        double cx = carry[3 * line + 0], cy = carry[3 * line + 1], cz = carry[3 * line + 2];
-        for (int s0 = 1; s0 <= g.nx - 2; s0 += 64 * kLlSeg) {
-            unsigned long long m[kLlSeg];
-            int base[kLlSeg];
-            int nh = 0;
+        cx = carry[3 * line + 0], cy = carry[3 * line + 1], cz = carry[3 * line + 2];
+    }
+    double acc = 0.0;
+    for (int s0 = 1; s0 <= g.nx - 2; s0 += 64 * kLlSeg) {  // uniform per block
+        unsigned long long m[kLlSeg];
+        int base[kLlSeg];
+        int nw = 0;
 #pragma unroll
-            for (int c = 0; c < kLlSeg; ++c) {
-                const int i = s0 + 64 * c + lane;
-                const bool busy = i <= g.nx - 2 && !ll_quiet(psi, g.nx, g.ny, i, j, k);
-                m[c] = __ballot(busy);
-                base[c] = nh;
-                if (busy) sl_i[wv][nh + __popcll(m[c] & below)] = i;
-                nh += __popcll(m[c]);
-            }
-            ll_wave_sync();
-            for (int r0 = 0; r0 < nh; r0 += 64) {
-                const int q = r0 + lane;
-                if (q < nh) {
-                    const LlCell c = ll_cell(psi, g.nx, g.ny, sl_i[wv][q], j, k);
-                    const bool two = c.nhits >= 2, single = c.ended == 0 && c.nhits == 1;
-                    if (c.ended == 1) acc += 0.5;
-                    else if (c.ended == 2) acc += 0.25;
-                    else if (two) acc += ll_pair(c.h1, c.h2);
-                    sl_i[wv][q] = (two ? 1 : 0) | (single ? 2 : 0);
+        for (int c = 0; c < kLlSeg; ++c) {
+            const int i = s0 + 64 * c + lane;
+            m[c] = __ballot(valid && i <= g.nx - 2 && !ll_quiet(psi, g.nx, g.ny, i, j, k));
+            base[c] = nw;
+            nw += __popcll(m[c]);
+        }
+        if (lane == 0) cnt[wv] = nw;
+        __syncthreads();
+        int off = 0, nh = 0;
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) sl_h[wv][d][q] = two ? c.h2[d] : c.h1[d];
-                }
-            }
-            ll_wave_sync();
+        for (int w = 0; w < kLlWaves; ++w) {
+            off += w < wv ? cnt[w] : 0;
+            nh += cnt[w];
+        }
+#pragma unroll
+        for (int c = 0; c < kLlSeg; ++c)
+            if ((m[c] >> lane) & 1ull)
+                pool_i[off + base[c] + __popcll(m[c] & below)] = (s0 + 64 * c + lane) | (wv << kLlWaveBits);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nh; q += kLlThreads) {
+            const int e = pool_i[q], w = e >> kLlWaveBits;
+            int jq, kq;
+            ll_line(g, (long long)blockIdx.x * kLlWaves + w, jq, kq);
+            const LlCell c = ll_cell(psi, g.nx, g.ny, e & ((1 << kLlWaveBits) - 1), jq, kq);
+            const bool two = c.nhits >= 2, single = c.ended == 0 && c.nhits == 1;
+            if (c.ended == 1) acc += 0.5;
+            else if (c.ended == 2) acc += 0.25;
+            else if (two) acc += ll_pair(c.h1, c.h2);
+            pool_i[q] = (two ? 1 : 0) | (single ? 2 : 0);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) pool_h[d][q] = two ? c.h2[d] : c.h1[d];
+        }
+        __syncthreads();
+        if (valid) {
 #pragma unroll
             for (int c = 0; c < kLlSeg; ++c) {
                 int fl = 0;
                 double hv[3] = {0.0, 0.0, 0.0};
                 if ((m[c] >> lane) & 1ull) {
-                    const int q = base[c] + __popcll(m[c] & below);
-                    fl = sl_i[wv][q];
+                    const int q = off + base[c] + __popcll(m[c] & below);
+                    fl = pool_i[q];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) hv[d] = sl_h[wv][d][q];
+                    for (int d = 0; d < 3; ++d) hv[d] = pool_h[d][q];
                 }
                 const unsigned long long w = __ballot(fl & 1);
                 // the last two-hit lane before this one, else the carry
@@ -392,14 +410,14 @@ __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict
                     acc += ll_pair(hv, h2);
                 }
                 if (w) {
-                    const int s = ll_last_lane(w);
-                    cx = __shfl(hv[0], s, 64);
-                    cy = __shfl(hv[1], s, 64);
-                    cz = __shfl(hv[2], s, 64);
+                    const int src2 = ll_last_lane(w);
+                    cx = __shfl(hv[0], src2, 64);
+                    cy = __shfl(hv[1], src2, 64);
+                    cz = __shfl(hv[2], src2, 64);
                 }
             }
-            ll_wave_sync();  // the next segment reuses sl_i and sl_h
         }
+        __syncthreads();  // the next segment rewrites cnt and the pool
     }
     const double s = block_reduce<kLlThreads, RED_SUM>(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
