@@ -1497,7 +1497,9 @@ int gpe_condensed_particles(gpe_ctx* c, double row[7], double eta[7], int64_t nh
     if ((rc = norm_mass(c, &M, 0))) return rc;
     const int nb = 512;
     const int nhalf = nx / 2;
-    if ((rc = ensure_partial(c, (size_t)nb * (2 * kSpectrumShells + nhalf + 1) + nx + ny + nz))) return rc;
+    // block partials, the k-grid symbols, then the per-k2 spectrum table (4-byte entries)
+    const size_t tab_at = (size_t)nb * (2 * kSpectrumShells + nhalf + 1) + nx + ny + nz;
+    if ((rc = ensure_partial(c, tab_at + (size_t)(spectrum_table_len(nx, ny, nz) + 1) / 2))) return rc;
     double d[3];
     spacing(c->p, d);
     const double rho0 = n0 / V;
@@ -1523,10 +1525,10 @@ int gpe_condensed_particles(gpe_ctx* c, double row[7], double eta[7], int64_t nh
     }
     if (dbl) {
         HIPCHK(c, launch_spectrum<double>((const double2*)c->fbuf, nx, ny, nz, nhalf, c->partial, nb, c->sres,
-                                          c->stream));
+                                          (unsigned*)(c->partial + tab_at), c->stream));
     } else {
         HIPCHK(c, launch_spectrum<float>((const float2*)c->fbuf, nx, ny, nz, nhalf, c->partial, nb, c->sres,
-                                         c->stream));
+                                         (unsigned*)(c->partial + tab_at), c->stream));
     }
     std::vector<double> h(nrows);
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -124,9 +124,11 @@ hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double*
 // Spectral diagnostics (gpe_spectral.hip); out holds 2 kSpectrumShells + nhalf
 // values: shell sums, shell counts, F bins.
 constexpr int kSpectrumShells = 7;
+// entries of the per-k2 table launch_spectrum fills and reads (k2 = 0 .. max)
+long long spectrum_table_len(int nx, int ny, int nz);
 template <typename R>
 hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, double* partial, int nb, double* out,
-                           hipStream_t s);
+                           unsigned* tab, hipStream_t s);
 hipError_t launch_condensed_tot(int nx, int ny, int nz, const double d[3], double rho0, double* partial, int nb,
                                 double* out, hipStream_t s);
 

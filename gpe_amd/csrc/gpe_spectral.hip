@@ -27,12 +27,36 @@ __device__ __forceinline__ long long mode_sq(int i, int n) {
     return m * m;
 }
 
+// tab[k2] for k2 = 0 .. max: the F bin s (the smallest kk with
+// sqrt(k2) <= kk; 0 when k2 = 0 or s > nhalf) and, in bits 24..31, the log2
+// shell m with |log2(sqrt(k2) / (2 pi))| in [m, m + 1) (kSpectrumShells when
+// none).  Every mode of one k2 shares them, so the sqrt, the search and the
+// log of each mode become one table read; the arithmetic is the per-mode
+// arithmetic it replaces.
+__global__ __launch_bounds__(kSpThreads) void k_spectrum_table(long long len, int nhalf, unsigned* __restrict__ tab) {
+    const long long k2 = (long long)blockIdx.x * kSpThreads + threadIdx.x;
+    if (k2 >= len) return;
+    unsigned sbin = 0, m = kSpectrumShells;
+    if (k2 > 0) {
+        long long s = (long long)sqrt((double)k2);
+        while (s * s < k2) ++s;
+        while (s > 1 && (s - 1) * (s - 1) >= k2) --s;
+        sbin = s <= nhalf ? (unsigned)s : 0u;
+        const double lk = fabs(log(0.5 * sqrt((double)k2) / kSpPi) / log(2.0));
+        for (int q = kSpectrumShells - 1; q >= 0; --q)
+            if (lk < (double)(q + 1) && lk >= (double)q) m = (unsigned)q;
+    }
+    tab[k2] = sbin | (m << 24);
+}
+
 // Per block b, rows of partial[row * nb + b]: 0..6 the shell sums of |a|^2,
 // 7..13 the shell counts, 14 + (s - 1) for s = 1..nhalf the sum of |a|^2 over
 // the modes whose smallest kk with k2 <= kk^2 is s.
 template <typename R>
 __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __restrict__ a, int nx, int ny, int nz,
-                                                                 int nhalf, double inv_n, double* __restrict__ partial) {
+                                                                 int nhalf, double inv_n,
+                                                                 const unsigned* __restrict__ tab,
+                                                                 double* __restrict__ partial) {
     extern __shared__ double hsh[];  // nhalf bins, then kSpThreads / 64 reduction slots
     double* red = hsh + nhalf;
     for (int s = threadIdx.x; s < nhalf; s += kSpThreads) hsh[s] = 0.0;
@@ -40,7 +64,6 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
     double e[kSpectrumShells], cnt[kSpectrumShells];
 #pragma unroll
     for (int m = 0; m < kSpectrumShells; ++m) e[m] = cnt[m] = 0.0;
-    const double l2 = log(2.0);
     // rows (j, k) block by block, x across the threads: the row's indices
     // are formed once per row, not divided out of a linear index per mode
     const long long nrow = (long long)ny * nz;
@@ -53,15 +76,12 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
         if (k2 == 0) continue;
         const C<R> v = ar[i];
         const double p2 = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * inv_n;
-        // smallest kk with kk^2 >= k2, i.e. sqrt(k2) <= kk
-        long long s = (long long)sqrt((double)k2);
-        while (s * s < k2) ++s;
-        while (s > 1 && (s - 1) * (s - 1) >= k2) --s;
-        if (s <= nhalf) atomicAdd(&hsh[s - 1], p2);
-        const double lk = fabs(log(0.5 * sqrt((double)k2) / kSpPi) / l2);
+        const unsigned t = tab[k2];
+        const unsigned sbin = t & 0xFFFFFFu, sh = t >> 24;
+        if (sbin) atomicAdd(&hsh[sbin - 1], p2);
 #pragma unroll
         for (int m = 0; m < kSpectrumShells; ++m) {
-            if (lk < (double)(m + 1) && lk >= (double)m) {
+            if (sh == (unsigned)m) {
                 e[m] += p2;
                 cnt[m] += 1.0;
             }
@@ -131,13 +151,24 @@ __global__ __launch_bounds__(kSpThreads) void k_sum_rows(const double* __restric
     out[row] = s;
 }
 
+long long spectrum_table_len(int nx, int ny, int nz) {
+    auto mx = [](int n) {
+        const int n1 = n - 1, h = n1 / 2 + 1 < n1 ? n1 / 2 + 1 : n1;
+        return (long long)h * h;
+    };
+    return mx(nx) + mx(ny) + mx(nz) + 1;
+}
+
 template <typename R>
 hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, double* partial, int nb, double* out,
-                           hipStream_t s) {
+                           unsigned* tab, hipStream_t s) {
     const double inv_n = 1.0 / ((double)nx * (double)ny * (double)nz);
     const size_t lds = ((size_t)nhalf + kSpThreads / 64) * sizeof(double);
+    const long long len = spectrum_table_len(nx, ny, nz);
+    hipLaunchKernelGGL(k_spectrum_table, dim3((unsigned)((len + kSpThreads - 1) / kSpThreads)), dim3(kSpThreads), 0, s,
+                       len, nhalf, tab);
     hipLaunchKernelGGL((k_spectrum_partial<R>), dim3(nb), dim3(kSpThreads), lds, s, a, nx, ny, nz, nhalf, inv_n,
-                       partial);
+                       (const unsigned*)tab, partial);
     const int nrows = 2 * kSpectrumShells + nhalf;
     hipLaunchKernelGGL(k_sum_rows, dim3((nrows + kSpThreads - 1) / kSpThreads), dim3(kSpThreads), 0, s, partial, nrows,
                        nb, out);
@@ -156,7 +187,9 @@ hipError_t launch_condensed_tot(int nx, int ny, int nz, const double d[3], doubl
     return hipGetLastError();
 }
 
-template hipError_t launch_spectrum<double>(const double2*, int, int, int, int, double*, int, double*, hipStream_t);
-template hipError_t launch_spectrum<float>(const float2*, int, int, int, int, double*, int, double*, hipStream_t);
+template hipError_t launch_spectrum<double>(const double2*, int, int, int, int, double*, int, double*, unsigned*,
+                                            hipStream_t);
+template hipError_t launch_spectrum<float>(const float2*, int, int, int, int, double*, int, double*, unsigned*,
+                                           hipStream_t);
 
 }  // namespace gpe
