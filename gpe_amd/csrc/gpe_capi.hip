@@ -207,23 +207,7 @@ template <typename R> static Geom make_geom(const gpe_ctx* c) {
 
 template <typename R> static RhsC<R> make_rhs(const gpe_ctx* c) { return make_rhs_p<R>(c->p, c->dtab); }
 
-template <typename R> static CkTab<R> make_tab() {
-    // Cash-Karp coefficients (src/solve.f90:255-281)
-    const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0, c6 = 512.0 / 1771.0;
-    CkTab<R> t;
-    t.b21 = (R)0.2; t.b31 = (R)0.075; t.b32 = (R)0.225; t.b41 = (R)0.3; t.b42 = (R)-0.9; t.b43 = (R)1.2;
-    t.b51 = (R)(-11.0 / 54.0); t.b52 = (R)2.5; t.b53 = (R)(-70.0 / 27.0); t.b54 = (R)(35.0 / 27.0);
-    t.b61 = (R)(1631.0 / 55296.0); t.b62 = (R)(175.0 / 512.0); t.b63 = (R)(575.0 / 13824.0);
-    t.b64 = (R)(44275.0 / 110592.0); t.b65 = (R)(253.0 / 4096.0);
-    t.c1 = (R)c1; t.c3 = (R)c3; t.c4 = (R)c4; t.c6 = (R)c6;
-    t.dc1 = c1 - 2825.0 / 27648.0;
-    t.dc3 = c3 - 18575.0 / 48384.0;
-    t.dc4 = c4 - 13525.0 / 55296.0;
-    t.dc5 = c5 - 277.0 / 14336.0;
-    t.dc6 = c6 - 0.25;
-    (void)c2;
-    return t;
-}
+template <typename R> static CkTab<R> make_tab() { return ck_tab<R>(); }
 
 static int ensure_partial(gpe_ctx* c, size_t ndoubles) {
     if (c->partial_cap >= ndoubles) return 0;

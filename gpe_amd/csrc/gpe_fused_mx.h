@@ -74,7 +74,7 @@ __device__ __forceinline__ void ckmx_pair_body(const Geom& g, const StW<double>&
     const CD G = g_of<double>(k.pim, a.dt);  // k = dt f = G w
     const SumSrc sum1{a.p0, a.d, mkc<double>(1.0, 0.0)};
     const SumSrc sum21{a.p0, a.f0, a.s21};
-    const CkTab<double>& t = a.t;
+    constexpr CkTab<double> t = ck_tab<double>();  // constants, not kernel arguments held in registers
     double red0 = 0.0, red1 = 0.0;
     auto src = [&](Cell ci) -> CD {
         if constexpr (PAIR == 1) return *(a.p0 + ci);
