@@ -1420,7 +1420,7 @@ int gpe_linelength(gpe_ctx* c, double* length) {
     if (!length) return fail(c, GPE_EINVAL, "gpe_linelength: output is NULL");
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const long long nl = linelength_lines(ny, nz), nb = linelength_blocks(ny, nz);
-    int rc = dalloc(c, (void**)&c->llbuf, (size_t)(7 * nl + nb + 1) * sizeof(double), "line length");
+    int rc = dalloc(c, (void**)&c->llbuf, (size_t)(8 * nl + 2 * nb + 1) * sizeof(double), "line length");
     if (rc) return rc;
     const double2* src = (const double2*)sel_field(c);
     if (c->p.storage == GPE_C64) {
@@ -1429,7 +1429,7 @@ int gpe_linelength(gpe_ctx* c, double* length) {
         src = (const double2*)c->llpsi;
     }
     double* last = c->llbuf;
-    double* carry = last + 4 * nl;
+    double* carry = last + 5 * nl;
     double* partial = carry + 3 * nl;
     HIPCHK(c, launch_linelength(src, nx, ny, nz, last, carry, partial, c->dres, c->stream));
     double s;

@@ -113,8 +113,8 @@ hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const doub
 hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,
                       hipStream_t s);
 
-// Line length (gpe_linelength.hip): last holds 4, carry 3 doubles per x-line
-// ((ny-2)(nz-2) lines), partial linelength_blocks doubles; out[0] = the sum
+// Line length (gpe_linelength.hip): last holds 5, carry 3 doubles per x-line
+// ((ny-2)(nz-2) lines), partial 2 linelength_blocks doubles; out[0] = the sum
 // of the cell contributions (times dx gives the reference's value).
 long long linelength_lines(int ny, int nz);
 long long linelength_blocks(int ny, int nz);

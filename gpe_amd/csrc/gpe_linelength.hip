@@ -10,12 +10,17 @@
 // pairs it with the second hit of the last earlier cell (in k, j, i order)
 // that recorded two.  That one serial dependency is carried here in three
 // passes:
-//   1. one wavefront per x-line (j, k) finds the line's last two-hit cell;
+//   1. one wavefront per x-line (j, k) walks its cells in chunks of 64 and
+//      sums their contributions, resolving a single-hit cell against the
+//      last two-hit lane before it in the chunk (ballot + shuffle) or the
+//      line's last two-hit cell so far; the single-hit cells before the
+//      line's first two-hit cell ("leading" cells) need the carry and are
+//      only counted.  It also records the line's last two-hit cell;
 //   2. one block scans the lines in order: the carry into each line;
-//   3. one wavefront per line walks its cells in chunks of 64, resolves
-//      single-hit cells against the last two-hit lane before them in the
-//      chunk (ballot + shuffle) or the carry, and sums the contributions.
-// In pass 3 the face rules run only for the cells that are not quiet
+//   3. one wavefront per line that has leading cells walks its start again,
+//      chunk by chunk up to the chunk of its first two-hit cell, and adds
+//      them against the carry.
+// In pass 1 the face rules run only for the cells that are not quiet
 // (ll_quiet), pooled over the block's four lines and evaluated on all of its
 // threads.
 // The cell arithmetic keeps the reference's operation order with
@@ -225,40 +230,8 @@ __device__ __forceinline__ int ll_last_lane(unsigned long long mask) { return 63
 constexpr int kLlWaves = kLlThreads / 64;
 constexpr int kLlWaveBits = 28;  // pooled cell: i | wavefront << 28 (i < nx < 2^28)
 
-// Pass 1: last[line] = (1, h2) of the line's last two-hit cell, or (0, ...).
-// The chunks of 64 cells are visited from the end of the line, and the first
-// chunk with a two-hit cell ends the search: its last such lane is the
-// line's last two-hit cell.  Where vortex lines are dense that is the last
-// chunk, so most lines evaluate 64 cells instead of nx - 2.
-__global__ __launch_bounds__(kLlThreads) void k_ll_last(const double2* __restrict__ psi, LlGeom g,
-                                                        double* __restrict__ last) {
-    const int lane = threadIdx.x & 63;
-    const long long line = (long long)blockIdx.x * (kLlThreads / 64) + (threadIdx.x >> 6);
-    if (line >= g.nlines) return;  // uniform per wavefront
-    int j, k;
-    ll_line(g, line, j, k);
-    double has = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
-    for (int i0 = 1 + ((g.nx - 3) / 64) * 64; i0 >= 1; i0 -= 64) {
-        const int i = i0 + lane;
-        LlCell c{};
-        if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
-        const unsigned long long w = __ballot(c.nhits >= 2);
-        if (w) {  // uniform per wavefront
-            const int src = ll_last_lane(w);
-            lx = __shfl(c.h2[0], src, 64);
-            ly = __shfl(c.h2[1], src, 64);
-            lz = __shfl(c.h2[2], src, 64);
-            has = 1.0;
-            break;
-        }
-    }
-    if (lane == 0) {
-        last[4 * line + 0] = has;
-        last[4 * line + 1] = lx;
-        last[4 * line + 2] = ly;
-        last[4 * line + 3] = lz;
-    }
-}
+// last[] entries per line: has, h2 (3), leading single-hit cells
+constexpr int kLlLast = 5;
 
 // Pass 2 (one block of 1024 threads): carry[line] = the second hit of the
 // last two-hit cell of the lines before it, (0, 0, 0) before any.
@@ -276,12 +249,12 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
         const long long g0 = g1 - 8 > l0 ? g1 - 8 : l0;
         double f[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) f[q] = g0 + q < g1 ? last[4 * (g0 + q)] : 0.0;
+        for (int q = 0; q < 8; ++q) f[q] = g0 + q < g1 ? last[kLlLast * (g0 + q)] : 0.0;
 #pragma unroll
         for (int q = 7; q >= 0; --q)
             if (has == 0.0 && f[q] != 0.0) {
                 const long long lq = g0 + q;
-                has = 1.0, x = last[4 * lq + 1], y = last[4 * lq + 2], z = last[4 * lq + 3];
+                has = 1.0, x = last[kLlLast * lq + 1], y = last[kLlLast * lq + 2], z = last[kLlLast * lq + 3];
             }
     }
     chunk_last[threadIdx.x][0] = has;
@@ -307,7 +280,7 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
 #pragma unroll
         for (int q = 0; q < kLlGroup; ++q)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) e[q][c] = last[4 * (l + q) + c];
+            for (int c = 0; c < 4; ++c) e[q][c] = last[kLlLast * (l + q) + c];
 #pragma unroll
         for (int q = 0; q < kLlGroup; ++q) {
             carry[3 * (l + q) + 0] = x, carry[3 * (l + q) + 1] = y, carry[3 * (l + q) + 2] = z;
@@ -316,7 +289,7 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
     }
     for (; l < l1; ++l) {
         carry[3 * l + 0] = x, carry[3 * l + 1] = y, carry[3 * l + 2] = z;
-        if (last[4 * l] != 0.0) x = last[4 * l + 1], y = last[4 * l + 2], z = last[4 * l + 3];
+        if (last[kLlLast * l] != 0.0) x = last[kLlLast * l + 1], y = last[kLlLast * l + 2], z = last[kLlLast * l + 3];
     }
 }
 
@@ -332,9 +305,11 @@ __global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__
 // single-hit cells chunk by chunk in cell order.
 constexpr int kLlSeg = 4;
 
-// Pass 3: partial[block] = the block's sum of cell contributions.
+// Pass 1: partial[block] = the block's sum of the contributions of all but
+// the leading single-hit cells; last[line] = (1, h2) of the line's last
+// two-hit cell, or (0, ...), then the number of leading cells.
 __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict__ psi, LlGeom g,
-                                                       const double* __restrict__ carry, double* __restrict__ partial) {
+                                                       double* __restrict__ last, double* __restrict__ partial) {
     constexpr int NP = kLlWaves * kLlSeg * 64;
     __shared__ double red[kLlWaves];
     __shared__ int pool_i[NP];          // pooled cells (i | wavefront << 28), then their flags
@@ -345,11 +320,11 @@ __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict
     const bool valid = line < g.nlines;  // uniform per wavefront
     const unsigned long long below = (1ull << lane) - 1ull;
     int j = 1, k = 1;
+    if (valid) ll_line(g, line, j, k);
+    // the line's last two-hit cell so far (seen: uniform per wavefront)
     double cx = 0.0, cy = 0.0, cz = 0.0;
-    if (valid) {
-        ll_line(g, line, j, k);
-        cx = carry[3 * line + 0], cy = carry[3 * line + 1], cz = carry[3 * line + 2];
-    }
+    bool seen = false;
+    int nlead = 0;
     double acc = 0.0;
     for (int s0 = 1; s0 <= g.nx - 2; s0 += 64 * kLlSeg) {  // uniform per block
         unsigned long long m[kLlSeg];
@@ -401,23 +376,62 @@ __global__ __launch_bounds__(kLlThreads) void k_ll_sum(const double2* __restrict
                     for (int d = 0; d < 3; ++d) hv[d] = pool_h[d][q];
                 }
                 const unsigned long long w = __ballot(fl & 1);
-                // the last two-hit lane before this one, else the carry
+                // the last two-hit lane before this one, else the line's last
+                // two-hit cell so far, else (a leading cell) the carry
                 const unsigned long long bw = w & below;
                 const int src = bw ? ll_last_lane(bw) : lane;
                 const double sx = __shfl(hv[0], src, 64), sy = __shfl(hv[1], src, 64), sz = __shfl(hv[2], src, 64);
-                if (fl & 2) {
+                const bool defer = (fl & 2) && !bw && !seen;
+                if ((fl & 2) && !defer) {
                     const double h2[3] = {bw ? sx : cx, bw ? sy : cy, bw ? sz : cz};
                     acc += ll_pair(hv, h2);
                 }
+                nlead += __popcll(__ballot(defer));
                 if (w) {
                     const int src2 = ll_last_lane(w);
                     cx = __shfl(hv[0], src2, 64);
                     cy = __shfl(hv[1], src2, 64);
                     cz = __shfl(hv[2], src2, 64);
+                    seen = true;
                 }
             }
         }
         __syncthreads();  // the next segment rewrites cnt and the pool
+    }
+    if (valid && lane == 0) {
+        last[kLlLast * line + 0] = seen ? 1.0 : 0.0;
+        last[kLlLast * line + 1] = cx;
+        last[kLlLast * line + 2] = cy;
+        last[kLlLast * line + 3] = cz;
+        last[kLlLast * line + 4] = (double)nlead;
+    }
+    const double s = block_reduce<kLlThreads, RED_SUM>(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Pass 3: partial[block] = the block's sum over its lines' leading
+// single-hit cells, each against the carry into its line: a line that has
+// them evaluates its cells again from the start, chunk by chunk, up to the
+// chunk of its first two-hit cell.
+__global__ __launch_bounds__(kLlThreads) void k_ll_lead(const double2* __restrict__ psi, LlGeom g,
+                                                        const double* __restrict__ last,
+                                                        const double* __restrict__ carry, double* __restrict__ partial) {
+    __shared__ double red[kLlWaves];
+    const int lane = threadIdx.x & 63;
+    const long long line = (long long)blockIdx.x * kLlWaves + (threadIdx.x >> 6);
+    double acc = 0.0;
+    if (line < g.nlines && last[kLlLast * line + 4] != 0.0) {  // uniform per wavefront
+        const double cv[3] = {carry[3 * line + 0], carry[3 * line + 1], carry[3 * line + 2]};
+        int j, k;
+        ll_line(g, line, j, k);
+        for (int i0 = 1; i0 <= g.nx - 2; i0 += 64) {
+            const int i = i0 + lane;
+            LlCell c{};
+            if (i <= g.nx - 2) c = ll_cell(psi, g.nx, g.ny, i, j, k);
+            const unsigned long long w = __ballot(c.nhits >= 2);
+            if (c.ended == 0 && c.nhits == 1 && !(w & ((1ull << lane) - 1ull))) acc += ll_pair(c.h1, cv);
+            if (w) break;  // uniform per wavefront
+        }
     }
     const double s = block_reduce<kLlThreads, RED_SUM>(acc, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -441,10 +455,11 @@ hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double*
                              double* out, hipStream_t s) {
     const LlGeom g{nx, ny, nz, linelength_lines(ny, nz)};
     const long long nb = linelength_blocks(ny, nz);
-    hipLaunchKernelGGL(k_ll_last, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, last);
+    hipLaunchKernelGGL(k_ll_sum, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, last, partial);
     hipLaunchKernelGGL(k_ll_carry, dim3(1), dim3(kLlScan), 0, s, last, g.nlines, carry);
-    hipLaunchKernelGGL(k_ll_sum, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, carry, partial);
-    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(kLlThreads), 0, s, partial, nb, out);
+    hipLaunchKernelGGL(k_ll_lead, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, (const double*)last,
+                       (const double*)carry, partial + nb);
+    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(kLlThreads), 0, s, partial, 2 * nb, out);
     return hipGetLastError();
 }
 
