@@ -1420,7 +1420,8 @@ int gpe_linelength(gpe_ctx* c, double* length) {
     if (!length) return fail(c, GPE_EINVAL, "gpe_linelength: output is NULL");
     const int nx = c->p.nx, ny = c->p.ny, nz = c->p.nz;
     const long long nl = linelength_lines(ny, nz), nb = linelength_blocks(ny, nz);
-    int rc = dalloc(c, (void**)&c->llbuf, (size_t)(8 * nl + 2 * nb + 1) * sizeof(double), "line length");
+    const long long nr = linelength_runs(ny, nz);
+    int rc = dalloc(c, (void**)&c->llbuf, (size_t)(8 * nl + 7 * nr + 2 * nb + 1) * sizeof(double), "line length");
     if (rc) return rc;
     const double2* src = (const double2*)sel_field(c);
     if (c->p.storage == GPE_C64) {
@@ -1430,8 +1431,9 @@ int gpe_linelength(gpe_ctx* c, double* length) {
     }
     double* last = c->llbuf;
     double* carry = last + 5 * nl;
-    double* partial = carry + 3 * nl;
-    HIPCHK(c, launch_linelength(src, nx, ny, nz, last, carry, partial, c->dres, c->stream));
+    double* runs = carry + 3 * nl;
+    double* partial = runs + 7 * nr;
+    HIPCHK(c, launch_linelength(src, nx, ny, nz, last, carry, runs, partial, c->dres, c->stream));
     double s;
     if ((rc = read_result(c, &s))) return rc;
     double d[3];

@@ -114,12 +114,14 @@ hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partia
                       hipStream_t s);
 
 // Line length (gpe_linelength.hip): last holds 5, carry 3 doubles per x-line
-// ((ny-2)(nz-2) lines), partial 2 linelength_blocks doubles; out[0] = the sum
-// of the cell contributions (times dx gives the reference's value).
+// ((ny-2)(nz-2) lines), runs 7 doubles per linelength_runs, partial 2
+// linelength_blocks doubles; out[0] = the sum of the cell contributions
+// (times dx gives the reference's value).
 long long linelength_lines(int ny, int nz);
 long long linelength_blocks(int ny, int nz);
-hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double* last, double* carry, double* partial,
-                             double* out, hipStream_t s);
+long long linelength_runs(int ny, int nz);
+hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double* last, double* carry, double* runs,
+                             double* partial, double* out, hipStream_t s);
 
 // Spectral diagnostics (gpe_spectral.hip); out holds 2 kSpectrumShells + nhalf
 // values: shell sums, shell counts, F bins.

@@ -16,7 +16,8 @@
 //      line's last two-hit cell so far; the single-hit cells before the
 //      line's first two-hit cell ("leading" cells) need the carry and are
 //      only counted.  It also records the line's last two-hit cell;
-//   2. one block scans the lines in order: the carry into each line;
+//   2. a scan over the lines in runs (run ends, one block over the runs, run
+//      fills): the carry into each line;
 //   3. one wavefront per line that has leading cells walks its start again,
 //      chunk by chunk up to the chunk of its first two-hit cell, and adds
 //      them against the carry.
@@ -233,64 +234,107 @@ constexpr int kLlWaveBits = 28;  // pooled cell: i | wavefront << 28 (i < nx < 2
 // last[] entries per line: has, h2 (3), leading single-hit cells
 constexpr int kLlLast = 5;
 
-// Pass 2 (one block of 1024 threads): carry[line] = the second hit of the
-// last two-hit cell of the lines before it, (0, 0, 0) before any.
+// Pass 2: carry[line] = the second hit of the last two-hit cell of the
+// lines before it, (0, 0, 0) before any.  An exclusive "last entry with
+// has != 0" scan over the lines, in runs of kLlRun lines: (a) each thread
+// finds its run's last such line, (b) one block scans the runs, (c) each
+// thread fills its run's lines.
+constexpr int kLlRun = 16;
 constexpr int kLlScan = 1024;
-__global__ __launch_bounds__(kLlScan) void k_ll_carry(const double* __restrict__ last, long long nlines,
-                                                      double* __restrict__ carry) {
-    __shared__ double chunk_last[kLlScan][4];
-    __shared__ double chunk_in[kLlScan][3];
-    const long long per = (nlines + kLlScan - 1) / kLlScan;
-    const long long l0 = (long long)threadIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
-    double has = 0.0, x = 0.0, y = 0.0, z = 0.0;
-    // the chunk's last line with a two-hit cell: groups from the end, their
-    // flags loaded together
+constexpr int kLlRunThreads = 256;
+
+// the last entry in [l0, l1) of e (entries of STRIDE doubles: has, x, y, z,
+// ...) with has != 0: groups of 8 from the end, their flags loaded together
+template <int STRIDE>
+__device__ __forceinline__ void ll_scan_last(const double* __restrict__ e, long long l0, long long l1, double& has,
+                                             double& x, double& y, double& z) {
+    has = 0.0, x = 0.0, y = 0.0, z = 0.0;
     for (long long g1 = l1; g1 > l0 && has == 0.0; g1 -= 8) {
         const long long g0 = g1 - 8 > l0 ? g1 - 8 : l0;
         double f[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) f[q] = g0 + q < g1 ? last[kLlLast * (g0 + q)] : 0.0;
+        for (int q = 0; q < 8; ++q) f[q] = g0 + q < g1 ? e[STRIDE * (g0 + q)] : 0.0;
 #pragma unroll
         for (int q = 7; q >= 0; --q)
             if (has == 0.0 && f[q] != 0.0) {
                 const long long lq = g0 + q;
-                has = 1.0, x = last[kLlLast * lq + 1], y = last[kLlLast * lq + 2], z = last[kLlLast * lq + 3];
+                has = 1.0, x = e[STRIDE * lq + 1], y = e[STRIDE * lq + 2], z = e[STRIDE * lq + 3];
             }
     }
-    chunk_last[threadIdx.x][0] = has;
-    chunk_last[threadIdx.x][1] = x;
-    chunk_last[threadIdx.x][2] = y;
-    chunk_last[threadIdx.x][3] = z;
+}
+
+// out[3 l .. 3 l + 2] for l in [l0, l1): the value carried into entry l,
+// starting from (x, y, z); groups of 8 entries loaded before the serial
+// update, so their loads are in flight together
+template <int STRIDE>
+__device__ __forceinline__ void ll_scan_fill(const double* __restrict__ e, long long l0, long long l1, double x,
+                                             double y, double z, double* __restrict__ out) {
+    constexpr int kG = 8;
+    long long l = l0;
+    for (; l + kG <= l1; l += kG) {
+        double v[kG][4];
+#pragma unroll
+        for (int q = 0; q < kG; ++q)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[q][c] = e[STRIDE * (l + q) + c];
+#pragma unroll
+        for (int q = 0; q < kG; ++q) {
+            out[3 * (l + q) + 0] = x, out[3 * (l + q) + 1] = y, out[3 * (l + q) + 2] = z;
+            if (v[q][0] != 0.0) x = v[q][1], y = v[q][2], z = v[q][3];
+        }
+    }
+    for (; l < l1; ++l) {
+        out[3 * l + 0] = x, out[3 * l + 1] = y, out[3 * l + 2] = z;
+        if (e[STRIDE * l] != 0.0) x = e[STRIDE * l + 1], y = e[STRIDE * l + 2], z = e[STRIDE * l + 3];
+    }
+}
+
+// (a) rl[run] = (has, h2) of the run's last line with a two-hit cell
+__global__ __launch_bounds__(kLlRunThreads) void k_ll_run_last(const double* __restrict__ last, long long nlines,
+                                                               double* __restrict__ rl) {
+    const long long run = (long long)blockIdx.x * kLlRunThreads + threadIdx.x;
+    const long long l0 = run * kLlRun;
+    if (l0 >= nlines) return;
+    const long long l1 = l0 + kLlRun < nlines ? l0 + kLlRun : nlines;
+    double has, x, y, z;
+    ll_scan_last<kLlLast>(last, l0, l1, has, x, y, z);
+    rl[4 * run + 0] = has, rl[4 * run + 1] = x, rl[4 * run + 2] = y, rl[4 * run + 3] = z;
+}
+
+// (b) one block: rin[run] = the value carried into each run
+__global__ __launch_bounds__(kLlScan) void k_ll_run_scan(const double* __restrict__ rl, long long nruns,
+                                                         double* __restrict__ rin) {
+    __shared__ double part_last[kLlScan][4];
+    __shared__ double part_in[kLlScan][3];
+    const long long per = (nruns + kLlScan - 1) / kLlScan;
+    const long long r0 = (long long)threadIdx.x * per, r1 = r0 + per < nruns ? r0 + per : nruns;
+    double has, x, y, z;
+    ll_scan_last<4>(rl, r0, r1 > r0 ? r1 : r0, has, x, y, z);
+    part_last[threadIdx.x][0] = has;
+    part_last[threadIdx.x][1] = x;
+    part_last[threadIdx.x][2] = y;
+    part_last[threadIdx.x][3] = z;
     __syncthreads();
     if (threadIdx.x == 0) {
         double cx = 0.0, cy = 0.0, cz = 0.0;
         for (int t = 0; t < kLlScan; ++t) {
-            chunk_in[t][0] = cx, chunk_in[t][1] = cy, chunk_in[t][2] = cz;
-            if (chunk_last[t][0] != 0.0) cx = chunk_last[t][1], cy = chunk_last[t][2], cz = chunk_last[t][3];
+            part_in[t][0] = cx, part_in[t][1] = cy, part_in[t][2] = cz;
+            if (part_last[t][0] != 0.0) cx = part_last[t][1], cy = part_last[t][2], cz = part_last[t][3];
         }
     }
     __syncthreads();
-    x = chunk_in[threadIdx.x][0], y = chunk_in[threadIdx.x][1], z = chunk_in[threadIdx.x][2];
-    // groups of kLlGroup lines: their entries are loaded before the serial
-    // update, so the loads of a group are in flight together
-    constexpr int kLlGroup = 8;
-    long long l = l0;
-    for (; l + kLlGroup <= l1; l += kLlGroup) {
-        double e[kLlGroup][4];
-#pragma unroll
-        for (int q = 0; q < kLlGroup; ++q)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) e[q][c] = last[kLlLast * (l + q) + c];
-#pragma unroll
-        for (int q = 0; q < kLlGroup; ++q) {
-            carry[3 * (l + q) + 0] = x, carry[3 * (l + q) + 1] = y, carry[3 * (l + q) + 2] = z;
-            if (e[q][0] != 0.0) x = e[q][1], y = e[q][2], z = e[q][3];
-        }
-    }
-    for (; l < l1; ++l) {
-        carry[3 * l + 0] = x, carry[3 * l + 1] = y, carry[3 * l + 2] = z;
-        if (last[kLlLast * l] != 0.0) x = last[kLlLast * l + 1], y = last[kLlLast * l + 2], z = last[kLlLast * l + 3];
-    }
+    if (r1 > r0) ll_scan_fill<4>(rl, r0, r1, part_in[threadIdx.x][0], part_in[threadIdx.x][1], part_in[threadIdx.x][2], rin);
+}
+
+// (c) carry of every line of the run
+__global__ __launch_bounds__(kLlRunThreads) void k_ll_run_fill(const double* __restrict__ last, long long nlines,
+                                                               const double* __restrict__ rin,
+                                                               double* __restrict__ carry) {
+    const long long run = (long long)blockIdx.x * kLlRunThreads + threadIdx.x;
+    const long long l0 = run * kLlRun;
+    if (l0 >= nlines) return;
+    const long long l1 = l0 + kLlRun < nlines ? l0 + kLlRun : nlines;
+    ll_scan_fill<kLlLast>(last, l0, l1, rin[3 * run + 0], rin[3 * run + 1], rin[3 * run + 2], carry);
 }
 
 // Segments of kLlSeg chunks: per segment each wavefront ranks the
@@ -437,12 +481,12 @@ __global__ __launch_bounds__(kLlThreads) void k_ll_lead(const double2* __restric
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ __launch_bounds__(kLlThreads) void k_ll_final(const double* __restrict__ partial, long long nb,
-                                                         double* __restrict__ out) {
-    __shared__ double red[kLlThreads / 64];
+__global__ __launch_bounds__(kLlScan) void k_ll_final(const double* __restrict__ partial, long long nb,
+                                                      double* __restrict__ out) {
+    __shared__ double red[kLlScan / 64];
     double s = 0.0;
-    for (long long b = threadIdx.x; b < nb; b += kLlThreads) s += partial[b];
-    s = block_reduce<kLlThreads, RED_SUM>(s, red);
+    for (long long b = threadIdx.x; b < nb; b += kLlScan) s += partial[b];
+    s = block_reduce<kLlScan, RED_SUM>(s, red);
     if (threadIdx.x == 0) out[0] = s;
 }
 
@@ -451,15 +495,22 @@ long long linelength_blocks(int ny, int nz) {
     return (linelength_lines(ny, nz) + kLlThreads / 64 - 1) / (kLlThreads / 64);
 }
 
-hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double* last, double* carry, double* partial,
-                             double* out, hipStream_t s) {
+long long linelength_runs(int ny, int nz) { return (linelength_lines(ny, nz) + kLlRun - 1) / kLlRun; }
+
+hipError_t launch_linelength(const double2* psi, int nx, int ny, int nz, double* last, double* carry, double* runs,
+                             double* partial, double* out, hipStream_t s) {
     const LlGeom g{nx, ny, nz, linelength_lines(ny, nz)};
     const long long nb = linelength_blocks(ny, nz);
     hipLaunchKernelGGL(k_ll_sum, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, last, partial);
-    hipLaunchKernelGGL(k_ll_carry, dim3(1), dim3(kLlScan), 0, s, last, g.nlines, carry);
+    const long long nruns = linelength_runs(ny, nz);
+    const unsigned rb = (unsigned)((nruns + kLlRunThreads - 1) / kLlRunThreads);
+    hipLaunchKernelGGL(k_ll_run_last, dim3(rb), dim3(kLlRunThreads), 0, s, (const double*)last, g.nlines, runs);
+    hipLaunchKernelGGL(k_ll_run_scan, dim3(1), dim3(kLlScan), 0, s, (const double*)runs, nruns, runs + 4 * nruns);
+    hipLaunchKernelGGL(k_ll_run_fill, dim3(rb), dim3(kLlRunThreads), 0, s, (const double*)last, g.nlines,
+                       (const double*)(runs + 4 * nruns), carry);
     hipLaunchKernelGGL(k_ll_lead, dim3((unsigned)nb), dim3(kLlThreads), 0, s, psi, g, (const double*)last,
                        (const double*)carry, partial + nb);
-    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(kLlThreads), 0, s, partial, 2 * nb, out);
+    hipLaunchKernelGGL(k_ll_final, dim3(1), dim3(kLlScan), 0, s, partial, 2 * nb, out);
     return hipGetLastError();
 }
 
