@@ -347,7 +347,7 @@ __global__ __launch_bounds__(kLlRunThreads) void k_ll_run_fill(const double* __r
 // pass needs: whether a cell has two hits (its h2 feeds later single-hit
 // cells) or one (its h1).  Each wavefront then resolves its line's
 // single-hit cells chunk by chunk in cell order.
-constexpr int kLlSeg = 4;
+constexpr int kLlSeg = 8;
 
 // Pass 1: partial[block] = the block's sum of the contributions of all but
 // the leading single-hit cells; last[line] = (1, h2) of the line's last
