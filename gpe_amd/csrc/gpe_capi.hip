@@ -26,6 +26,9 @@ thread_local std::string g_thread_error;
 
 }  // namespace
 
+// device and pinned host result slots (reductions' final values)
+constexpr int kNResults = 16;
+
 struct gpe_ctx {
     gpe_params p;
     int device = 0;
@@ -78,7 +81,7 @@ struct gpe_ctx {
     double* partial = nullptr;
     size_t partial_cap = 0;  // doubles
     double* dres = nullptr;
-    double* hres = nullptr;  // pinned, 4 doubles
+    double* hres = nullptr;  // pinned, kNResults doubles
     void* dtab = nullptr;    // dissipation tables (storage precision)
     Tiles tiles{64, 8, 0};
     std::string err;
@@ -249,7 +252,11 @@ template <class Op, class Launch> static int run_launch(gpe_ctx* c, Geom g, Laun
         HIPCHK(c, hipEventRecord(rec.b, c->stream));
         c->recs.push_back(rec);
     }
-    if (Op::kNRed > 0) {
+    if constexpr (Op::kMode == RED_MIXED) {
+        HIPCHK(c, launch_reduce_final(RED_SUM, c->partial, nb, Op::kNSum, c->dres, c->stream));
+        HIPCHK(c, launch_reduce_final(RED_MAX, c->partial + (size_t)Op::kNSum * nb, nb, Op::kNRed - Op::kNSum,
+                                      c->dres + Op::kNSum, c->stream));
+    } else if (Op::kNRed > 0) {
         HIPCHK(c, launch_reduce_final(Op::kMode, c->partial, nb, Op::kNRed, c->dres, c->stream));
     }
     return 0;
@@ -1002,9 +1009,9 @@ int gpe_ctx_create(const gpe_params* p, gpe_ctx** out) {
         return rc;
     }
     if (!rc) rc = dalloc(c, &c->psi, field_bytes(c), "psi");
-    if (!rc) rc = dalloc(c, (void**)&c->dres, 8 * sizeof(double), "results");
+    if (!rc) rc = dalloc(c, (void**)&c->dres, kNResults * sizeof(double), "results");
     if (!rc) {
-        e = hipHostMalloc((void**)&c->hres, 8 * sizeof(double), hipHostMallocDefault);
+        e = hipHostMalloc((void**)&c->hres, kNResults * sizeof(double), hipHostMallocDefault);
         if (e != hipSuccess) rc = fail(c, GPE_ENOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     }
     if (!rc && p->diss_amp != 0.0 && (p->eqn_to_solve == 1 || p->eqn_to_solve == 3))
@@ -1238,18 +1245,25 @@ int gpe_momentum(gpe_ctx* c, double mom[3]) {
 }
 
 // Velocity field into c->vel (OpVelocity), masked as get_pdf_velocity or not.
-static int velocity_field(gpe_ctx* c, int masked, int ncomp) {
+// stats: the same pass also reduces the statistics of get_pdf_velocity
+// into c->dres (OpVelocity<R, true>)
+static int velocity_field(gpe_ctx* c, int masked, int ncomp, bool stats = false) {
     int rc;
     if ((rc = dalloc(c, &c->vel, (size_t)c->n * 3 * sizeof(double), "velocity"))) return rc;
     double d[3];
     spacing(c->p, d);
     const double f = c->p.order == 2 ? 2.0 : 12.0;  // (2.0_pr*dx) or (12.0_pr*dx), src/derivs.f90:41-56
+    const double den[3] = {f * d[0], f * d[1], f * d[2]};
+    auto go = [&](auto o) {
+        using R = typename decltype(o)::Real;
+        return c->p.order == 2 ? run_op<R, 2>(c, sel_field(c), o) : run_op<R, 4>(c, sel_field(c), o);
+    };
     if (c->p.storage != GPE_C64) {
-        OpVelocity<double> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
-        return c->p.order == 2 ? run_op<double, 2>(c, sel_field(c), o) : run_op<double, 4>(c, sel_field(c), o);
+        if (stats) return go(OpVelocity<double, true>{(double*)c->vel, c->n, masked, ncomp, {den[0], den[1], den[2]}});
+        return go(OpVelocity<double, false>{(double*)c->vel, c->n, masked, ncomp, {den[0], den[1], den[2]}});
     }
-    OpVelocity<float> o{(double*)c->vel, c->n, masked, ncomp, {f * d[0], f * d[1], f * d[2]}};
-    return c->p.order == 2 ? run_op<float, 2>(c, sel_field(c), o) : run_op<float, 4>(c, sel_field(c), o);
+    if (stats) return go(OpVelocity<float, true>{(double*)c->vel, c->n, masked, ncomp, {den[0], den[1], den[2]}});
+    return go(OpVelocity<float, false>{(double*)c->vel, c->n, masked, ncomp, {den[0], den[1], den[2]}});
 }
 
 int gpe_velocity_pdf(gpe_ctx* c, int nbins, int64_t count[3], double mean[3], double stdev[3], double vmax[3],
@@ -1258,28 +1272,18 @@ int gpe_velocity_pdf(gpe_ctx* c, int nbins, int64_t count[3], double mean[3], do
     if (!count || !mean || !stdev || !vmax || !pdf) return fail(c, GPE_EINVAL, "gpe_velocity_pdf: output is NULL");
     if (nbins < 2 || nbins > 4096 || (nbins & 1)) return fail(c, GPE_EINVAL, "gpe_velocity_pdf: nbins must be even, 2..4096");
     int rc;
-    if ((rc = velocity_field(c, 1, 3))) return rc;
-    const int nb = kNormBlocks;
-    if ((rc = ensure_partial(c, (size_t)15 * nb))) return rc;
-    HIPCHK(c, launch_vel_stats((const double*)c->vel, c->n, c->partial, nb, c->stream));
-    // c->dres holds 8 values: the first 8 sums now, the ninth with the maxima below
-    HIPCHK(c, launch_reduce_final(RED_SUM, c->partial, nb, 8, c->dres, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // the velocity field and, in the same pass, its statistics: dres rows
+    // 0..8 count, sum, sum of squares per component, 9..14 max and -min
+    if ((rc = velocity_field(c, 1, 3, true))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 15 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    double sums[9];
-    for (int j = 0; j < 8; ++j) sums[j] = c->hres[j];
-    HIPCHK(c, launch_reduce_final(RED_SUM, c->partial + (size_t)8 * nb, nb, 1, c->dres, c->stream));
-    HIPCHK(c, launch_reduce_final(RED_MAX, c->partial + (size_t)9 * nb, nb, 6, c->dres + 1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->hres, c->dres, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    sums[8] = c->hres[0];
     double vm[3];
     for (int a = 0; a < 3; ++a) {
-        const double n = sums[3 * a], s = sums[3 * a + 1], s2 = sums[3 * a + 2];
+        const double n = c->hres[3 * a], s = c->hres[3 * a + 1], s2 = c->hres[3 * a + 2];
         count[a] = (int64_t)n;
         mean[a] = s / n;                                  // mean (src/variables.f90:827-848)
         stdev[a] = sqrt(s2 / n - mean[a] * mean[a]);      // stdev (src/variables.f90:852-875)
-        const double mx = c->hres[1 + 2 * a], mn = -c->hres[2 + 2 * a];
+        const double mx = c->hres[9 + 2 * a], mn = -c->hres[10 + 2 * a];
         vm[a] = fabs(mx) > fabs(mn) ? fabs(mx) : fabs(mn);  // max_vel (src/variables.f90:441-442)
         vmax[a] = vm[a];
     }
@@ -1294,7 +1298,8 @@ int gpe_velocity_pdf(gpe_ctx* c, int nbins, int64_t count[3], double mean[3], do
     for (int a = 0; a < 3; ++a) c->hres[a] = vm[a];  // pinned staging
     HIPCHK(c, hipMemcpyAsync(c->dres, c->hres, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemsetAsync(c->hist, 0, hb, c->stream));
-    HIPCHK(c, launch_vel_hist((const double*)c->vel, c->n, nbins, c->dres, (unsigned long long*)c->hist, nb, c->stream));
+    HIPCHK(c, launch_vel_hist((const double*)c->vel, c->n, nbins, c->dres, (unsigned long long*)c->hist, kNormBlocks,
+                              c->stream));
     std::vector<unsigned long long> h((size_t)3 * nbins);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->hist, hb, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

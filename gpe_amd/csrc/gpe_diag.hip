@@ -1,6 +1,7 @@
-// Diagnostics of the state: velocity statistics and PDF (get_pdf_velocity +
-// get_pdf, src/variables.f90:377-463) and the velocity correlation function
-// along x (get_vcf, src/variables.f90:467-507), from a velocity field that
+// Diagnostics of the state: the velocity PDF (get_pdf_velocity + get_pdf,
+// src/variables.f90:377-463; OpVelocity<R, true> forms the velocity field and
+// its statistics in one pass) and the velocity correlation function along x
+// (get_vcf, src/variables.f90:467-507), from a velocity field that
 // OpVelocity wrote.  All sums are FP64 in a fixed order (per-block partials,
 // then one block); histogram counts are integers.
 #include <hip/hip_runtime.h>
@@ -11,40 +12,6 @@ namespace gpe {
 
 constexpr int kDiagThreads = 256;
 constexpr int kVelValid = 500;  // the reference keeps values < 0.5e3 (marker 1e3)
-
-// Per block: for each component a, count/sum/sum of squares of the valid
-// values (partial rows 3a+0..2) and max / -min (rows 9+2a, 10+2a).
-__global__ __launch_bounds__(kDiagThreads) void k_vel_stats(const double* __restrict__ vel, long long n,
-                                                           double* __restrict__ partial) {
-    __shared__ double sh[kDiagThreads / 64];
-    const int nb = gridDim.x;
-    for (int a = 0; a < 3; ++a) {
-        const double* v = vel + a * n;
-        double cnt = 0.0, s = 0.0, s2 = 0.0, mx = -1e300, mn = -1e300;
-        for (long long i = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i < n; i += (long long)nb * kDiagThreads) {
-            const double x = v[i];
-            if (x < kVelValid) {
-                cnt += 1.0;
-                s += x;
-                s2 += x * x;
-                mx = x > mx ? x : mx;
-                mn = -x > mn ? -x : mn;
-            }
-        }
-        const double r0 = block_reduce<kDiagThreads, RED_SUM>(cnt, sh);
-        const double r1 = block_reduce<kDiagThreads, RED_SUM>(s, sh);
-        const double r2 = block_reduce<kDiagThreads, RED_SUM>(s2, sh);
-        const double r3 = block_reduce<kDiagThreads, RED_MAX>(mx, sh);
-        const double r4 = block_reduce<kDiagThreads, RED_MAX>(mn, sh);
-        if (threadIdx.x == 0) {
-            partial[(long long)(3 * a + 0) * nb + blockIdx.x] = r0;
-            partial[(long long)(3 * a + 1) * nb + blockIdx.x] = r1;
-            partial[(long long)(3 * a + 2) * nb + blockIdx.x] = r2;
-            partial[(long long)(9 + 2 * a) * nb + blockIdx.x] = r3;
-            partial[(long long)(10 + 2 * a) * nb + blockIdx.x] = r4;
-        }
-    }
-}
 
 // Bin edges of get_pdf (src/variables.f90:448-451): bin i in
 // (-nbins/2, nbins/2] holds (thr(i-1), thr(i)], thr(i) = ((2 i) vmax) / nbins,
@@ -220,11 +187,6 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_final(const double* __rest
     double s = 0.0;
     for (int b = 0; b < nb; ++b) s += partial[(long long)b * nx + r];
     out[r] = s;
-}
-
-hipError_t launch_vel_stats(const double* vel, long long n, double* partial, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_vel_stats, dim3(nblocks), dim3(kDiagThreads), 0, s, vel, n, partial);
-    return hipGetLastError();
 }
 
 hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,

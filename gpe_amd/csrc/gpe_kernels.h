@@ -10,7 +10,8 @@ namespace gpe {
 // ---------------------------------------------------------------------------
 // Block reductions (deterministic: fixed butterfly within a wavefront, fixed
 // wave order across the block).
-enum { RED_NONE = 0, RED_SUM = 1, RED_MAX = 2 };
+// RED_MIXED: rows [0, Op::kNSum) are sums, the rest maxima (starting at -1e300)
+enum { RED_NONE = 0, RED_SUM = 1, RED_MAX = 2, RED_MIXED = 3 };
 
 template <int MODE> __device__ __forceinline__ double red_op(double a, double b) {
     if constexpr (MODE == RED_SUM) return a + b;
@@ -237,6 +238,10 @@ __device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w,
 #pragma unroll
     for (int d = 0; d < 2 * H; ++d) q[d] = src(Cell{g.plane * bcmap(z0 - H + d, g.nz, g.bcs), col}, qa[d]);
     double red[Op::kNRed > 2 ? Op::kNRed : 2] = {};
+    if constexpr (Op::kMode == RED_MIXED) {
+#pragma unroll
+        for (int r = Op::kNSum; r < Op::kNRed; ++r) red[r] = -1e300;
+    }
 
     // The loads for plane z: centre of plane z+H (the window's new entry),
     // the halo of plane z and the operator's pointwise inputs.
@@ -302,7 +307,11 @@ __device__ __forceinline__ void stencil_body_src(const Geom& g, const StW<R>& w,
     if constexpr (Op::kNRed > 0) {
 #pragma unroll
         for (int r = 0; r < Op::kNRed; ++r) {
-            const double v = block_reduce<NT, Op::kMode>(red[r], red_sh);
+            double v;
+            if constexpr (Op::kMode == RED_MIXED)
+                v = r < Op::kNSum ? block_reduce<NT, RED_SUM>(red[r], red_sh) : block_reduce<NT, RED_MAX>(red[r], red_sh);
+            else
+                v = block_reduce<NT, Op::kMode>(red[r], red_sh);
             if (tid == 0) partial[(long long)r * g.nblocks + pidx] = v;
         }
     }
@@ -721,16 +730,21 @@ __device__ __forceinline__ double2 d1_ref(const S& st, int axis, double den) {
 // masked: cells with |psi|^2 <= 0.01 get the reference's marker 1e3 (as
 // get_pdf_velocity); otherwise every cell is divided (get_vcf).  ncomp = 3
 // writes v_x, v_y, v_z, 1 only v_x.
-template <typename R> struct OpVelocity {
+// STATS: also the statistics of get_pdf_velocity (src/variables.f90:377-463)
+// per component a of the values below the validity marker: rows 3a + 0..2
+// count, sum, sum of squares; rows 9 + 2a, 10 + 2a max and -min.
+template <typename R, bool STATS = false> struct OpVelocity {
     static constexpr int kKind = 28, kBytesPerCell = (int)sizeof(C<R>) + 24;
-    static constexpr int kNRed = 0, kMode = RED_NONE;
+    static constexpr int kNRed = STATS ? 15 : 0, kMode = STATS ? RED_MIXED : RED_NONE, kNSum = 9;
+    static constexpr double kValid = 500.0;  // the reference keeps values < 0.5e3 (marker 1e3)
+    using Real = R;
     using Pre = NoPre;
     double* __restrict__ vel;  // component a at vel + a * n
     long long n;
     int masked, ncomp;
     double den[3];             // 2 d or 12 d per axis
     __device__ Pre load(Cell) const { return {}; }
-    template <class S> __device__ void apply(Cell i, int, int, int, const S& st, const Pre&, double*) const {
+    template <class S> __device__ void apply(Cell i, int, int, int, const S& st, const Pre&, double* red) const {
 #pragma clang fp contract(off)
         const C<R> p = st.c();
         const double pr = p.x, pi = p.y;
@@ -744,9 +758,20 @@ template <typename R> struct OpVelocity {
                 const double v = ok ? -(pi * g) / a2 : 1e3;
                 double* dst = vel + a * n;
                 dst[i.zb + i.own] = v;
+                if constexpr (STATS) {
+                    if (v < kValid) {
+                        red[3 * a + 0] += 1.0;
+                        red[3 * a + 1] += v;
+                        red[3 * a + 2] += v * v;
+                        red[9 + 2 * a] = v > red[9 + 2 * a] ? v : red[9 + 2 * a];
+                        red[10 + 2 * a] = -v > red[10 + 2 * a] ? -v : red[10 + 2 * a];
+                    }
+                }
             }
         }
     }
 };
+
+template <typename R> using OpVelocityStats = OpVelocity<R, true>;
 
 }  // namespace gpe

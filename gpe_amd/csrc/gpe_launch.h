@@ -54,7 +54,8 @@ hipError_t launch_stencil(const Tiles& t, Geom g, const StW<R>& w, const C<R>* u
     X(R, OpEnergy<R>)                                                                                              \
     X(R, OpHamiltonian<R>)                                                                                         \
     X(R, OpMomentum<R>)                                                                                            \
-    X(R, OpVelocity<R>)
+    X(R, OpVelocity<R>)                                                                                            \
+    X(R, OpVelocityStats<R>)
 
 // Mixed-storage operators: those over a plain complex128 stencil use
 // launch_stencil<double>, the others launch_stencil_sum.
@@ -107,7 +108,6 @@ hipError_t launch_stream5(void* c, void* d, const void* a, const void* b, const 
 hipError_t launch_reduce_final(int mode, const double* partial, int n, int nval, double* out, hipStream_t s);
 
 // Diagnostics (gpe_diag.hip).
-hipError_t launch_vel_stats(const double* vel, long long n, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,
                            int nblocks, hipStream_t s);
 hipError_t launch_vcf(const double* vx, int nx, long long nlines, double* partial, int nblocks, double* out,

@@ -305,12 +305,13 @@ def attempts(n: int, count: int, storages, copy_mib: int = 0):
 
 # Algorithmic HBM bytes per cell of each diagnostic (complex128 state):
 #   momentum   one stencil pass over psi                                    16
-#   vel_pdf    velocity pass (psi 16 + v 24), statistics (24), histogram (24) 88
+#   vel_pdf    velocity pass with its statistics (psi 16 + v 24), histogram (24) 64
 #   vcf        velocity pass (16 + 24), one read of v_x (8)                  48
 #   condensed  copy for the FFT (16 + 16), spectrum read (16); the FFT's
 #              own traffic (library) is not counted                         48
-#   linelength two passes over psi (the corners of a cell are shared)       32
-DIAG_BYTES = dict(momentum=16, vel_pdf=88, vcf=48, condensed=48, linelength=32)
+#   linelength one pass over psi (the corners of a cell are shared; the
+#              leading-cell pass re-reads only the starts of some lines)    16
+DIAG_BYTES = dict(momentum=16, vel_pdf=64, vcf=48, condensed=48, linelength=16)
 
 
 def diag(n: int, reps: int):
