@@ -43,15 +43,18 @@ KINDS = [
 ]
 # algorithmic field bytes per cell: accesses x element size, or fixed bytes (mixed storage)
 ACCESS = {"rk4_stage1": 2, "rk4_stage2": 3, "rk4_stage3": 5, "rk4_stage4": 3,
+          "euler_or_rk2_stage1": 2, "rk2_stage2": 3,
           "rkck_stage1": 3, "rkck_stage2": 5, "rkck_stage3": 6, "rkck_stage4": 7, "rkck_stage5": 8,
           "rkck_stage6_error": 7}
 MIXED_BYTES = {"mx_rk4_stage1": 24, "mx_rk4_stage2_3": 32, "mx_rk4_stage4": 56,
                "mx_rkck_stage1": 24, "mx_rkck_stage2": 40, "mx_rkck_stage3": 56, "mx_rkck_stage4": 64,
                "mx_rkck_stage5": 72, "mx_rkck_stage6_error": 72}
-FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3,  # field accesses per cell
+FUSED_ACCESS = {"rk4_fused12": 2, "rk4_fused34": 3, "rk2_fused": 2, "euler2_fused": 2,  # field accesses per cell
                 "rkck_fused12": 4, "rkck_fused34": 7, "rkck_fused56_error": 7}
 CK_PAIR = {"1": "rkck_fused12", "2": "rkck_fused34", "3": "rkck_fused56_error"}
 MIXED_BYTES.update({"mx_rkck_fused12": 40, "mx_rkck_fused34": 64, "mx_rkck_fused56_error": 72})
+# kernel name forms the summaries cover
+KERNEL_FORMS = ("k_stencil", "k_rk4_f", "k_rk2_fused", "k_euler2_fused", "k_ck_pair", "k_ckmx_pair")
 # kernels whose calibrated traffic bench.py reads back (traffic_<kind>_<prec>_<n>.json)
 TRAFFIC_KINDS = {"rk4_fused12", "rk4_fused34", "rk4_stage3", "mx_rk4_stage4"}
 
@@ -96,6 +99,12 @@ def kernel_label(name: str) -> str:
         h = int(mf.group(3)) // 2
         trap = " with-trap" if mf.group(4) != "0" else ""
         return f"rk4_fused{mf.group(1)} {prec} order{mf.group(3)} tile{64 + 2 * h}x15{trap}{_gm(mf.group(5))}"
+    mp = re.search(r"k_(rk2_fused|euler2_fused)<(double|float), (\d), (\d+)(?:, (\d))?>", name)
+    if mp:
+        prec = "c128" if mp.group(2) == "double" else "c64"
+        h = int(mp.group(3)) // 2
+        trap = " with-trap" if mp.group(4) != "0" else ""
+        return f"{mp.group(1)} {prec} order{mp.group(3)} tile{64 + 2 * h}x15{trap}{_gm(mp.group(5))}"
     mc = re.search(r"k_ck_pair<(double|float), (\d), (\d+), (\d)(?:, (\d))?>", name)
     if mc:
         prec = "c128" if mc.group(1) == "double" else "c64"
@@ -152,7 +161,7 @@ def sq_lines(a_dir, b_dir, run_cmd):
              "waves waiting for LDS issue | VALU / LDS / SALU / VMEM-read instructions per wave |",
              "|---|---|---|---|---|---|---|---|"]
     for name, a in A.items():
-        if not any(f in name for f in ("k_stencil", "k_rk4_f", "k_ck_pair", "k_ckmx_pair")) or name not in B:
+        if not any(f in name for f in KERNEL_FORMS) or name not in B:
             continue
         label = kernel_label(name)
         if len(label.split()) < 2:
@@ -227,7 +236,7 @@ def main(argv=None):
     traffic = {}
     for r in stats:
         name = r["Name"]
-        if not any(f in name for f in ("k_stencil", "k_rk4_f", "k_ck_pair", "k_ckmx_pair")):
+        if not any(f in name for f in KERNEL_FORMS):
             continue
         label = kernel_label(name)
         if len(label.split()) < 2:  # not a kernel of a known form (e.g. the batched ensemble's)
