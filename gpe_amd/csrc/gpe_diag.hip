@@ -39,6 +39,7 @@ __device__ __forceinline__ bool above_edge(double x, int i, double vmax, int nbi
 // values pile up in a few central bins, and atomics of one wavefront to one
 // LDS word are serialised.
 constexpr int kHistMaxCopies = 8;
+constexpr int kHistUnroll = 4;
 constexpr int kHistLdsBytes = 64 * 1024;
 inline int hist_copies(int nbins) {
     const int c = kHistLdsBytes / ((3 * nbins + 1) * (int)sizeof(unsigned int));
@@ -59,18 +60,29 @@ __global__ __launch_bounds__(kDiagThreads) void k_vel_hist(const double* __restr
         const double vmax = vmax3[a];
         if (!(vmax > 0.0)) continue;  // uniform
         const double scale = (double)nbins / (2.0 * vmax);
-        for (long long i = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i < n;
-             i += (long long)gridDim.x * kDiagThreads) {
-            const double x = v[i];
-            if (!(x < kVelValid)) continue;
-            // a first guess of the bin, then settle on the bin whose edges (as
-            // the reference forms them) hold x
-            double q = ceil(x * scale);
-            q = q < lo - 1 ? lo - 1 : (q > hi + 1 ? hi + 1 : q);
-            int b = (int)q;
-            while (b <= hi && above_edge(x, b, vmax, nbins, inv_nb)) ++b;
-            while (b >= lo && !above_edge(x, b - 1, vmax, nbins, inv_nb)) --b;
-            if (b >= lo && b <= hi) atomicAdd(&mine[a * nbins + (b - lo)], 1u);
+        // kHistUnroll values of the grid-stride sequence are loaded together,
+        // so their loads are in flight at once, then binned one by one
+        const long long stride = (long long)gridDim.x * kDiagThreads;
+        for (long long i0 = (long long)blockIdx.x * kDiagThreads + threadIdx.x; i0 < n; i0 += kHistUnroll * stride) {
+            double xs[kHistUnroll];
+#pragma unroll
+            for (int u = 0; u < kHistUnroll; ++u) {
+                const long long i = i0 + u * stride;
+                xs[u] = i < n ? v[i] : 1e3;  // 1e3: the invalid marker, not counted
+            }
+#pragma unroll
+            for (int u = 0; u < kHistUnroll; ++u) {
+                const double x = xs[u];
+                if (!(x < kVelValid)) continue;
+                // a first guess of the bin, then settle on the bin whose edges
+                // (as the reference forms them) hold x
+                double q = ceil(x * scale);
+                q = q < lo - 1 ? lo - 1 : (q > hi + 1 ? hi + 1 : q);
+                int b = (int)q;
+                while (b <= hi && above_edge(x, b, vmax, nbins, inv_nb)) ++b;
+                while (b >= lo && !above_edge(x, b - 1, vmax, nbins, inv_nb)) --b;
+                if (b >= lo && b <= hi) atomicAdd(&mine[a * nbins + (b - lo)], 1u);
+            }
         }
     }
     __syncthreads();
