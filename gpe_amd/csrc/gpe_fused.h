@@ -522,10 +522,11 @@ __device__ __forceinline__ void rk4_f34_body(const Geom& g, const StW<R>& w0, co
         __syncthreads();
         // the next plane's loads issued at a raised wave priority, so they
         // leave before the other waves' stencil arithmetic (k_rk4_f34 1.5-3%
-        // faster; the same in k_rk4_f12 made it slower)
-        __builtin_amdgcn_s_setprio(2);
+        // faster at 512^3; with the trap term, 2.5% slower on the 256^3
+        // trapped vortex, so not there; in k_rk4_f12 9% slower)
+        if constexpr (TERMS == 0) __builtin_amdgcn_s_setprio(2);
         fetch(F, z + DEPTH);
-        __builtin_amdgcn_s_setprio(0);
+        if constexpr (TERMS == 0) __builtin_amdgcn_s_setprio(0);
         if (c.active) {
             // third stage at plane p, whole block: u3 = psi0 + dt f(u2)
             FusedSten<R, ORD, S::LWA, TERMS> sa;
