@@ -562,8 +562,8 @@ static int ck_attempt_mixed_fused(gpe_ctx* c, double dre, double dim, double d0r
     a.k3 = (const CF*)c->mx[2];
     a.k4 = (const CF*)c->mx[3];
     a.dt = cdt<double>(dre, dim, 1.0);
-    a.t = make_tab<double>();
-    a.s21 = mkc<double>(a.t.b21 * a.dt.x, a.t.b21 * a.dt.y);
+    const CkTab<double> t = make_tab<double>();
+    a.s21 = mkc<double>(t.b21 * a.dt.x, t.b21 * a.dt.y);
     a.dt0_re = d0re;
     a.dt0_im = d0im;
     CF* DA = (CF*)c->mx[5];
@@ -620,7 +620,6 @@ static int ck_attempt_fused(gpe_ctx* c, double dre, double dim, double d0re, dou
     a.k3 = (const C<R>*)c->k3;
     a.k4 = (const C<R>*)c->k4;
     a.dt = cdt<R>(dre, dim, 1.0);
-    a.t = make_tab<R>();
     a.dt0_re = d0re;
     a.dt0_im = d0im;
     int rc;

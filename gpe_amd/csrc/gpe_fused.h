@@ -621,7 +621,6 @@ template <typename R> struct CkPairArgs {
     C<R>* unext;  // pair 1: u3, pair 2: u5
     C<R>* out;    // pair 3: psi_new
     C<R> dt;
-    CkTab<R> t;
     double dt0_re, dt0_im;  // the step's dt on entry to rkqs (psi_scal)
 };
 
@@ -713,7 +712,7 @@ __device__ __forceinline__ void ck_pair_body(const Geom& g, const StW<R>& w0, co
         };
         Pf F{};
         fetch(F, start);
-        const CkTab<R>& t = a.t;
+        constexpr CkTab<R> t = ck_tab<R>();  // constants, not kernel arguments held in registers
         auto step = [&](int z, auto phase) {
             constexpr int PH = decltype(phase)::value;
             const int p = z + H;

@@ -40,7 +40,6 @@ struct CkMxArgs {
     CD* out;    // pair 3: psi_new
     CD dt;
     CD s21;  // b21 dt
-    CkTab<double> t;
     double dt0_re, dt0_im;
 };
 
