@@ -126,67 +126,100 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_partial(const double* __re
     }
 }
 
-// The same sums, lag-blocked (nx <= 64 K): the block's four wavefronts each
-// take one line of a group of four; lane t owns the K lags r = tK .. tK+K-1
-// and runs i = 0, K, 2K, .. with the K x K products of a step unrolled, a
-// broadcast read of v(i + ii) and K + K reads of v(i + r) per step instead of
-// two LDS reads per product.  Lines are zero-padded in LDS to nx + 2K, so
-// the products past the line's end add exact zeros and need no masks.  The
-// four wavefronts' sums are added in a fixed order.  Element e of a line
-// sits at e + e / K (one unused slot after every K): the lanes' windows,
-// K elements apart, are then K + 1 doubles apart, an odd stride that spreads
-// a read of the 64 lanes over all LDS banks (K apart, they fell on a few).
+// The same sums, lag-blocked (nx <= 64 K).  The lags fall in 64 blocks of
+// K, r0 = tK .. tK+K-1 for t = 0 .. 63; a step of block t takes i = i0 ..
+// i0+K-1 (i0 a multiple of K) and does its K x K products unrolled, from K
+// reads of v(i) and 2K reads of v(i + r) instead of two LDS reads per
+// product.  Block t needs ceil((nx - tK) / K) steps, so blocks t and 63 - t
+// together need about the same number for every t (65 at nx = 64 K).  Each
+// wavefront takes two lines at a time, lanes 0-31 the first and 32-63 the
+// second, and lane u of a half runs block u and then block 63 - u in one
+// loop of uniform length: no lane idles while the short lags finish.  Lines
+// are zero-padded in LDS to nx + 2K, so the products past the line's end
+// add exact zeros and need no masks.  The eight (wavefront, half) sums are
+// added in a fixed order.  Element e of a line sits at e + e / K (one unused
+// slot after every K): windows K elements apart are then K + 1 doubles
+// apart, an odd stride that spreads a read over all LDS banks.
 template <int K> __device__ __forceinline__ int vcf_slot(int e) { return e + e / K; }
 template <int K> constexpr int vcf_line_len(int nx) { return (nx + 2 * K) + (nx + 2 * K) / K + 1; }
 
 template <int K>
 __global__ __launch_bounds__(kDiagThreads) void k_vcf_lags(const double* __restrict__ vx, int nx, long long nlines,
                                                           double* __restrict__ partial) {
-    extern __shared__ double lines[];  // 4 lines of nx + 2K, reused for the wave sums
-    constexpr int NW = kDiagThreads / 64;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int lw = nx + 2 * K;
-    double* L = lines + wv * vcf_line_len<K>(nx);
+    extern __shared__ double lines[];  // 2 lines per wavefront, then the lag sums
+    constexpr int NW = kDiagThreads / 64, NL = 2 * NW;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, half = lane >> 5, u = lane & 31;
+    const int lw = nx + 2 * K, ll = vcf_line_len<K>(nx);
+    const double* L = lines + (2 * wv + half) * ll;
     const long long per = (nlines + gridDim.x - 1) / gridDim.x;
     const long long l0 = (long long)blockIdx.x * per, l1 = l0 + per < nlines ? l0 + per : nlines;
-    const int r0 = lane * K;
-    double acc[K];
+    auto nsteps = [&](int t) { const int r = t * K; return r < nx ? (nx - r + K - 1) / K : 0; };
+    const int b1 = u, b2 = 63 - u, n1 = nsteps(b1), n2 = nsteps(b2), nt = n1 + n2;
+    int tmax = nt;  // the longest lane of the wavefront
 #pragma unroll
-    for (int q = 0; q < K; ++q) acc[q] = 0.0;
-    for (long long lg = l0; lg < l1; lg += NW) {
-        const long long l = lg + wv;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(tmax, o, 64);
+        tmax = v > tmax ? v : tmax;
+    }
+    double acc1[K], acc2[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) acc1[q] = acc2[q] = 0.0;
+    for (long long lg = l0; lg < l1; lg += NL) {
         __syncthreads();
-        for (int i = lane; i < lw; i += 64) L[vcf_slot<K>(i)] = (l < l1 && i < nx) ? vx[l * nx + i] : 0.0;
+        // the wavefront stages its two lines
+        for (int i = lane; i < 2 * lw; i += 64) {
+            const int h = i >= lw, e = i - h * lw;
+            const long long l = lg + 2 * wv + h;
+            lines[(2 * wv + h) * ll + vcf_slot<K>(e)] = (l < l1 && e < nx) ? vx[l * nx + e] : 0.0;
+        }
         __syncthreads();
-        if (l < l1 && r0 < nx) {
-            double b[2 * K];
+        if (lg + 2 * wv + half < l1) {
+            double cur[K];
 #pragma unroll
-            for (int j = 0; j < K; ++j) b[j] = L[vcf_slot<K>(r0) + j];
-            for (int i0 = 0; i0 < nx - r0; i0 += K) {
-                const int sb = vcf_slot<K>(i0 + r0 + K), sa = vcf_slot<K>(i0);  // i0, r0: multiples of K
+            for (int q = 0; q < K; ++q) cur[q] = 0.0;
+            for (int st = 0; st < tmax; ++st) {
+                if (st < nt) {
+                    const bool first = st < n1;
+                    const int r0 = (first ? b1 : b2) * K, i0 = (first ? st : st - n1) * K;
+                    const int sa = vcf_slot<K>(i0), sb = vcf_slot<K>(i0 + r0), sc = vcf_slot<K>(i0 + r0 + K);
+                    double b[2 * K];
 #pragma unroll
-                for (int j = 0; j < K; ++j) b[K + j] = L[sb + j];
+                    for (int j = 0; j < K; ++j) {
+                        b[j] = L[sb + j];
+                        b[K + j] = L[sc + j];
+                    }
 #pragma unroll
-                for (int ii = 0; ii < K; ++ii) {
-                    const double a = L[sa + ii];
+                    for (int ii = 0; ii < K; ++ii) {
+                        const double a = L[sa + ii];
 #pragma unroll
-                    for (int q = 0; q < K; ++q) acc[q] += a * b[ii + q];
+                        for (int q = 0; q < K; ++q) cur[q] += a * b[ii + q];
+                    }
+                    if (st == n1 - 1) {  // the first block is done
+#pragma unroll
+                        for (int q = 0; q < K; ++q) {
+                            acc1[q] += cur[q];
+                            cur[q] = 0.0;
+                        }
+                    }
                 }
-#pragma unroll
-                for (int j = 0; j < K; ++j) b[j] = b[K + j];
             }
+#pragma unroll
+            for (int q = 0; q < K; ++q) acc2[q] += cur[q];
         }
     }
     __syncthreads();
-    double* red = lines;  // NW x nx
+    double* red = lines;  // NL x nx: the lag sums of each (wavefront, half)
+    const int row = 2 * wv + half;
 #pragma unroll
-    for (int q = 0; q < K; ++q)
-        if (r0 + q < nx) red[wv * nx + r0 + q] = acc[q];
+    for (int q = 0; q < K; ++q) {
+        if (b1 * K + q < nx) red[row * nx + b1 * K + q] = acc1[q];
+        if (b2 * K + q < nx) red[row * nx + b2 * K + q] = acc2[q];
+    }
     __syncthreads();
     for (int r = threadIdx.x; r < nx; r += kDiagThreads) {
         double sum = red[r];
 #pragma unroll
-        for (int w = 1; w < NW; ++w) sum += red[w * nx + r];
+        for (int w = 1; w < NL; ++w) sum += red[w * nx + r];
         partial[(long long)blockIdx.x * nx + r] = sum;
     }
 }
@@ -196,8 +229,17 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_final(const double* __rest
                                                            double* __restrict__ out) {
     const int r = blockIdx.x * kDiagThreads + threadIdx.x;
     if (r >= nx) return;
+    // groups of 8 partials loaded together, added in block order
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[(long long)b * nx + r];
+    int b = 0;
+    for (; b + 8 <= nb; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = partial[(long long)(b + q) * nx + r];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += v[q];
+    }
+    for (; b < nb; ++b) s += partial[(long long)b * nx + r];
     out[r] = s;
 }
 
@@ -211,7 +253,7 @@ hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const doub
 
 template <int K>
 static void launch_vcf_lags(const double* vx, int nx, long long nlines, double* partial, int nblocks, hipStream_t s) {
-    const size_t lds = (size_t)(kDiagThreads / 64) * vcf_line_len<K>(nx) * sizeof(double);
+    const size_t lds = (size_t)(2 * (kDiagThreads / 64)) * vcf_line_len<K>(nx) * sizeof(double);
     hipLaunchKernelGGL((k_vcf_lags<K>), dim3(nblocks), dim3(kDiagThreads), lds, s, vx, nx, nlines, partial);
 }
 
