@@ -37,29 +37,51 @@ __device__ __forceinline__ bool above_edge(double x, int i, double vmax, int nbi
 // interleaved copies of the bins (lane % ncopy; each copy padded by one word
 // so equal bins of different copies fall in different LDS banks): the
 // values pile up in a few central bins, and atomics of one wavefront to one
-// LDS word are serialised.
+// LDS word are serialised.  EDGES: the block first tabulates the nbins + 1
+// edges of each component in LDS with pdf_edge, so a comparison with an edge
+// is one LDS read (the same edges, so the same bins); else above_edge.
 constexpr int kHistMaxCopies = 8;
 constexpr int kHistUnroll = 4;
 constexpr int kHistLdsBytes = 64 * 1024;
+constexpr int kHistEdgeBytes = 32 * 1024;  // the edge table is used up to this size
 inline int hist_copies(int nbins) {
     const int c = kHistLdsBytes / ((3 * nbins + 1) * (int)sizeof(unsigned int));
     return c < 1 ? 1 : (c > kHistMaxCopies ? kHistMaxCopies : c);
 }
+inline size_t hist_edge_bytes(int nbins) { return (size_t)3 * (nbins + 1) * sizeof(double); }
+// 32-bit words of bins, rounded up to an even count so the edges that follow
+// are 8-byte aligned
+inline int hist_bin_words(int nbins, int ncopy) { return (ncopy * (3 * nbins + 1) + 1) & ~1; }
+
+template <bool EDGES>
 __global__ __launch_bounds__(kDiagThreads) void k_vel_hist(const double* __restrict__ vel, long long n, int nbins,
                                                           int ncopy, const double* __restrict__ vmax3,
                                                           unsigned long long* __restrict__ hist) {
-    extern __shared__ unsigned int hsh[];  // ncopy * (3 * nbins + 1)
+    extern __shared__ unsigned int hsh[];  // ncopy * (3 * nbins + 1), then (EDGES) 3 * (nbins + 1) doubles
     const int stride = 3 * nbins + 1;
     for (int j = threadIdx.x; j < ncopy * stride; j += kDiagThreads) hsh[j] = 0u;
+    const int lo = -nbins / 2 + 1, hi = nbins / 2;
+    double* edges = reinterpret_cast<double*>(hsh + ((ncopy * stride + 1) & ~1));
+    if constexpr (EDGES) {
+        // edges[a * (nbins + 1) + (i - lo + 1)] = pdf_edge(i) for i = lo - 1 .. hi
+        for (int j = threadIdx.x; j < 3 * (nbins + 1); j += kDiagThreads) {
+            const int a = j / (nbins + 1), i = j - a * (nbins + 1) + lo - 1;
+            edges[j] = pdf_edge(i, vmax3[a], nbins);
+        }
+    }
     __syncthreads();
     unsigned int* mine = hsh + (threadIdx.x % ncopy) * stride;
-    const int lo = -nbins / 2 + 1, hi = nbins / 2;
     const double inv_nb = 1.0 / (double)nbins;
     for (int a = 0; a < 3; ++a) {
         const double* v = vel + a * n;
         const double vmax = vmax3[a];
         if (!(vmax > 0.0)) continue;  // uniform
         const double scale = (double)nbins / (2.0 * vmax);
+        const double* E = edges + a * (nbins + 1) - (lo - 1);  // E[i] = pdf_edge(i), i = lo - 1 .. hi
+        auto above = [&](double x, int i) {
+            if constexpr (EDGES) return x > E[i];
+            else return above_edge(x, i, vmax, nbins, inv_nb);
+        };
         // kHistUnroll values of the grid-stride sequence are loaded together,
         // so their loads are in flight at once, then binned one by one
         const long long stride = (long long)gridDim.x * kDiagThreads;
@@ -79,8 +101,8 @@ __global__ __launch_bounds__(kDiagThreads) void k_vel_hist(const double* __restr
                 double q = ceil(x * scale);
                 q = q < lo - 1 ? lo - 1 : (q > hi + 1 ? hi + 1 : q);
                 int b = (int)q;
-                while (b <= hi && above_edge(x, b, vmax, nbins, inv_nb)) ++b;
-                while (b >= lo && !above_edge(x, b - 1, vmax, nbins, inv_nb)) --b;
+                while (b <= hi && above(x, b)) ++b;
+                while (b >= lo && !above(x, b - 1)) --b;
                 if (b >= lo && b <= hi) atomicAdd(&mine[a * nbins + (b - lo)], 1u);
             }
         }
@@ -246,8 +268,13 @@ __global__ __launch_bounds__(kDiagThreads) void k_vcf_final(const double* __rest
 hipError_t launch_vel_hist(const double* vel, long long n, int nbins, const double* vmax3, unsigned long long* hist,
                            int nblocks, hipStream_t s) {
     const int ncopy = hist_copies(nbins);
-    hipLaunchKernelGGL(k_vel_hist, dim3(nblocks), dim3(kDiagThreads),
-                       (size_t)ncopy * (3 * nbins + 1) * sizeof(unsigned int), s, vel, n, nbins, ncopy, vmax3, hist);
+    const size_t bins = (size_t)hist_bin_words(nbins, ncopy) * sizeof(unsigned int);
+    if (hist_edge_bytes(nbins) <= (size_t)kHistEdgeBytes)
+        hipLaunchKernelGGL(k_vel_hist<true>, dim3(nblocks), dim3(kDiagThreads), bins + hist_edge_bytes(nbins), s, vel,
+                           n, nbins, ncopy, vmax3, hist);
+    else
+        hipLaunchKernelGGL(k_vel_hist<false>, dim3(nblocks), dim3(kDiagThreads), bins, s, vel, n, nbins, ncopy,
+                           vmax3, hist);
     return hipGetLastError();
 }
 

@@ -151,6 +151,24 @@ def test_gpu_diagnostics_c64_and_larger_grid(orc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nbins", [2, 1364, 1366, 4096])
+def test_gpu_velocity_pdf_bin_counts(nbins):
+    """The histogram either side of nbins 1364, the largest count whose bin
+    edges the kernel tabulates in LDS (above it each comparison forms its
+    edge), up to the largest count accepted, and nbins 2 (one bin each side
+    of zero): against the oracle's get_pdf."""
+    from oracle import diag_oracle as dg
+
+    p = RunParams(nx=40, ny=36, nz=32, xr=0.3 * 39 / 2, yr=0.3 * 35 / 2, zr=0.3 * 31 / 2, order=4, bcs=1)
+    psi = ics.random_phase(p, seed=7)
+    exp = dg.velocity_pdf(p, psi, nbins)
+    with Context(p) as ctx:
+        ctx.upload(psi)
+        st = ctx.velocity_pdf(nbins)
+    check_stats(st, exp, nbins)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("nx", [40, 100, 256, 300, 512, 1000, 1500])
 def test_gpu_vcf_line_lengths(nx):
     """The velocity correlation over every lag-block width of the kernel
