@@ -30,9 +30,8 @@ __device__ __forceinline__ long long mode_sq(int i, int n) {
 // tab[k2] for k2 = 0 .. max: the F bin s (the smallest kk with
 // sqrt(k2) <= kk; 0 when k2 = 0 or s > nhalf) and, in bits 24..31, the log2
 // shell m with |log2(sqrt(k2) / (2 pi))| in [m, m + 1) (kSpectrumShells when
-// none).  Every mode of one k2 shares them, so the sqrt, the search and the
-// log of each mode become one table read; the arithmetic is the per-mode
-// arithmetic it replaces.
+// none), with the arithmetic of the spectrum's per-mode sqrt, search and
+// log.  k_spectrum_partial reads the shells from it.
 __global__ __launch_bounds__(kSpThreads) void k_spectrum_table(long long len, int nhalf, unsigned* __restrict__ tab) {
     const long long k2 = (long long)blockIdx.x * kSpThreads + threadIdx.x;
     if (k2 >= len) return;
@@ -52,14 +51,45 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_table(long long len, in
 // Per block b, rows of partial[row * nb + b]: 0..6 the shell sums of |a|^2,
 // 7..13 the shell counts, 14 + (s - 1) for s = 1..nhalf the sum of |a|^2 over
 // the modes whose smallest kk with k2 <= kk^2 is s.
+//
+// A mode's s comes from a float square root settled with integer compares
+// (exact), and its shell from a per-block LDS table over s built from tab,
+// so no mode reads tab[k2] (a scattered read per mode).  The k2 of one s
+// fill ((s-1)^2, s^2]; the shell boundaries sqrt(k2) = 2 pi 2^m (m = -3 .. 7) are more than one
+// apart above 1.6 and the range of s = 1 and 2 holds at most one of them, so
+// each range holds at most one change of shell: the table keeps the shells
+// at its two ends and the first k2 of the second, found by bisection of tab.
+// Entry s: bits 0..31 that k2 (past the range: none), 32..39 and 40..47 the
+// shells below and from it.
 template <typename R>
 __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __restrict__ a, int nx, int ny, int nz,
-                                                                 int nhalf, double inv_n,
+                                                                 int nhalf, double inv_n, long long len, int smax,
                                                                  const unsigned* __restrict__ tab,
                                                                  double* __restrict__ partial) {
-    extern __shared__ double hsh[];  // nhalf bins, then kSpThreads / 64 reduction slots
+    extern __shared__ double hsh[];  // nhalf bins, kSpThreads / 64 reduction slots, then smax + 1 entries
     double* red = hsh + nhalf;
+    unsigned long long* stab = reinterpret_cast<unsigned long long*>(red + kSpThreads / 64);
     for (int s = threadIdx.x; s < nhalf; s += kSpThreads) hsh[s] = 0.0;
+    for (int s = threadIdx.x; s <= smax; s += kSpThreads) {
+        const long long lo = (long long)(s - 1) * (s - 1) + 1;
+        const long long hi = (long long)s * s < len - 1 ? (long long)s * s : len - 1;
+        unsigned long long e = 0;
+        if (s >= 1 && lo <= hi) {
+            const unsigned sa = tab[lo] >> 24, sb = tab[hi] >> 24;
+            long long thr = hi + 1;
+            if (sa != sb) {  // tab[l] has shell sa, tab[h] not
+                long long l = lo, h = hi;
+                while (h - l > 1) {
+                    const long long mid = l + (h - l) / 2;
+                    if ((tab[mid] >> 24) == sa) l = mid;
+                    else h = mid;
+                }
+                thr = h;
+            }
+            e = (unsigned long long)(unsigned)thr | ((unsigned long long)sa << 32) | ((unsigned long long)sb << 40);
+        }
+        stab[s] = e;
+    }
     __syncthreads();
     double e[kSpectrumShells], cnt[kSpectrumShells];
 #pragma unroll
@@ -76,8 +106,14 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
         if (k2 == 0) continue;
         const C<R> v = ar[i];
         const double p2 = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * inv_n;
-        const unsigned t = tab[k2];
-        const unsigned sbin = t & 0xFFFFFFu, sh = t >> 24;
+        // the float root is within a few units of sqrt(k2) (exactly rounded
+        // below 2^24); the compares settle the smallest sq with sq^2 >= k2
+        int sq = (int)sqrtf((float)k2);
+        while ((long long)sq * sq < k2) ++sq;
+        while (sq > 1 && (long long)(sq - 1) * (sq - 1) >= k2) --sq;
+        const unsigned long long te = stab[sq];
+        const unsigned sbin = sq <= nhalf ? (unsigned)sq : 0u;
+        const unsigned sh = (unsigned)k2 < (unsigned)te ? (unsigned)(te >> 32) & 0xFFu : (unsigned)(te >> 40) & 0xFFu;
         if (sbin) atomicAdd(&hsh[sbin - 1], p2);
 #pragma unroll
         for (int m = 0; m < kSpectrumShells; ++m) {
@@ -167,8 +203,14 @@ hipError_t launch_spectrum(const C<R>* a, int nx, int ny, int nz, int nhalf, dou
     const long long len = spectrum_table_len(nx, ny, nz);
     hipLaunchKernelGGL(k_spectrum_table, dim3((unsigned)((len + kSpThreads - 1) / kSpThreads)), dim3(kSpThreads), 0, s,
                        len, nhalf, tab);
-    hipLaunchKernelGGL((k_spectrum_partial<R>), dim3(nb), dim3(kSpThreads), lds, s, a, nx, ny, nz, nhalf, inv_n,
-                       (const unsigned*)tab, partial);
+    // the s table: s = 1 .. smax, smax^2 >= the largest k2 (len - 1)
+    int smax = 1;
+    while ((long long)smax * smax < len - 1) ++smax;
+    // entries hold k2 in 32 bits; 64 KiB of entries reach past 9000^3 grids
+    const size_t stab = (size_t)(smax + 1) * sizeof(unsigned long long);
+    if (len >= (1LL << 32) || stab > 64 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_spectrum_partial<R>), dim3(nb), dim3(kSpThreads), lds + stab, s, a, nx, ny, nz, nhalf, inv_n,
+                       len, smax, (const unsigned*)tab, partial);
     const int nrows = 2 * kSpectrumShells + nhalf;
     hipLaunchKernelGGL(k_sum_rows, dim3((nrows + kSpThreads - 1) / kSpThreads), dim3(kSpThreads), 0, s, partial, nrows,
                        nb, out);
