@@ -91,9 +91,13 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
         stab[s] = e;
     }
     __syncthreads();
-    double e[kSpectrumShells], cnt[kSpectrumShells];
+    double e[kSpectrumShells];
+    unsigned cnt[kSpectrumShells];  // exact counts (a thread sees under 2^32 modes), summed as doubles per block
 #pragma unroll
-    for (int m = 0; m < kSpectrumShells; ++m) e[m] = cnt[m] = 0.0;
+    for (int m = 0; m < kSpectrumShells; ++m) {
+        e[m] = 0.0;
+        cnt[m] = 0u;
+    }
     // rows (j, k) block by block, x across the threads: the row's indices
     // are formed once per row, not divided out of a linear index per mode
     const long long nrow = (long long)ny * nz;
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
         for (int m = 0; m < kSpectrumShells; ++m) {
             if (sh == (unsigned)m) {
                 e[m] += p2;
-                cnt[m] += 1.0;
+                cnt[m] += 1u;
             }
         }
       }
@@ -128,7 +132,7 @@ __global__ __launch_bounds__(kSpThreads) void k_spectrum_partial(const C<R>* __r
 #pragma unroll
     for (int m = 0; m < kSpectrumShells; ++m) {
         const double se = block_reduce<kSpThreads, RED_SUM>(e[m], red);
-        const double sc = block_reduce<kSpThreads, RED_SUM>(cnt[m], red);
+        const double sc = block_reduce<kSpThreads, RED_SUM>((double)cnt[m], red);
         if (threadIdx.x == 0) {
             partial[(long long)m * nb + blockIdx.x] = se;
             partial[(long long)(kSpectrumShells + m) * nb + blockIdx.x] = sc;
